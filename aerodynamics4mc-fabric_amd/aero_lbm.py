@@ -1,0 +1,279 @@
+"""ctypes binding of libaero_lbm.so (the MI355X build of the reference's C ABI).
+
+This is the Python host mirror of the reference's native interface, the same
+way the reference's own tools bind it (native/tools/benchmark_solver_dll.py,
+native/tools/check_d3q27_fp16.py): the struct layouts follow
+native/include/aero_lbm_capi.h:76-164 and every function keeps the reference's
+name, argument meaning and 1/0 return convention.
+
+Loading fails loudly: there is no CPU fallback. If the shared library has not
+been built (``python -c "import __graft_entry__ as g; g.build()"``) or cannot
+find a GPU, calls raise ``AeroLbmError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_char_p, c_float, c_int, c_longlong,
+                    c_uint8, c_uint16, c_int16, c_uint32, c_uint64, c_double, c_void_p)
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libaero_lbm.so")
+
+# ---- enums (native/include/aero_lbm_capi.h:16-74) ----
+SOLVER_AUTO = 0
+SOLVER_CLASSIC_D3Q27 = 1
+SOLVER_COMPACT_EXPERIMENTAL = 2
+SOLVER_D3Q27_FP16_INPLACE = 3
+SOLVER_D3Q27_BF16_INPLACE = 4  # extension
+
+PRESET_NONE = 0
+PRESET_TAYLOR_GREEN_3D = 1
+
+HYDRO_INHERIT, HYDRO_PERIODIC, HYDRO_BOUNCE_BACK, HYDRO_MOVING_WALL = 0, 1, 2, 3
+HYDRO_VELOCITY, HYDRO_PRESSURE, HYDRO_CONVECTIVE, HYDRO_SYMMETRY = 4, 5, 6, 7
+THERMAL_INHERIT, THERMAL_DISABLED, THERMAL_PERIODIC, THERMAL_ADIABATIC = 0, 1, 2, 3
+THERMAL_DIRICHLET, THERMAL_NEUMANN = 4, 5
+
+FLAG_DISABLE_FAN_FORCING = 1 << 0
+FLAG_DISABLE_FAN_NOISE = 1 << 1
+FLAG_DISABLE_SPONGE = 1 << 2
+FLAG_DISABLE_CONVECTIVE_OUTFLOW = 1 << 3
+FLAG_DISABLE_OBSTACLE_BOUNCE_BLEND = 1 << 4
+FLAG_DISABLE_SGS = 1 << 5
+FLAG_DISABLE_INTERNAL_THERMAL_SOURCE = 1 << 6
+FLAG_DISABLE_BUOYANCY = 1 << 7
+
+KERNEL_FAST = 0
+KERNEL_EXACT = 1
+
+
+class BoundaryFace(Structure):
+    _fields_ = [("hydrodynamic_kind", c_int), ("thermal_kind", c_int),
+                ("velocity", c_float * 3), ("pressure", c_float),
+                ("temperature", c_float), ("heat_flux", c_float)]
+
+
+class BenchmarkConfig(Structure):
+    _fields_ = [("abi_version", c_uint32), ("struct_size", c_uint32), ("flags", c_uint32),
+                ("enabled", c_int), ("preset", c_int), ("reserved0", c_int),
+                ("reynolds_number", c_float), ("rayleigh_number", c_float),
+                ("prandtl_number", c_float), ("mach_number", c_float),
+                ("reference_density", c_float), ("reference_temperature", c_float),
+                ("reference_length", c_float), ("body_force", c_float * 3),
+                ("gravity", c_float * 3), ("initial_velocity", c_float * 3),
+                ("initial_pressure", c_float), ("initial_temperature", c_float),
+                ("x_min", BoundaryFace), ("x_max", BoundaryFace),
+                ("y_min", BoundaryFace), ("y_max", BoundaryFace),
+                ("z_min", BoundaryFace), ("z_max", BoundaryFace),
+                ("reserved", c_uint32 * 8)]
+
+
+class SolverOverrides(Structure):
+    _fields_ = [("struct_size", c_uint32), ("sgs_constant", c_float),
+                ("sgs_nut_cap_ratio", c_float), ("boussinesq_beta", c_float),
+                ("thermal_diffusivity", c_float), ("gravity_axis", c_int),
+                ("gravity_sign", c_int), ("kernel_variant", c_int), ("reserved", c_int * 8)]
+
+
+class TimingSnapshot(Structure):
+    _fields_ = [("ticks", c_uint64)] + [(n, c_double) for n in (
+        "last_payload_copy_ms", "last_solver_ms", "last_readback_ms", "last_total_ms",
+        "avg_payload_copy_ms", "avg_solver_ms", "avg_readback_ms", "avg_total_ms")]
+
+
+class MesoscaleConfig(Structure):
+    _fields_ = [("abi_version", c_uint32), ("struct_size", c_uint32),
+                ("nx", c_int), ("ny", c_int), ("nz", c_int),
+                ("dx_m", c_float), ("dt_s", c_float), ("molecular_nu_m2_s", c_float),
+                ("prandtl_air", c_float), ("turbulent_prandtl", c_float),
+                ("reference_density_kg_m3", c_float), ("ambient_air_temperature_k", c_float),
+                ("deep_ground_temperature_k", c_float), ("background_wind_m_s", c_float * 3),
+                ("reserved", c_uint32 * 8)]
+
+
+class MesoscaleTransport(Structure):
+    _fields_ = [("velocity_scale_m_s_per_lattice", c_float), ("nu_molecular_lattice", c_float),
+                ("alpha_molecular_lattice", c_float), ("tau_shear_molecular", c_float),
+                ("tau_thermal_molecular", c_float)]
+
+
+assert ctypes.sizeof(BoundaryFace) == 32
+assert ctypes.sizeof(BenchmarkConfig) == 320
+assert ctypes.sizeof(TimingSnapshot) == 72
+assert ctypes.sizeof(MesoscaleConfig) == 96
+assert ctypes.sizeof(MesoscaleTransport) == 20
+
+
+class AeroLbmError(RuntimeError):
+    pass
+
+
+_F = POINTER(c_float)
+_I = POINTER(c_int)
+
+# name -> (restype, argtypes); mirrors include/aero_lbm_capi.h
+PROTOTYPES = {
+    "aero_lbm_init": (c_int, [c_int, c_int, c_int]),
+    "aero_lbm_step": (c_int, [_F, c_int, c_longlong, _F]),
+    "aero_lbm_init_rect": (c_int, [c_int, c_int, c_int, c_int, c_int]),
+    "aero_lbm_step_rect": (c_int, [_F, c_int, c_int, c_int, c_longlong, _F]),
+    "aero_lbm_step_rect_scaled": (c_int, [_F, c_int, c_int, c_int, c_longlong, c_float, _F]),
+    "aero_lbm_step_rect_cached": (c_int, [c_int, c_int, c_int, c_longlong, _F]),
+    "aero_lbm_step_rect_cached_scaled": (c_int, [c_int, c_int, c_int, c_longlong, c_float, _F]),
+    "aero_lbm_has_context": (c_int, [c_longlong]),
+    "aero_lbm_context_realtime_cached_initialized": (c_int, [c_longlong]),
+    "aero_lbm_get_temperature_state_rect": (c_int, [c_int, c_int, c_int, c_longlong, _F]),
+    "aero_lbm_sample_temperature_point_rect": (c_int, [c_int, c_int, c_int, c_longlong, c_int, c_int, c_int, _F]),
+    "aero_lbm_sample_flow_point_rect": (c_int, [c_int, c_int, c_int, c_longlong, c_int, c_int, c_int, _F]),
+    "aero_lbm_extract_flow_atlas_rect": (c_int, [c_int, c_int, c_int, c_longlong, c_int, _F, c_int]),
+    "aero_lbm_copy_flow_temperature_subrect": (c_int, [c_int, c_int, c_int, c_longlong, c_int, c_int, c_int,
+                                                       c_int, c_int, c_int, _F, _F]),
+    "aero_lbm_get_flow_state_rect": (c_int, [c_int, c_int, c_int, c_longlong, _F]),
+    "aero_lbm_set_temperature_state_rect": (c_int, [c_int, c_int, c_int, c_longlong, _F]),
+    "aero_lbm_get_last_force": (c_int, [c_longlong, _F, _F, _F]),
+    "aero_lbm_release_context": (None, [c_longlong]),
+    "aero_lbm_shutdown": (None, []),
+    "aero_lbm_runtime_info": (c_char_p, []),
+    "aero_lbm_last_error": (c_char_p, []),
+    "aero_lbm_timing_info": (c_char_p, []),
+    "aero_lbm_memory_info": (c_char_p, []),
+    "aero_lbm_reset_timing": (None, []),
+    "aero_lbm_set_realtime_solver_mode": (None, [c_int]),
+    "aero_lbm_finish": (c_int, []),
+    "aero_lbm_get_timing_snapshot": (c_int, [POINTER(TimingSnapshot)]),
+    "aero_lbm_benchmark_default_config": (None, [POINTER(BenchmarkConfig)]),
+    "aero_lbm_benchmark_default_preset_config": (c_int, [c_int, POINTER(BenchmarkConfig)]),
+    "aero_lbm_benchmark_set_config": (c_int, [POINTER(BenchmarkConfig)]),
+    "aero_lbm_benchmark_get_config": (c_int, [POINTER(BenchmarkConfig)]),
+    "aero_lbm_benchmark_reset_config": (None, []),
+    "aero_lbm_benchmark_info": (c_char_p, []),
+    "aero_lbm_default_solver_overrides": (None, [POINTER(SolverOverrides)]),
+    "aero_lbm_set_solver_overrides": (c_int, [POINTER(SolverOverrides)]),
+    "aero_lbm_get_solver_overrides": (c_int, [POINTER(SolverOverrides)]),
+    "aero_lbm_run_cached_steps_timed": (c_int, [c_int, c_int, c_int, c_longlong, c_int, _F]),
+}
+
+
+def header_symbols(header_path: str) -> list[str]:
+    """Names of every function declared with AERO_LBM_CAPI_EXPORT / JNIEXPORT in a header."""
+    import re
+    text = open(header_path).read()
+    return sorted(set(re.findall(r"AERO_LBM_CAPI_EXPORT\s+[\w\s\*]+?\b(\w+)\s*\(", text)))
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libaero_lbm.so and attach prototypes. Raises AeroLbmError when absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise AeroLbmError(f"libaero_lbm.so not built at {path}; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_F)
+
+
+def last_error() -> str:
+    msg = load().aero_lbm_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(ok: int, what: str):
+    if not ok:
+        raise AeroLbmError(f"{what} failed: {last_error()}")
+
+
+def default_config() -> BenchmarkConfig:
+    cfg = BenchmarkConfig()
+    load().aero_lbm_benchmark_default_config(ctypes.byref(cfg))
+    return cfg
+
+
+def default_overrides() -> SolverOverrides:
+    ov = SolverOverrides()
+    load().aero_lbm_default_solver_overrides(ctypes.byref(ov))
+    return ov
+
+
+class Context:
+    """One solver context (a brick) of shape (nx, ny, nz), keyed like the reference."""
+
+    def __init__(self, nx: int, ny: int, nz: int, key: int = 1, in_ch: int = 11, out_ch: int = 4):
+        self.lib = load()
+        self.nx, self.ny, self.nz, self.key = nx, ny, nz, key
+        self.cells = nx * ny * nz
+        self.in_ch, self.out_ch = in_ch, out_ch
+
+    def step(self, packet: np.ndarray | None, out: np.ndarray | None = None, scale: float | None = None) -> None:
+        outp = fptr(out) if out is not None else None
+        if packet is None:
+            if scale is None:
+                ok = self.lib.aero_lbm_step_rect_cached(self.nx, self.ny, self.nz, self.key, outp)
+            else:
+                ok = self.lib.aero_lbm_step_rect_cached_scaled(self.nx, self.ny, self.nz, self.key, scale, outp)
+        else:
+            packet = np.ascontiguousarray(packet, dtype=np.float32)
+            if scale is None:
+                ok = self.lib.aero_lbm_step_rect(fptr(packet), self.nx, self.ny, self.nz, self.key, outp)
+            else:
+                ok = self.lib.aero_lbm_step_rect_scaled(fptr(packet), self.nx, self.ny, self.nz, self.key,
+                                                        scale, outp)
+        check(ok, "step")
+
+    def flow(self) -> np.ndarray:
+        out = np.zeros(self.cells * 4, dtype=np.float32)
+        check(self.lib.aero_lbm_get_flow_state_rect(self.nx, self.ny, self.nz, self.key, fptr(out)),
+              "get_flow_state_rect")
+        return out
+
+    def temperature(self) -> np.ndarray:
+        out = np.zeros(self.cells, dtype=np.float32)
+        check(self.lib.aero_lbm_get_temperature_state_rect(self.nx, self.ny, self.nz, self.key, fptr(out)),
+              "get_temperature_state_rect")
+        return out
+
+    def set_temperature(self, t: np.ndarray) -> None:
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        check(self.lib.aero_lbm_set_temperature_state_rect(self.nx, self.ny, self.nz, self.key, fptr(t)),
+              "set_temperature_state_rect")
+
+    def atlas(self, stride: int) -> np.ndarray:
+        sx = -(-self.nx // stride)
+        sy = -(-self.ny // stride)
+        sz = -(-self.nz // stride)
+        out = np.zeros(sx * sy * sz * 4, dtype=np.float32)
+        check(self.lib.aero_lbm_extract_flow_atlas_rect(self.nx, self.ny, self.nz, self.key, stride,
+                                                        fptr(out), out.size), "extract_flow_atlas_rect")
+        return out
+
+    def sample(self, x: int, y: int, z: int) -> np.ndarray:
+        out = np.zeros(4, dtype=np.float32)
+        check(self.lib.aero_lbm_sample_flow_point_rect(self.nx, self.ny, self.nz, self.key, x, y, z, fptr(out)),
+              "sample_flow_point_rect")
+        return out
+
+    def run_timed(self, steps: int) -> float:
+        ms = c_float(0.0)
+        check(self.lib.aero_lbm_run_cached_steps_timed(self.nx, self.ny, self.nz, self.key, steps,
+                                                       ctypes.byref(ms)), "run_cached_steps_timed")
+        return float(ms.value)
+
+    def release(self) -> None:
+        self.lib.aero_lbm_release_context(self.key)

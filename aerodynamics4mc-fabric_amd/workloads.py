@@ -1,0 +1,193 @@
+"""Synthetic workloads for the local-airflow solver (seeded, deterministic).
+
+Every generator uses SplitMix64 (the mixing constants the reference uses for
+its fan noise hash, native/src/aero_lbm_jni.cpp:1318-1327) so inputs are
+reproducible across machines. Payload packets follow the reference's 11-channel
+per-cell layout (native/src/aero_lbm_jni.cpp:64-74): [0] obstacle (>0.5 solid),
+[1] fan mask, [2-4] fan velocity, [5-7] reference velocity (lattice),
+[8] reference pressure, [9] thermal source, [10] reference temperature. Cell
+order is x-major, z fastest: cell = (x*ny + y)*nz + z.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import aero_lbm as A  # the package directory is on sys.path (see __graft_entry__.PKG_DIR)
+
+_M64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """Sequential SplitMix64 generator."""
+
+    def __init__(self, seed: int):
+        self.state = seed & _M64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        """Uniform in [0, 1) with 53-bit resolution."""
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi] inclusive."""
+        return lo + int(self.uniform() * (hi - lo + 1))
+
+
+def splitmix_array(seed: int, n: int) -> np.ndarray:
+    """n uniforms in [0,1) from SplitMix64 stream `seed` (vectorised, counter-based)."""
+    idx = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def solid_boxes(nx: int, ny: int, nz: int, fill: float, seed: int, edge=(2, 8),
+                keep_clear_x: int = 2) -> np.ndarray:
+    """Random axis-aligned solid boxes (edges uniform in `edge`) placed until the
+    solid fraction reaches `fill`. Planes x < keep_clear_x and x >= nx-keep_clear_x
+    stay fluid so inflow/outflow faces are open. Returns uint8 [nx,ny,nz]."""
+    rng = SplitMix64(seed)
+    solid = np.zeros((nx, ny, nz), dtype=np.uint8)
+    target = fill * nx * ny * nz
+    guard = 0
+    while solid.sum() < target and guard < 100000:
+        guard += 1
+        ex, ey, ez = (rng.randint(edge[0], edge[1]) for _ in range(3))
+        x0 = rng.randint(keep_clear_x, max(keep_clear_x, nx - keep_clear_x - ex))
+        y0 = rng.randint(0, max(0, ny - ey))
+        z0 = rng.randint(0, max(0, nz - ez))
+        solid[x0:min(x0 + ex, nx - keep_clear_x), y0:y0 + ey, z0:z0 + ez] = 1
+    return solid
+
+
+def make_packet(solid: np.ndarray, u0=(0.05, 0.0, 0.0), perturb: float = 1e-3, seed_u: int = 1,
+                temperature: np.ndarray | None = None, source: np.ndarray | None = None,
+                channels: int = 11) -> np.ndarray:
+    """Payload packet [cells*channels] float32 with reference velocity u0 + U(-perturb, perturb)."""
+    nx, ny, nz = solid.shape
+    cells = nx * ny * nz
+    p = np.zeros((cells, channels), dtype=np.float32)
+    p[:, 0] = solid.reshape(-1).astype(np.float32)
+    r = splitmix_array(seed_u, 3 * cells).reshape(cells, 3)
+    for k in range(3):
+        p[:, 5 + k] = (u0[k] + perturb * (2.0 * r[:, k] - 1.0)).astype(np.float32)
+    if temperature is not None and channels > 10:
+        p[:, 10] = temperature.reshape(-1).astype(np.float32)
+    if source is not None and channels > 9:
+        p[:, 9] = source.reshape(-1).astype(np.float32)
+    return p.reshape(-1)
+
+
+def hot_blob(nx: int, ny: int, nz: int, amplitude: float = 0.02, sigma: float = 8.0, seed: int = 4,
+             axis: int = 2) -> np.ndarray:
+    """Gaussian temperature blob centred on the floor (coordinate 0 of `axis`), with a
+    seeded jitter of the horizontal centre by up to +-2 cells."""
+    rng = SplitMix64(seed)
+    c = [nx / 2.0, ny / 2.0, nz / 2.0]
+    for k in range(3):
+        if k != axis:
+            c[k] += 4.0 * rng.uniform() - 2.0
+    c[axis] = 0.0
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+    return (amplitude * np.exp(-r2 / (2.0 * sigma * sigma))).astype(np.float32)
+
+
+def channel_config(u_in: float = 0.05, reynolds: float = 50.0, sgs: bool = True, buoyancy: bool = False,
+                   periodic_yz: bool = True, thermal_periodic: bool = True) -> "A.BenchmarkConfig":
+    """Benchmark-mode wind-tunnel configuration: velocity inlet at x-, convective
+    outflow at x+, periodic y/z (reference face kinds, aero_lbm_capi.h:45-54).
+    Reynolds 50 with reference length 1 gives nu = u_in / 50 (1e-3 at 0.05)."""
+    cfg = A.BenchmarkConfig()
+    cfg.abi_version = 1
+    cfg.struct_size = 320
+    cfg.flags = (A.FLAG_DISABLE_FAN_FORCING | A.FLAG_DISABLE_FAN_NOISE | A.FLAG_DISABLE_SPONGE
+                 | A.FLAG_DISABLE_OBSTACLE_BOUNCE_BLEND)
+    if not sgs:
+        cfg.flags |= A.FLAG_DISABLE_SGS
+    if not buoyancy:
+        cfg.flags |= A.FLAG_DISABLE_BUOYANCY
+    cfg.enabled = 1
+    cfg.preset = A.PRESET_NONE
+    cfg.reynolds_number = reynolds
+    cfg.rayleigh_number = 1.0e5
+    cfg.prandtl_number = 0.71
+    cfg.mach_number = 0.05
+    cfg.reference_density = 1.0
+    cfg.reference_length = 1.0
+    cfg.gravity[1] = -1.0
+    cfg.initial_velocity[0] = u_in
+    tk = A.THERMAL_PERIODIC if thermal_periodic else A.THERMAL_ADIABATIC
+    cfg.x_min.hydrodynamic_kind = A.HYDRO_VELOCITY
+    cfg.x_min.velocity[0] = u_in
+    cfg.x_min.thermal_kind = A.THERMAL_DIRICHLET
+    cfg.x_max.hydrodynamic_kind = A.HYDRO_CONVECTIVE
+    cfg.x_max.thermal_kind = A.THERMAL_ADIABATIC
+    for f in (cfg.y_min, cfg.y_max, cfg.z_min, cfg.z_max):
+        f.hydrodynamic_kind = A.HYDRO_PERIODIC if periodic_yz else A.HYDRO_SYMMETRY
+        f.thermal_kind = tk if periodic_yz else A.THERMAL_ADIABATIC
+    return cfg
+
+
+def config1(n: int = 32):
+    """Plain D3Q27 cumulant LBM, one n^3 brick: 15% solid boxes (seed 2), u=(0.05,0,0)
+    +-1e-3 (seed 1), nu 1e-3, SGS on (reference C), buoyancy off."""
+    solid = solid_boxes(n, n, n, 0.15, seed=2)
+    packet = make_packet(solid, seed_u=1)
+    cfg = channel_config(0.05, 50.0, sgs=True, buoyancy=False)
+    return cfg, packet, solid
+
+
+def config3(n: int = 128, beta: float = 0.1, cs: float = 0.1):
+    """D3Q27 cumulant + SGS + Boussinesq, one n^3 brick: ~10% solid boxes (seed 5),
+    inflow 0.05, Gaussian hot blob dT=0.02 sigma=8 at the z=0 floor (seed 4),
+    gravity -z with beta; returns (cfg, overrides, packet, temperature, solid)."""
+    solid = solid_boxes(n, n, n, 0.10, seed=5)
+    t = hot_blob(n, n, n, 0.02, 8.0, seed=4, axis=2)
+    t[solid.astype(bool)] = 0.0
+    packet = make_packet(solid, seed_u=1, temperature=t)
+    cfg = channel_config(0.05, 50.0, sgs=True, buoyancy=True)
+    # z is the gravity axis: keep the floor/ceiling as walls for the thermal field
+    cfg.z_min.hydrodynamic_kind = A.HYDRO_BOUNCE_BACK
+    cfg.z_max.hydrodynamic_kind = A.HYDRO_BOUNCE_BACK
+    cfg.z_min.thermal_kind = A.THERMAL_ADIABATIC
+    cfg.z_max.thermal_kind = A.THERMAL_ADIABATIC
+    ov = A.SolverOverrides()
+    ov.struct_size = ctypes_sizeof_overrides()
+    ov.sgs_constant = cs
+    ov.sgs_nut_cap_ratio = float("nan")
+    ov.boussinesq_beta = beta
+    ov.thermal_diffusivity = float("nan")
+    ov.gravity_axis = 2
+    ov.gravity_sign = -1
+    ov.kernel_variant = A.KERNEL_FAST
+    return cfg, ov, packet, t, solid
+
+
+def ctypes_sizeof_overrides() -> int:
+    import ctypes
+    return ctypes.sizeof(A.SolverOverrides)
+
+
+def reference_overrides(kernel_variant: int = 0):
+    """Overrides that keep every reference constant (NaN / -1 = keep)."""
+    ov = A.SolverOverrides()
+    ov.struct_size = ctypes_sizeof_overrides()
+    nan = float("nan")
+    ov.sgs_constant = nan
+    ov.sgs_nut_cap_ratio = nan
+    ov.boussinesq_beta = nan
+    ov.thermal_diffusivity = nan
+    ov.gravity_axis = -1
+    ov.gravity_sign = 0
+    ov.kernel_variant = kernel_variant
+    return ov

@@ -1,0 +1,281 @@
+/*
+ * aero_lbm C ABI -- MI355X (gfx950) implementation of the local-airflow solver.
+ *
+ * Drop-in boundary: every declaration below that carries a "ref:" citation has
+ * the same name, argument meaning, return convention (1 = ok, 0 = failure) and
+ * struct layout as the reference's native/include/aero_lbm_capi.h, so the
+ * reference's JNI layer, ctypes tools and Java callers bind unchanged.
+ * Declarations marked "extension" have no counterpart in the reference.
+ *
+ * Struct layouts are fixed by static_assert in the implementation
+ * (x86-64: AeroLbmBenchmarkConfig 320 B, AeroLbmBoundaryFaceConfig 32 B,
+ * AeroLbmTimingSnapshot 72 B, AeroLbmWorldDelta 48 B,
+ * AeroLbmMesoscaleConfig 96 B, AeroLbmMesoscaleTransport 20 B).
+ */
+#ifndef AERO_LBM_CAPI_H
+#define AERO_LBM_CAPI_H
+
+#include <stdint.h>
+
+#define AERO_LBM_CAPI_EXPORT __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ref: native/include/aero_lbm_capi.h:16-26 */
+enum {
+    AERO_LBM_BENCHMARK_ABI_VERSION = 1,
+    AERO_LBM_MESOSCALE_ABI_VERSION = 3,
+    AERO_LBM_SIMULATION_ABI_VERSION = 1,
+    AERO_LBM_MESOSCALE_FORCING_CHANNELS = 25,
+    AERO_LBM_MESOSCALE_STATE_CHANNELS = 6,
+    AERO_LBM_SIMULATION_FLOW_STATE_CHANNELS = 4,
+    AERO_LBM_SIMULATION_PACKED_ATLAS_CHANNELS = 4,
+    AERO_LBM_SIMULATION_PLAYER_PROBE_CHANNELS = 6,
+    AERO_LBM_SIMULATION_BRICK_RUNTIME_STATUS_FIELDS = 8
+};
+
+/* ref: native/include/aero_lbm_capi.h:28-33; value 4 is an extension
+ * (BF16 equilibrium-shifted in-place storage). */
+enum {
+    AERO_LBM_REALTIME_SOLVER_AUTO = 0,
+    AERO_LBM_REALTIME_SOLVER_CLASSIC_D3Q27 = 1,
+    AERO_LBM_REALTIME_SOLVER_COMPACT_EXPERIMENTAL = 2,
+    AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL = 3,
+    AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE = 4
+};
+
+/* ref: native/include/aero_lbm_capi.h:35-43 */
+typedef enum AeroLbmBenchmarkPreset {
+    AERO_LBM_BENCHMARK_PRESET_NONE = 0,
+    AERO_LBM_BENCHMARK_PRESET_TAYLOR_GREEN_3D = 1,
+    AERO_LBM_BENCHMARK_PRESET_LID_DRIVEN_CAVITY_2D = 2,
+    AERO_LBM_BENCHMARK_PRESET_LID_DRIVEN_CAVITY_3D = 3,
+    AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D = 4,
+    AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_2D = 5,
+    AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_3D = 6
+} AeroLbmBenchmarkPreset;
+
+/* ref: native/include/aero_lbm_capi.h:45-54 */
+typedef enum AeroLbmHydrodynamicBoundaryKind {
+    AERO_LBM_HYDRO_BOUNDARY_INHERIT_GAME = 0,
+    AERO_LBM_HYDRO_BOUNDARY_PERIODIC = 1,
+    AERO_LBM_HYDRO_BOUNDARY_BOUNCE_BACK = 2,
+    AERO_LBM_HYDRO_BOUNDARY_MOVING_WALL = 3,
+    AERO_LBM_HYDRO_BOUNDARY_VELOCITY_DIRICHLET = 4,
+    AERO_LBM_HYDRO_BOUNDARY_PRESSURE_DIRICHLET = 5,
+    AERO_LBM_HYDRO_BOUNDARY_CONVECTIVE_OUTFLOW = 6,
+    AERO_LBM_HYDRO_BOUNDARY_SYMMETRY = 7
+} AeroLbmHydrodynamicBoundaryKind;
+
+/* ref: native/include/aero_lbm_capi.h:56-63 */
+typedef enum AeroLbmThermalBoundaryKind {
+    AERO_LBM_THERMAL_BOUNDARY_INHERIT_GAME = 0,
+    AERO_LBM_THERMAL_BOUNDARY_DISABLED = 1,
+    AERO_LBM_THERMAL_BOUNDARY_PERIODIC = 2,
+    AERO_LBM_THERMAL_BOUNDARY_ADIABATIC = 3,
+    AERO_LBM_THERMAL_BOUNDARY_TEMPERATURE_DIRICHLET = 4,
+    AERO_LBM_THERMAL_BOUNDARY_HEAT_FLUX_NEUMANN = 5
+} AeroLbmThermalBoundaryKind;
+
+/* ref: native/include/aero_lbm_capi.h:65-74 */
+typedef enum AeroLbmBenchmarkFlags {
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING = 1u << 0,
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_NOISE = 1u << 1,
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_SPONGE = 1u << 2,
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_CONVECTIVE_OUTFLOW = 1u << 3,
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_OBSTACLE_BOUNCE_BLEND = 1u << 4,
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_SGS = 1u << 5,
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_INTERNAL_THERMAL_SOURCE = 1u << 6,
+    AERO_LBM_BENCHMARK_FLAG_DISABLE_BUOYANCY = 1u << 7
+} AeroLbmBenchmarkFlags;
+
+/* ref: native/include/aero_lbm_capi.h:76-83 */
+typedef struct AeroLbmBoundaryFaceConfig {
+    int hydrodynamic_kind;
+    int thermal_kind;
+    float velocity[3];
+    float pressure;
+    float temperature;
+    float heat_flux;
+} AeroLbmBoundaryFaceConfig;
+
+/* ref: native/include/aero_lbm_capi.h:85-111 */
+typedef struct AeroLbmBenchmarkConfig {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    uint32_t flags;
+    int enabled;
+    int preset;
+    int reserved0;
+    float reynolds_number;
+    float rayleigh_number;
+    float prandtl_number;
+    float mach_number;
+    float reference_density;
+    float reference_temperature;
+    float reference_length;
+    float body_force[3];
+    float gravity[3];
+    float initial_velocity[3];
+    float initial_pressure;
+    float initial_temperature;
+    AeroLbmBoundaryFaceConfig x_min;
+    AeroLbmBoundaryFaceConfig x_max;
+    AeroLbmBoundaryFaceConfig y_min;
+    AeroLbmBoundaryFaceConfig y_max;
+    AeroLbmBoundaryFaceConfig z_min;
+    AeroLbmBoundaryFaceConfig z_max;
+    uint32_t reserved[8];
+} AeroLbmBenchmarkConfig;
+
+/* ref: native/include/aero_lbm_capi.h:113-123 */
+typedef struct AeroLbmTimingSnapshot {
+    uint64_t ticks;
+    double last_payload_copy_ms;
+    double last_solver_ms;
+    double last_readback_ms;
+    double last_total_ms;
+    double avg_payload_copy_ms;
+    double avg_solver_ms;
+    double avg_readback_ms;
+    double avg_total_ms;
+} AeroLbmTimingSnapshot;
+
+/* ref: native/include/aero_lbm_capi.h:125-141 */
+typedef struct AeroLbmMesoscaleConfig {
+    uint32_t abi_version;
+    uint32_t struct_size;
+    int nx;
+    int ny;
+    int nz;
+    float dx_m;
+    float dt_s;
+    float molecular_nu_m2_s;
+    float prandtl_air;
+    float turbulent_prandtl;
+    float reference_density_kg_m3;
+    float ambient_air_temperature_k;
+    float deep_ground_temperature_k;
+    float background_wind_m_s[3];
+    uint32_t reserved[8];
+} AeroLbmMesoscaleConfig;
+
+/* ref: native/include/aero_lbm_capi.h:143-149 */
+typedef struct AeroLbmMesoscaleTransport {
+    float velocity_scale_m_s_per_lattice;
+    float nu_molecular_lattice;
+    float alpha_molecular_lattice;
+    float tau_shear_molecular;
+    float tau_thermal_molecular;
+} AeroLbmMesoscaleTransport;
+
+/* ref: native/include/aero_lbm_capi.h:151-164 */
+typedef struct AeroLbmWorldDelta {
+    int type;
+    int x;
+    int y;
+    int z;
+    int data0;
+    int data1;
+    int data2;
+    int data3;
+    float value0;
+    float value1;
+    float value2;
+    float value3;
+} AeroLbmWorldDelta;
+
+/* ------------------------------------------------------------------------
+ * Extension: physics and kernel overrides.
+ *
+ * The reference hard-codes the Smagorinsky-type constant (C = 0.025), the
+ * Boussinesq coefficient (beta = 0.001635, acting along +y) and the benchmark
+ * thermal diffusivity (0.035) (ref: native/src/aero_lbm_jni.cpp:131-167,
+ * 2288-2295, 2346-2366). This struct lets a caller re-parametrise them. A NaN
+ * float or a negative int keeps the reference value. Buoyancy always acts
+ * against the gravity axis given here (default: axis 1 = y, sign -1, so the
+ * force points to +y exactly as in the reference).
+ * ---------------------------------------------------------------------- */
+enum {
+    AERO_LBM_KERNEL_FAST = 0,   /* chimera moment transforms, FMA contraction */
+    AERO_LBM_KERNEL_EXACT = 1   /* operation order of the reference CPU path */
+};
+
+typedef struct AeroLbmSolverOverrides {
+    uint32_t struct_size;
+    float sgs_constant;          /* C in nu_t = C^2 |Pi_neq| / (2 rho nu0) */
+    float sgs_nut_cap_ratio;     /* nu_t <= ratio * nu0 */
+    float boussinesq_beta;
+    float thermal_diffusivity;   /* lattice units, benchmark mode */
+    int gravity_axis;            /* 0 = x, 1 = y, 2 = z */
+    int gravity_sign;            /* -1: gravity points to -axis (buoyancy to +axis) */
+    int kernel_variant;          /* AERO_LBM_KERNEL_FAST / AERO_LBM_KERNEL_EXACT */
+    int reserved[8];
+} AeroLbmSolverOverrides;
+
+/* ---- core solver context API (ref: native/include/aero_lbm_capi.h:166-285) ---- */
+AERO_LBM_CAPI_EXPORT int aero_lbm_init(int grid_size, int input_channels, int output_channels);
+AERO_LBM_CAPI_EXPORT int aero_lbm_step(const float* packet, int grid_size, long long context_key, float* output_flow);
+AERO_LBM_CAPI_EXPORT int aero_lbm_init_rect(int nx, int ny, int nz, int input_channels, int output_channels);
+AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect(const float* packet, int nx, int ny, int nz, long long context_key, float* output_flow);
+AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect_scaled(const float* packet, int nx, int ny, int nz, long long context_key, float output_velocity_scale, float* output_flow);
+AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect_cached(int nx, int ny, int nz, long long context_key, float* output_flow);
+AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect_cached_scaled(int nx, int ny, int nz, long long context_key, float output_velocity_scale, float* output_flow);
+AERO_LBM_CAPI_EXPORT int aero_lbm_has_context(long long context_key);
+AERO_LBM_CAPI_EXPORT int aero_lbm_context_realtime_cached_initialized(long long context_key);
+AERO_LBM_CAPI_EXPORT int aero_lbm_get_temperature_state_rect(int nx, int ny, int nz, long long context_key, float* out_temperature);
+AERO_LBM_CAPI_EXPORT int aero_lbm_sample_temperature_point_rect(
+    int nx, int ny, int nz, long long context_key,
+    int sample_x, int sample_y, int sample_z, float* out_temperature);
+AERO_LBM_CAPI_EXPORT int aero_lbm_sample_flow_point_rect(
+    int nx, int ny, int nz, long long context_key,
+    int sample_x, int sample_y, int sample_z, float* out_flow);
+AERO_LBM_CAPI_EXPORT int aero_lbm_extract_flow_atlas_rect(
+    int nx, int ny, int nz, long long context_key,
+    int stride, float* out_flow_atlas, int value_count);
+AERO_LBM_CAPI_EXPORT int aero_lbm_copy_flow_temperature_subrect(
+    int nx, int ny, int nz, long long context_key,
+    int offset_x, int offset_y, int offset_z,
+    int copy_nx, int copy_ny, int copy_nz,
+    float* out_flow, float* out_temperature);
+AERO_LBM_CAPI_EXPORT int aero_lbm_get_flow_state_rect(int nx, int ny, int nz, long long context_key, float* out_flow);
+AERO_LBM_CAPI_EXPORT int aero_lbm_set_temperature_state_rect(int nx, int ny, int nz, long long context_key, const float* temperature);
+AERO_LBM_CAPI_EXPORT int aero_lbm_get_last_force(long long context_key, float* out_fx, float* out_fy, float* out_fz);
+AERO_LBM_CAPI_EXPORT void aero_lbm_release_context(long long context_key);
+AERO_LBM_CAPI_EXPORT void aero_lbm_shutdown(void);
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_runtime_info(void);
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_last_error(void);
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_timing_info(void);
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_memory_info(void);
+AERO_LBM_CAPI_EXPORT void aero_lbm_reset_timing(void);
+AERO_LBM_CAPI_EXPORT void aero_lbm_set_realtime_solver_mode(int solver_mode);
+AERO_LBM_CAPI_EXPORT int aero_lbm_finish(void);
+AERO_LBM_CAPI_EXPORT int aero_lbm_get_timing_snapshot(AeroLbmTimingSnapshot* out_snapshot);
+
+/* ---- benchmark configuration (ref: native/include/aero_lbm_capi.h:698-703) ---- */
+AERO_LBM_CAPI_EXPORT void aero_lbm_benchmark_default_config(AeroLbmBenchmarkConfig* out_config);
+AERO_LBM_CAPI_EXPORT int aero_lbm_benchmark_default_preset_config(int preset, AeroLbmBenchmarkConfig* out_config);
+AERO_LBM_CAPI_EXPORT int aero_lbm_benchmark_set_config(const AeroLbmBenchmarkConfig* config);
+AERO_LBM_CAPI_EXPORT int aero_lbm_benchmark_get_config(AeroLbmBenchmarkConfig* out_config);
+AERO_LBM_CAPI_EXPORT void aero_lbm_benchmark_reset_config(void);
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_benchmark_info(void);
+
+/* ---- extension: overrides (no counterpart in the reference; for callers that
+ * need the BASELINE-style parametrisation: Cs, beta, gravity axis, exact vs fast) ---- */
+AERO_LBM_CAPI_EXPORT void aero_lbm_default_solver_overrides(AeroLbmSolverOverrides* out_overrides);
+AERO_LBM_CAPI_EXPORT int aero_lbm_set_solver_overrides(const AeroLbmSolverOverrides* overrides);
+AERO_LBM_CAPI_EXPORT int aero_lbm_get_solver_overrides(AeroLbmSolverOverrides* out_overrides);
+
+/* ---- extension: device-resident timing (no counterpart in the reference).
+ * Runs `steps` cached steps of a context back to back on the device stream with
+ * no readback and returns the elapsed device time (HIP events) in *out_ms. ---- */
+AERO_LBM_CAPI_EXPORT int aero_lbm_run_cached_steps_timed(
+    int nx, int ny, int nz, long long context_key, int steps, float* out_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AERO_LBM_CAPI_H */

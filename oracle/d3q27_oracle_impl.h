@@ -1,0 +1,831 @@
+/*
+ * D3Q27 oracle body -- TEST INFRASTRUCTURE ONLY (tests/, smoke(), bench.py's
+ * cpu_baseline). Never linked into or called by libaero_lbm.
+ *
+ * Plain-C restatement of the reference's single-threaded CPU step in benchmark
+ * mode, native/src/aero_lbm_jni.cpp (run_cpu_step, 8480-8507): ingest ->
+ * initialise (first step) -> temperature update -> collide -> stream -> output.
+ * Included twice by d3q27_oracle.c: once with R = float (operation order of the
+ * reference, so FP32 results are bit-identical to the reference CPU path when
+ * built with -ffp-contract=off on x86-64 SSE), once with R = double (the FP64
+ * oracle used for accuracy claims).
+ *
+ * Expected macros: R, SFX(name), K(literal), RSQRT_(x) sqrt, RFLOOR_(x),
+ * RFABS_(x), RISFINITE_(x).
+ */
+
+/* std::max / std::min / clampf as the reference defines them
+ * (jni.cpp:1193-1195): min(a,b) = (b<a)?b:a, max(a,b) = (a<b)?b:a. */
+static inline R SFX(rmax)(R a, R b) { return (a < b) ? b : a; }
+static inline R SFX(rmin)(R a, R b) { return (b < a) ? b : a; }
+static inline R SFX(clampr)(R v, R lo, R hi) { return SFX(rmin)(hi, SFX(rmax)(lo, v)); }
+static inline R SFX(finite_or)(R v, R fb) { return RISFINITE_(v) ? v : fb; }
+
+/* jni.cpp:46-50 kMomentInv1D */
+static const R SFX(kInv1D)[3][3] = {
+    {K(0.0), K(-0.5), K(0.5)},
+    {K(1.0), K(0.0), K(-1.0)},
+    {K(0.0), K(0.5), K(0.5)},
+};
+
+typedef struct SFX(Ctx) {
+    int nx, ny, nz;
+    size_t cells;
+    int in_ch, out_ch;
+    AeroLbmBenchmarkConfig cfg;
+    /* resolved physics */
+    R tau_shear_base, tau_normal_base, nu_base, alpha, beta;
+    R sgs_c2, nut_cap_ratio;
+    int g_axis, g_sign;
+    int initialized;
+    uint64_t step_counter;
+    R *f, *f_post;
+    R *rho, *ux, *uy, *uz;
+    R *ref_ux, *ref_uy, *ref_uz, *ref_p, *ref_T, *src;
+    unsigned char* obstacle;
+    R *T, *T_next;
+    R w[27];
+    R kmax_speed;
+} SFX(Ctx);
+
+/* hydro_core.h:130-205 (tables): q = (ix*3+iy)*3+iz, c = ix-1, ..., w = product of
+ * the 1-D weights {1/6, 2/3, 1/6} evaluated left to right in R. */
+static void SFX(init_weights)(R w[27]) {
+    const R w1[3] = {K(1.0) / K(6.0), K(2.0) / K(3.0), K(1.0) / K(6.0)};
+    int q = 0;
+    for (int ix = 0; ix < 3; ++ix)
+        for (int iy = 0; iy < 3; ++iy)
+            for (int iz = 0; iz < 3; ++iz) w[q++] = w1[ix] * w1[iy] * w1[iz];
+}
+
+/* hydro_core.cpp:364-368 equilibrium_d3q27 */
+static inline R SFX(feq)(const SFX(Ctx) * c, int q, R rho, R ux, R uy, R uz) {
+    const R cu = K(3.0) * ((R)kCX[q] * ux + (R)kCY[q] * uy + (R)kCZ[q] * uz);
+    const R uu = ux * ux + uy * uy + uz * uz;
+    return c->w[q] * rho * (K(1.0) + cu + K(0.5) * cu * cu - K(1.5) * uu);
+}
+
+static inline size_t SFX(cell_index)(const SFX(Ctx) * c, int x, int y, int z) {
+    return ((size_t)x * (size_t)c->ny + (size_t)y) * (size_t)c->nz + (size_t)z;
+}
+
+static inline const AeroLbmBoundaryFaceConfig* SFX(face)(const SFX(Ctx) * c, int idx) {
+    switch (idx) {
+        case 0: return &c->cfg.x_min;
+        case 1: return &c->cfg.x_max;
+        case 2: return &c->cfg.y_min;
+        case 3: return &c->cfg.y_max;
+        case 4: return &c->cfg.z_min;
+        default: return &c->cfg.z_max;
+    }
+}
+
+/* jni.cpp:729-741 wrap_axis_periodic */
+static int SFX(wrap_axis)(const SFX(Ctx) * c, int* coord, int dim, int min_face, int max_face, int thermal) {
+    if (*coord >= 0 && *coord < dim) return 0;
+    const AeroLbmBoundaryFaceConfig* f = SFX(face)(c, *coord < 0 ? min_face : max_face);
+    const int kind = thermal ? f->thermal_kind : f->hydrodynamic_kind;
+    const int periodic = thermal ? AERO_LBM_THERMAL_BOUNDARY_PERIODIC : AERO_LBM_HYDRO_BOUNDARY_PERIODIC;
+    if (kind != periodic) return 0;
+    *coord %= dim;
+    if (*coord < 0) *coord += dim;
+    return 1;
+}
+
+static void SFX(remap)(const SFX(Ctx) * c, int* x, int* y, int* z, int thermal) {
+    SFX(wrap_axis)(c, x, c->nx, 0, 1, thermal);
+    SFX(wrap_axis)(c, y, c->ny, 2, 3, thermal);
+    SFX(wrap_axis)(c, z, c->nz, 4, 5, thermal);
+}
+
+/* jni.cpp:743-761 face lookup order x, y, z */
+static const AeroLbmBoundaryFaceConfig* SFX(face_for_oob)(const SFX(Ctx) * c, int x, int y, int z) {
+    if (x < 0) return &c->cfg.x_min;
+    if (x >= c->nx) return &c->cfg.x_max;
+    if (y < 0) return &c->cfg.y_min;
+    if (y >= c->ny) return &c->cfg.y_max;
+    if (z < 0) return &c->cfg.z_min;
+    if (z >= c->nz) return &c->cfg.z_max;
+    return NULL;
+}
+
+static inline int SFX(flag)(const SFX(Ctx) * c, uint32_t bit) {
+    return c->cfg.enabled != 0 && (c->cfg.flags & bit) != 0;
+}
+
+/* ---------------- parameter resolution (jni.cpp:787-993) ---------------- */
+static R SFX(clamp_nu)(R nu) {
+    const R nu_min = K(1.0e-8);
+    const R nu_max = (K(0.95) - K(0.5)) / K(3.0);
+    return SFX(clampr)(nu, nu_min, nu_max);
+}
+static R SFX(tau_from_nu)(R nu) { return K(0.5) + K(3.0) * SFX(clamp_nu)(nu); }
+
+static R SFX(reference_speed)(const AeroLbmBenchmarkConfig* g) {
+    /* jni.cpp:819-859, presets other than heated cavity / cylinder */
+    const float* iv = g->initial_velocity;
+    R speed = RSQRT_((R)iv[0] * (R)iv[0] + (R)iv[1] * (R)iv[1] + (R)iv[2] * (R)iv[2]);
+    if (g->preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) {
+        if (speed > K(1.0e-8)) return speed;
+        const float* v = g->x_min.velocity;
+        const R fs = RSQRT_((R)v[0] * (R)v[0] + (R)v[1] * (R)v[1] + (R)v[2] * (R)v[2]);
+        return (K(2.0) / K(3.0)) * fs;
+    }
+    const AeroLbmBoundaryFaceConfig* faces[6] = {&g->x_min, &g->x_max, &g->y_min, &g->y_max, &g->z_min, &g->z_max};
+    for (int i = 0; i < 6; ++i) {
+        const float* v = faces[i]->velocity;
+        const R fs = RSQRT_((R)v[0] * (R)v[0] + (R)v[1] * (R)v[1] + (R)v[2] * (R)v[2]);
+        speed = SFX(rmax)(speed, fs);
+    }
+    return speed;
+}
+
+static R SFX(finite_or_f)(float v, R fb) { return isfinite(v) ? (R)v : fb; }
+
+static int SFX(resolve)(SFX(Ctx) * c, const AeroLbmSolverOverrides* ov) {
+    const AeroLbmBenchmarkConfig* g = &c->cfg;
+    if (!g->enabled) return 0; /* runtime mode is not restated here */
+    if (g->preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_2D ||
+        g->preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_3D)
+        return 0; /* BFECC thermal path not restated */
+    if (g->preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) return 0;
+    if (!(g->flags & AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING)) return 0;
+    /* jni.cpp:917-929 effective_base_nu_shear */
+    const R re = SFX(finite_or_f)(g->reynolds_number, K(0.0));
+    const R ref_speed = SFX(reference_speed)(g);
+    const R ref_len = SFX(rmax)(K(1.0e-6), SFX(finite_or_f)(g->reference_length, K(1.0)));
+    /* runtime molecular nu0 = 1.5e-5 m^2/s * 0.05 s / 1 m^2 (jni.cpp:100-104) */
+    const R nu_rt = SFX(clamp_nu)(K(1.50e-5) * (K(1.0) / K(20.0)) / (K(1.0) * K(1.0)));
+    R nu;
+    if (re <= K(1.0e-6) || ref_speed <= K(1.0e-8)) nu = nu_rt;
+    else nu = SFX(clamp_nu)(ref_speed * ref_len / re);
+    c->nu_base = nu;
+    const R tau_min = K(0.5) + K(3.0) * K(1.0e-8);
+    c->tau_shear_base = SFX(clampr)(SFX(tau_from_nu)(nu), tau_min, K(0.95));
+    c->tau_normal_base = SFX(clampr)(SFX(tau_from_nu)(SFX(clamp_nu)(nu)), tau_min, K(0.95));
+    /* jni.cpp:950-955 benchmark thermal diffusivity */
+    c->alpha = K(0.035);
+    /* thermal_core.h:268-279 with jni.cpp:161-167 constants */
+    c->beta = ((K(9.81) * (K(1.0) / K(300.0))) * K(20.0)) * (((K(1.0) / K(20.0)) * (K(1.0) / K(20.0))) / K(1.0));
+    R sgs_c = K(0.025);
+    c->nut_cap_ratio = K(5.0);
+    c->g_axis = 1;
+    c->g_sign = -1;
+    if (ov) {
+        if (!isnan(ov->sgs_constant)) sgs_c = (R)ov->sgs_constant;
+        if (!isnan(ov->sgs_nut_cap_ratio)) c->nut_cap_ratio = (R)ov->sgs_nut_cap_ratio;
+        if (!isnan(ov->boussinesq_beta)) c->beta = (R)ov->boussinesq_beta;
+        if (!isnan(ov->thermal_diffusivity)) c->alpha = (R)ov->thermal_diffusivity;
+        if (ov->gravity_axis >= 0 && ov->gravity_axis <= 2) c->g_axis = ov->gravity_axis;
+        if (ov->gravity_sign == 1 || ov->gravity_sign == -1) c->g_sign = ov->gravity_sign;
+    }
+    c->sgs_c2 = sgs_c * sgs_c;
+    c->kmax_speed = K(0.57735026919) * K(0.60);
+    return 1;
+}
+
+/* ---------------- per-cell helpers ---------------- */
+static void SFX(reseed)(SFX(Ctx) * c, size_t cell) {
+    /* jni.cpp:1724-1750 */
+    const R p = SFX(clampr)(SFX(finite_or)(c->ref_p[cell], K(0.0)), K(-0.03), K(0.03));
+    const R rho = SFX(clampr)(K(1.0) + p, K(0.97), K(1.03));
+    R ux = SFX(finite_or)(c->ref_ux[cell], K(0.0));
+    R uy = SFX(finite_or)(c->ref_uy[cell], K(0.0));
+    R uz = SFX(finite_or)(c->ref_uz[cell], K(0.0));
+    const R s2 = ux * ux + uy * uy + uz * uz;
+    if (!RISFINITE_(s2) || s2 > c->kmax_speed * c->kmax_speed) {
+        if (!RISFINITE_(s2) || s2 <= K(0.0)) {
+            ux = uy = uz = K(0.0);
+        } else {
+            const R sc = c->kmax_speed / RSQRT_(s2);
+            ux *= sc; uy *= sc; uz *= sc;
+        }
+    }
+    c->rho[cell] = rho;
+    c->ux[cell] = ux; c->uy[cell] = uy; c->uz[cell] = uz;
+    for (int q = 0; q < 27; ++q) c->f_post[(size_t)q * c->cells + cell] = SFX(feq)(c, q, rho, ux, uy, uz);
+}
+
+static int SFX(binom)(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    if (n <= 1 || k == 0 || k == n) return 1;
+    return 2;
+}
+
+/* jni.cpp:1335-1357 */
+static void SFX(raw_moments)(const R f[27], R raw[3][3][3]) {
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) for (int cc = 0; cc < 3; ++cc) raw[a][b][cc] = K(0.0);
+    for (int q = 0; q < 27; ++q) {
+        const R fq = f[q];
+        const R px[3] = {K(1.0), (R)kCX[q], (R)(kCX[q] * kCX[q])};
+        const R py[3] = {K(1.0), (R)kCY[q], (R)(kCY[q] * kCY[q])};
+        const R pz[3] = {K(1.0), (R)kCZ[q], (R)(kCZ[q] * kCZ[q])};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int cc = 0; cc < 3; ++cc) raw[a][b][cc] += fq * px[a] * py[b] * pz[cc];
+    }
+}
+
+/* jni.cpp:1359-1382 */
+static void SFX(central_moments)(const R raw[3][3][3], R ux, R uy, R uz, R cen[3][3][3]) {
+    const R sx[3] = {K(1.0), -ux, ux * ux};
+    const R sy[3] = {K(1.0), -uy, uy * uy};
+    const R sz[3] = {K(1.0), -uz, uz * uz};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int cc = 0; cc < 3; ++cc) {
+                R sum = K(0.0);
+                for (int p = 0; p <= a; ++p)
+                    for (int q = 0; q <= b; ++q)
+                        for (int r = 0; r <= cc; ++r) {
+                            const R coeff = (R)(SFX(binom)(a, p) * SFX(binom)(b, q) * SFX(binom)(cc, r));
+                            sum += coeff * sx[a - p] * sy[b - q] * sz[cc - r] * raw[p][q][r];
+                        }
+                cen[a][b][cc] = sum;
+            }
+}
+
+/* jni.cpp:1384-1430 */
+static void SFX(relax_closure)(R rho, R in[3][3][3], R om_diag, R om_off, R out[3][3][3]) {
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) for (int cc = 0; cc < 3; ++cc) out[a][b][cc] = K(0.0);
+    out[0][0][0] = rho;
+    const R cs2 = K(1.0) / K(3.0);
+    const R eq2 = rho * cs2;
+    const R sd = SFX(clampr)(om_diag, K(0.0), K(1.95));
+    const R so = SFX(clampr)(om_off, K(0.0), K(1.95));
+    out[2][0][0] = in[2][0][0] + sd * (eq2 - in[2][0][0]);
+    out[0][2][0] = in[0][2][0] + sd * (eq2 - in[0][2][0]);
+    out[0][0][2] = in[0][0][2] + sd * (eq2 - in[0][0][2]);
+    out[1][1][0] = (K(1.0) - so) * in[1][1][0];
+    out[1][0][1] = (K(1.0) - so) * in[1][0][1];
+    out[0][1][1] = (K(1.0) - so) * in[0][1][1];
+    const R c200 = out[2][0][0], c020 = out[0][2][0], c002 = out[0][0][2];
+    const R c110 = out[1][1][0], c101 = out[1][0][1], c011 = out[0][1][1];
+    const R ir = K(1.0) / SFX(rmax)(K(1e-6), rho);
+    out[2][2][0] = c200 * c020 * ir + K(2.0) * c110 * c110 * ir;
+    out[2][0][2] = c200 * c002 * ir + K(2.0) * c101 * c101 * ir;
+    out[0][2][2] = c020 * c002 * ir + K(2.0) * c011 * c011 * ir;
+    out[2][1][1] = (c200 * c011 + K(2.0) * c110 * c101) * ir;
+    out[1][2][1] = (c020 * c101 + K(2.0) * c110 * c011) * ir;
+    out[1][1][2] = (c002 * c110 + K(2.0) * c101 * c011) * ir;
+    out[2][2][2] = (c200 * c020 * c002 + K(2.0) * c110 * c110 * c002 + K(2.0) * c101 * c101 * c020 +
+                    K(2.0) * c011 * c011 * c200 + K(8.0) * c110 * c101 * c011) * ir * ir;
+}
+
+/* jni.cpp:1432-1455 */
+static void SFX(central_to_raw)(const R cen[3][3][3], R ux, R uy, R uz, R raw[3][3][3]) {
+    const R px[3] = {K(1.0), ux, ux * ux};
+    const R py[3] = {K(1.0), uy, uy * uy};
+    const R pz[3] = {K(1.0), uz, uz * uz};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int cc = 0; cc < 3; ++cc) {
+                R sum = K(0.0);
+                for (int p = 0; p <= a; ++p)
+                    for (int q = 0; q <= b; ++q)
+                        for (int r = 0; r <= cc; ++r) {
+                            const R coeff = (R)(SFX(binom)(a, p) * SFX(binom)(b, q) * SFX(binom)(cc, r));
+                            sum += coeff * px[a - p] * py[b - q] * pz[cc - r] * cen[p][q][r];
+                        }
+                raw[a][b][cc] = sum;
+            }
+}
+
+/* jni.cpp:1457-1473 */
+static void SFX(raw_to_pop)(const R raw[3][3][3], R f[27]) {
+    for (int ix = 0; ix < 3; ++ix)
+        for (int iy = 0; iy < 3; ++iy)
+            for (int iz = 0; iz < 3; ++iz) {
+                R sum = K(0.0);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        for (int cc = 0; cc < 3; ++cc)
+                            sum += SFX(kInv1D)[ix][a] * SFX(kInv1D)[iy][b] * SFX(kInv1D)[iz][cc] * raw[a][b][cc];
+                f[(ix * 3 + iy) * 3 + iz] = sum;
+            }
+}
+
+/* jni.cpp:1270-1281 */
+static R SFX(guo)(const SFX(Ctx) * c, int q, R ux, R uy, R uz, R fx, R fy, R fz, R omega) {
+    const R cx = (R)kCX[q], cy = (R)kCY[q], cz = (R)kCZ[q];
+    const R cu = cx * ux + cy * uy + cz * uz;
+    const R cf = cx * fx + cy * fy + cz * fz;
+    const R uf = ux * fx + uy * fy + uz * fz;
+    const R inv_cs2 = K(1.0) / (K(1.0) / K(3.0));
+    const R inv_cs4 = inv_cs2 * inv_cs2;
+    const R pref = SFX(rmax)(K(0.0), K(1.0) - K(0.5) * SFX(clampr)(omega, K(0.0), K(1.999)));
+    return pref * c->w[q] * ((cf - uf) * inv_cs2 + cu * cf * inv_cs4);
+}
+
+/* jni.cpp:767-774 effective_sponge_alpha (benchmark flag gates it) */
+static R SFX(sponge_alpha)(const SFX(Ctx) * c, int x, int y, int z) {
+    if (SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_SPONGE)) return K(0.0);
+    int d = x;
+    if (y < d) d = y;
+    if (z < d) d = z;
+    if (c->nx - 1 - x < d) d = c->nx - 1 - x;
+    if (c->ny - 1 - y < d) d = c->ny - 1 - y;
+    if (c->nz - 1 - z < d) d = c->nz - 1 - z;
+    if (d >= 4) return K(0.0);
+    const R eta = (R)(4 - d) / (R)4;
+    return SFX(clampr)(K(0.03) * eta * eta, K(0.0), K(0.95));
+}
+
+/* ---------------- collide (jni.cpp:2209-2413, benchmark mode) ---------------- */
+static void SFX(collide)(SFX(Ctx) * c) {
+    const size_t N = c->cells;
+    const int sgs_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_SGS);
+    const int buoy_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_BUOYANCY);
+    const R cs2 = K(1.0) / K(3.0);
+    for (size_t cell = 0; cell < N; ++cell) {
+        if (c->obstacle[cell]) {
+            c->rho[cell] = K(1.0);
+            c->ux[cell] = c->uy[cell] = c->uz[cell] = K(0.0);
+            for (int q = 0; q < 27; ++q) c->f_post[(size_t)q * N + cell] = c->f[(size_t)q * N + cell];
+            continue;
+        }
+        R fl[27];
+        R rho = K(0.0), mx = K(0.0), my = K(0.0), mz = K(0.0);
+        int bad = 0;
+        for (int q = 0; q < 27; ++q) {
+            const R fq = c->f[(size_t)q * N + cell];
+            if (!RISFINITE_(fq)) { bad = 1; break; }
+            fl[q] = fq;
+            rho += fq;
+            mx += fq * (R)kCX[q];
+            my += fq * (R)kCY[q];
+            mz += fq * (R)kCZ[q];
+        }
+        if (bad || !RISFINITE_(rho) || !RISFINITE_(mx) || !RISFINITE_(my) || !RISFINITE_(mz)) {
+            SFX(reseed)(c, cell);
+            continue;
+        }
+        const R rho_safe = SFX(rmax)(K(1e-6), rho);
+        const R inv_rho = K(1.0) / rho_safe;
+        R ux = mx * inv_rho, uy = my * inv_rho, uz = mz * inv_rho;
+        if (!RISFINITE_(rho_safe) || !RISFINITE_(inv_rho) || !RISFINITE_(ux) || !RISFINITE_(uy) || !RISFINITE_(uz)) {
+            SFX(reseed)(c, cell);
+            continue;
+        }
+        R F[3] = {K(0.0), K(0.0), K(0.0)};
+        if (buoy_on) {
+            const R b = SFX(clampr)(c->beta * c->T[cell], K(-0.02), K(0.02));
+            if (c->g_sign < 0) F[c->g_axis] += rho_safe * b;
+            else F[c->g_axis] += rho_safe * (-b);
+        }
+        const R dux = F[0] * inv_rho, duy = F[1] * inv_rho, duz = F[2] * inv_rho;
+        ux += K(0.5) * dux;
+        uy += K(0.5) * duy;
+        uz += K(0.5) * duz;
+        /* benchmark mode: state nudge 0 (jni.cpp:679-681) */
+        ux = (K(1.0) - K(0.0)) * ux + K(0.0) * c->ref_ux[cell];
+        uy = (K(1.0) - K(0.0)) * uy + K(0.0) * c->ref_uy[cell];
+        uz = (K(1.0) - K(0.0)) * uz + K(0.0) * c->ref_uz[cell];
+        if (!RISFINITE_(ux) || !RISFINITE_(uy) || !RISFINITE_(uz)) { SFX(reseed)(c, cell); continue; }
+        const R s2 = ux * ux + uy * uy + uz * uz;
+        if (!RISFINITE_(s2)) { SFX(reseed)(c, cell); continue; }
+        if (s2 > c->kmax_speed * c->kmax_speed) {
+            const R sc = c->kmax_speed / RSQRT_(s2);
+            ux *= sc; uy *= sc; uz *= sc;
+        }
+        c->rho[cell] = rho;
+        c->ux[cell] = ux; c->uy[cell] = uy; c->uz[cell] = uz;
+
+        R raw[3][3][3], cen[3][3][3], cpost[3][3][3], rpost[3][3][3];
+        SFX(raw_moments)(fl, raw);
+        SFX(central_moments)(raw, ux, uy, uz, cen);
+
+        R ts = c->tau_shear_base, tn = c->tau_normal_base;
+        if (sgs_on) {
+            const R nu0 = SFX(rmax)(K(1e-6), c->nu_base);
+            const R nxx = cen[2][0][0] - rho_safe * cs2;
+            const R nyy = cen[0][2][0] - rho_safe * cs2;
+            const R nzz = cen[0][0][2] - rho_safe * cs2;
+            const R nxy = cen[1][1][0], nxz = cen[1][0][1], nyz = cen[0][1][1];
+            const R qn2 = nxx * nxx + nyy * nyy + nzz * nzz + K(2.0) * (nxy * nxy + nxz * nxz + nyz * nyz);
+            const R qmag = RSQRT_(SFX(rmax)(K(0.0), qn2));
+            const R smag = qmag / SFX(rmax)(K(1e-6), K(2.0) * rho_safe * nu0);
+            R nut = c->sgs_c2 * smag;
+            nut = SFX(rmin)(nut, c->nut_cap_ratio * nu0);
+            const R tau_min = K(0.5) + K(3.0) * K(1.0e-8);
+            ts = SFX(clampr)(SFX(tau_from_nu)(nu0 + nut), tau_min, K(0.95));
+            tn = SFX(clampr)(c->tau_normal_base + (ts - c->tau_shear_base) * K(0.30), tau_min, K(0.95));
+        }
+        const R om_diag = K(1.0) / tn;
+        const R om_off = K(1.0) / ts;
+        SFX(relax_closure)(rho_safe, cen, om_diag, om_off, cpost);
+        SFX(central_to_raw)(cpost, ux, uy, uz, rpost);
+        R fc[27];
+        SFX(raw_to_pop)(rpost, fc);
+        R rc = K(0.0), mxc = K(0.0), myc = K(0.0), mzc = K(0.0);
+        for (int q = 0; q < 27; ++q) {
+            const R fq = fc[q];
+            rc += fq;
+            mxc += fq * (R)kCX[q];
+            myc += fq * (R)kCY[q];
+            mzc += fq * (R)kCZ[q];
+        }
+        const R rcs = SFX(rmax)(K(1e-6), rc);
+        const R uxc = mxc / rcs, uyc = myc / rcs, uzc = mzc / rcs;
+        for (int q = 0; q < 27; ++q)
+            fc[q] += SFX(feq)(c, q, rho_safe, ux, uy, uz) - SFX(feq)(c, q, rcs, uxc, uyc, uzc);
+        for (int q = 0; q < 27; ++q) {
+            const R s = SFX(guo)(c, q, ux, uy, uz, F[0], F[1], F[2], om_off);
+            c->f_post[(size_t)q * N + cell] = fc[q] + s;
+        }
+        {
+            const int yz = c->ny * c->nz;
+            const int x = (int)(cell / (size_t)yz);
+            const int rem = (int)(cell - (size_t)x * (size_t)yz);
+            const int y = rem / c->nz;
+            const int z = rem - y * c->nz;
+            const R al = SFX(sponge_alpha)(c, x, y, z);
+            if (al > K(0.0)) {
+                const R keep = K(1.0) - al;
+                for (int q = 0; q < 27; ++q) {
+                    const R fe = SFX(feq)(c, q, K(1.0), K(0.0), K(0.0), K(0.0));
+                    R* fr = &c->f_post[(size_t)q * N + cell];
+                    *fr = keep * *fr + al * fe;
+                }
+            }
+        }
+    }
+}
+
+/* ---------------- streaming (jni.cpp:1475-1602, 2415-2452) ---------------- */
+static R SFX(bounce)(const SFX(Ctx) * c, size_t cell, int q) {
+    /* kObstacleBounceBlend = 0 (jni.cpp:140) so the blend never applies */
+    return c->f_post[(size_t)kOPP[q] * c->cells + cell];
+}
+
+static int SFX(iclamp)(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static R SFX(convective)(const SFX(Ctx) * c, size_t cell, int x, int y, int z, int q, int sx, int sy, int sz) {
+    const int dx = (sx < 0 || sx >= c->nx) ? kCX[q] : 0;
+    const int dy = (sy < 0 || sy >= c->ny) ? kCY[q] : 0;
+    const int dz = (sz < 0 || sz >= c->nz) ? kCZ[q] : 0;
+    const size_t s1 = SFX(cell_index)(c, SFX(iclamp)(x + dx, 0, c->nx - 1), SFX(iclamp)(y + dy, 0, c->ny - 1),
+                                      SFX(iclamp)(z + dz, 0, c->nz - 1));
+    if (c->obstacle[s1]) return SFX(bounce)(c, cell, q);
+    const R f1 = c->f_post[(size_t)q * c->cells + s1];
+    const size_t s2 = SFX(cell_index)(c, SFX(iclamp)(x + 2 * dx, 0, c->nx - 1), SFX(iclamp)(y + 2 * dy, 0, c->ny - 1),
+                                      SFX(iclamp)(z + 2 * dz, 0, c->nz - 1));
+    if (c->obstacle[s2]) return f1;
+    const R f2 = c->f_post[(size_t)q * c->cells + s2];
+    return f1 + K(0.15) * (f1 - f2);
+}
+
+static R SFX(equilibrium_face)(const SFX(Ctx) * c, size_t cell, int q, const AeroLbmBoundaryFaceConfig* fc) {
+    R rho = K(1.0) + SFX(clampr)(c->ref_p[cell], K(-0.03), K(0.03));
+    R ux = K(0.0), uy = K(0.0), uz = K(0.0);
+    switch (fc->hydrodynamic_kind) {
+        case AERO_LBM_HYDRO_BOUNDARY_MOVING_WALL:
+        case AERO_LBM_HYDRO_BOUNDARY_VELOCITY_DIRICHLET:
+            ux = (R)fc->velocity[0]; uy = (R)fc->velocity[1]; uz = (R)fc->velocity[2];
+            break;
+        case AERO_LBM_HYDRO_BOUNDARY_PRESSURE_DIRICHLET:
+            rho = K(1.0) + SFX(clampr)((R)fc->pressure, K(-0.03), K(0.03));
+            break;
+        default: break;
+    }
+    rho = SFX(clampr)(rho, K(0.97), K(1.03));
+    const R s2 = ux * ux + uy * uy + uz * uz;
+    if (s2 > c->kmax_speed * c->kmax_speed && s2 > K(0.0)) {
+        const R sc = c->kmax_speed / RSQRT_(s2);
+        ux *= sc; uy *= sc; uz *= sc;
+    }
+    return SFX(feq)(c, q, rho, ux, uy, uz);
+}
+
+static R SFX(face_value)(const SFX(Ctx) * c, size_t cell, int x, int y, int z, int q, int sx, int sy, int sz) {
+    const AeroLbmBoundaryFaceConfig* fc = SFX(face_for_oob)(c, sx, sy, sz);
+    const int kind = fc ? fc->hydrodynamic_kind : AERO_LBM_HYDRO_BOUNDARY_INHERIT_GAME;
+    switch (kind) {
+        case AERO_LBM_HYDRO_BOUNDARY_BOUNCE_BACK:
+            return SFX(bounce)(c, cell, q);
+        case AERO_LBM_HYDRO_BOUNDARY_MOVING_WALL:
+        case AERO_LBM_HYDRO_BOUNDARY_VELOCITY_DIRICHLET:
+        case AERO_LBM_HYDRO_BOUNDARY_PRESSURE_DIRICHLET:
+            return SFX(equilibrium_face)(c, cell, q, fc);
+        case AERO_LBM_HYDRO_BOUNDARY_SYMMETRY: {
+            const size_t s = SFX(cell_index)(c, SFX(iclamp)(sx, 0, c->nx - 1), SFX(iclamp)(sy, 0, c->ny - 1),
+                                             SFX(iclamp)(sz, 0, c->nz - 1));
+            return c->obstacle[s] ? SFX(bounce)(c, cell, q) : c->f_post[(size_t)q * c->cells + s];
+        }
+        case AERO_LBM_HYDRO_BOUNDARY_CONVECTIVE_OUTFLOW:
+            return SFX(convective)(c, cell, x, y, z, q, sx, sy, sz);
+        case AERO_LBM_HYDRO_BOUNDARY_PERIODIC:
+            break;
+        default:
+            if (SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_CONVECTIVE_OUTFLOW)) return SFX(bounce)(c, cell, q);
+            return SFX(convective)(c, cell, x, y, z, q, sx, sy, sz);
+    }
+    return SFX(convective)(c, cell, x, y, z, q, sx, sy, sz);
+}
+
+static void SFX(stream)(SFX(Ctx) * c) {
+    const size_t N = c->cells;
+    for (int x = 0; x < c->nx; ++x)
+        for (int y = 0; y < c->ny; ++y)
+            for (int z = 0; z < c->nz; ++z) {
+                const size_t cell = SFX(cell_index)(c, x, y, z);
+                R fl[27];
+                if (c->obstacle[cell]) {
+                    for (int q = 0; q < 27; ++q) fl[q] = c->f_post[(size_t)kOPP[q] * N + cell];
+                } else {
+                    for (int q = 0; q < 27; ++q) {
+                        int sx = x - kCX[q], sy = y - kCY[q], sz = z - kCZ[q];
+                        SFX(remap)(c, &sx, &sy, &sz, 0);
+                        if (sx < 0 || sy < 0 || sz < 0 || sx >= c->nx || sy >= c->ny || sz >= c->nz) {
+                            fl[q] = SFX(face_value)(c, cell, x, y, z, q, sx, sy, sz);
+                            continue;
+                        }
+                        const size_t src = SFX(cell_index)(c, sx, sy, sz);
+                        fl[q] = c->obstacle[src] ? SFX(bounce)(c, cell, q) : c->f_post[(size_t)q * N + src];
+                    }
+                }
+                for (int q = 0; q < 27; ++q) c->f[(size_t)q * N + cell] = fl[q];
+            }
+}
+
+/* ---------------- thermal scalar, benchmark first-order path (jni.cpp:2145-2188) ---------------- */
+static R SFX(t_neighbor)(const SFX(Ctx) * c, const R* field, int x, int y, int z, R self) {
+    SFX(remap)(c, &x, &y, &z, 1);
+    if (x < 0 || y < 0 || z < 0 || x >= c->nx || y >= c->ny || z >= c->nz) {
+        const AeroLbmBoundaryFaceConfig* fc = SFX(face_for_oob)(c, x, y, z);
+        const int kind = fc ? fc->thermal_kind : AERO_LBM_THERMAL_BOUNDARY_INHERIT_GAME;
+        switch (kind) {
+            case AERO_LBM_THERMAL_BOUNDARY_TEMPERATURE_DIRICHLET:
+                return SFX(clampr)((R)fc->temperature, K(-1.0), K(1.0));
+            case AERO_LBM_THERMAL_BOUNDARY_HEAT_FLUX_NEUMANN:
+                return SFX(clampr)(self + (R)fc->heat_flux, K(-1.0), K(1.0));
+            default:
+                return self;
+        }
+    }
+    const size_t cell = SFX(cell_index)(c, x, y, z);
+    if (c->obstacle[cell]) return self;
+    return field[cell];
+}
+
+static R SFX(trilinear)(const SFX(Ctx) * c, const R* field, R x, R y, R z, R fb) {
+    const int x0 = (int)RFLOOR_(x), y0 = (int)RFLOOR_(y), z0 = (int)RFLOOR_(z);
+    const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+    const R fx = x - (R)x0, fy = y - (R)y0, fz = z - (R)z0;
+    const R c000 = SFX(t_neighbor)(c, field, x0, y0, z0, fb);
+    const R c100 = SFX(t_neighbor)(c, field, x1, y0, z0, fb);
+    const R c010 = SFX(t_neighbor)(c, field, x0, y1, z0, fb);
+    const R c110 = SFX(t_neighbor)(c, field, x1, y1, z0, fb);
+    const R c001 = SFX(t_neighbor)(c, field, x0, y0, z1, fb);
+    const R c101 = SFX(t_neighbor)(c, field, x1, y0, z1, fb);
+    const R c011 = SFX(t_neighbor)(c, field, x0, y1, z1, fb);
+    const R c111 = SFX(t_neighbor)(c, field, x1, y1, z1, fb);
+    const R c00 = c000 + (c100 - c000) * fx;
+    const R c10 = c010 + (c110 - c010) * fx;
+    const R c01 = c001 + (c101 - c001) * fx;
+    const R c11 = c011 + (c111 - c011) * fx;
+    const R c0 = c00 + (c10 - c00) * fy;
+    const R c1 = c01 + (c11 - c01) * fy;
+    return c0 + (c1 - c0) * fz;
+}
+
+static void SFX(update_temperature)(SFX(Ctx) * c) {
+    const R dt = (R)1; /* benchmark stride 1 (jni.cpp:901-909) */
+    const int src_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_INTERNAL_THERMAL_SOURCE);
+    for (size_t cell = 0; cell < c->cells; ++cell) {
+        if (c->obstacle[cell]) { c->T_next[cell] = K(0.0); continue; }
+        const int yz = c->ny * c->nz;
+        const int x = (int)(cell / (size_t)yz);
+        const int rem = (int)(cell - (size_t)x * (size_t)yz);
+        const int y = rem / c->nz;
+        const int z = rem - y * c->nz;
+        const R tc = c->T[cell];
+        const R adv = SFX(trilinear)(c, c->T, (R)x - dt * c->ux[cell], (R)y - dt * c->uy[cell], (R)z - dt * c->uz[cell], tc);
+        const R tsum = SFX(t_neighbor)(c, c->T, x + 1, y, z, tc) + SFX(t_neighbor)(c, c->T, x - 1, y, z, tc) +
+                       SFX(t_neighbor)(c, c->T, x, y + 1, z, tc) + SFX(t_neighbor)(c, c->T, x, y - 1, z, tc) +
+                       SFX(t_neighbor)(c, c->T, x, y, z + 1, tc) + SFX(t_neighbor)(c, c->T, x, y, z - 1, tc);
+        const R lap = tsum - K(6.0) * tc;
+        R src = K(0.0);
+        if (src_on && c->in_ch > 9) src = SFX(clampr)(c->src[cell], K(-0.006), K(0.006));
+        const R tn = adv + dt * (c->alpha * lap + src - K(0.0) * adv);
+        c->T_next[cell] = SFX(clampr)(tn, K(-1.0), K(1.0));
+    }
+    R* tmp = c->T;
+    c->T = c->T_next;
+    c->T_next = tmp;
+}
+
+/* ---------------- ingest / init (jni.cpp:1671-1694, 1752-1781) ---------------- */
+static void SFX(ingest)(SFX(Ctx) * c, const float* payload) {
+    const int ch = c->in_ch;
+    for (size_t cell = 0; cell < c->cells; ++cell) {
+        const float* p = payload + cell * (size_t)ch;
+        c->obstacle[cell] = p[0] > 0.5f ? 1 : 0;
+        c->ref_ux[cell] = SFX(finite_or)((R)p[5], K(0.0));
+        c->ref_uy[cell] = SFX(finite_or)((R)p[6], K(0.0));
+        c->ref_uz[cell] = SFX(finite_or)((R)p[7], K(0.0));
+        c->ref_p[cell] = SFX(clampr)((R)p[8], K(-0.03), K(0.03));
+        const R rt = ch > 10 ? (R)p[10] : K(0.0);
+        c->ref_T[cell] = SFX(clampr)(rt, K(-1.0), K(1.0));
+        const R ts = ch > 9 ? (R)p[9] : K(0.0);
+        c->src[cell] = SFX(clampr)(ts, K(-0.006), K(0.006));
+    }
+}
+
+static void SFX(initialize)(SFX(Ctx) * c) {
+    for (size_t cell = 0; cell < c->cells; ++cell) {
+        if (c->obstacle[cell]) {
+            c->rho[cell] = K(1.0);
+            c->ux[cell] = c->uy[cell] = c->uz[cell] = K(0.0);
+        } else {
+            c->rho[cell] = SFX(clampr)(K(1.0) + c->ref_p[cell], K(0.97), K(1.03));
+            c->ux[cell] = c->ref_ux[cell];
+            c->uy[cell] = c->ref_uy[cell];
+            c->uz[cell] = c->ref_uz[cell];
+        }
+        for (int q = 0; q < 27; ++q) {
+            const R e = SFX(feq)(c, q, c->rho[cell], c->ux[cell], c->uy[cell], c->uz[cell]);
+            c->f[(size_t)q * c->cells + cell] = e;
+            c->f_post[(size_t)q * c->cells + cell] = e;
+        }
+        if (RFABS_(c->T[cell]) < K(1e-8)) {
+            c->T[cell] = SFX(clampr)((R)c->cfg.initial_temperature, K(-1.0), K(1.0));
+        } else if (!c->obstacle[cell]) {
+            c->T[cell] = SFX(clampr)(c->ref_T[cell], K(-1.0), K(1.0));
+        }
+        c->T[cell] = SFX(clampr)(c->T[cell], K(-1.0), K(1.0));
+        c->T_next[cell] = c->T[cell];
+    }
+    c->initialized = 1;
+}
+
+/* ---------------- outputs (jni.cpp:2535-2581, 9378-9448) ---------------- */
+static void SFX(cell_output)(const SFX(Ctx) * c, size_t cell, R scale, R out[4]) {
+    out[0] = out[1] = out[2] = out[3] = K(0.0);
+    if (c->obstacle[cell]) return;
+    R rho = K(0.0), ux = K(0.0), uy = K(0.0), uz = K(0.0);
+    for (int q = 0; q < 27; ++q) {
+        const R fq = c->f[(size_t)q * c->cells + cell];
+        if (!RISFINITE_(fq)) return;
+        rho += fq;
+        ux += fq * (R)kCX[q];
+        uy += fq * (R)kCY[q];
+        uz += fq * (R)kCZ[q];
+    }
+    if (!RISFINITE_(rho) || !RISFINITE_(ux) || !RISFINITE_(uy) || !RISFINITE_(uz)) return;
+    rho = SFX(clampr)(rho, K(0.97), K(1.03));
+    if (!RISFINITE_(rho) || rho <= K(1e-6)) return;
+    ux /= rho; uy /= rho; uz /= rho;
+    if (!RISFINITE_(ux) || !RISFINITE_(uy) || !RISFINITE_(uz)) ux = uy = uz = K(0.0);
+    if (RFABS_(rho - K(1.0)) < K(1e-6)) rho = K(1.0);
+    if (RFABS_(ux) < K(1e-7)) ux = K(0.0);
+    if (RFABS_(uy) < K(1e-7)) uy = K(0.0);
+    if (RFABS_(uz) < K(1e-7)) uz = K(0.0);
+    out[0] = ux * scale;
+    out[1] = uy * scale;
+    out[2] = uz * scale;
+    out[3] = SFX(clampr)(rho - K(1.0), K(-0.03), K(0.03));
+}
+
+/* ================= public oracle API ================= */
+ORACLE_API void* SFX(oracle_create)(int nx, int ny, int nz, int in_ch, int out_ch,
+                                    const AeroLbmBenchmarkConfig* cfg, const AeroLbmSolverOverrides* ov) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || in_ch < 9 || out_ch < 4 || !cfg) return NULL;
+    SFX(Ctx)* c = (SFX(Ctx)*)calloc(1, sizeof(SFX(Ctx)));
+    c->nx = nx; c->ny = ny; c->nz = nz;
+    c->cells = (size_t)nx * (size_t)ny * (size_t)nz;
+    c->in_ch = in_ch; c->out_ch = out_ch;
+    c->cfg = *cfg;
+    oracle_sanitize_config(&c->cfg);
+    SFX(init_weights)(c->w);
+    if (!SFX(resolve)(c, ov)) { free(c); return NULL; }
+    const size_t N = c->cells;
+    c->f = (R*)calloc(27 * N, sizeof(R));
+    c->f_post = (R*)calloc(27 * N, sizeof(R));
+    c->rho = (R*)calloc(N, sizeof(R));
+    c->ux = (R*)calloc(N, sizeof(R)); c->uy = (R*)calloc(N, sizeof(R)); c->uz = (R*)calloc(N, sizeof(R));
+    c->ref_ux = (R*)calloc(N, sizeof(R)); c->ref_uy = (R*)calloc(N, sizeof(R)); c->ref_uz = (R*)calloc(N, sizeof(R));
+    c->ref_p = (R*)calloc(N, sizeof(R)); c->ref_T = (R*)calloc(N, sizeof(R)); c->src = (R*)calloc(N, sizeof(R));
+    c->obstacle = (unsigned char*)calloc(N, 1);
+    c->T = (R*)calloc(N, sizeof(R)); c->T_next = (R*)calloc(N, sizeof(R));
+    for (size_t i = 0; i < N; ++i) c->rho[i] = K(1.0);
+    return c;
+}
+
+ORACLE_API void SFX(oracle_destroy)(void* h) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    if (!c) return;
+    free(c->f); free(c->f_post); free(c->rho); free(c->ux); free(c->uy); free(c->uz);
+    free(c->ref_ux); free(c->ref_uy); free(c->ref_uz); free(c->ref_p); free(c->ref_T); free(c->src);
+    free(c->obstacle); free(c->T); free(c->T_next); free(c);
+}
+
+/* jni.cpp:9582-9601 assign_temperature_state */
+ORACLE_API void SFX(oracle_set_temperature)(void* h, const float* t) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    for (size_t i = 0; i < c->cells; ++i) {
+        c->T[i] = SFX(clampr)((R)t[i], K(-1.0), K(1.0));
+        c->T_next[i] = c->T[i];
+    }
+}
+
+ORACLE_API void SFX(oracle_get_temperature)(void* h, float* t) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    for (size_t i = 0; i < c->cells; ++i) t[i] = (float)c->T[i];
+}
+
+/* run_cpu_step (jni.cpp:8480-8507). `packet` may be NULL after the first step,
+ * meaning "same payload as last step" (the cached-step semantics). */
+ORACLE_API int SFX(oracle_step)(void* h, const float* packet, float* out, float out_scale) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    if (packet) SFX(ingest)(c, packet);
+    else if (!c->initialized) return 0;
+    if (!c->initialized) SFX(initialize)(c);
+    SFX(update_temperature)(c);
+    SFX(collide)(c);
+    SFX(stream)(c);
+    if (out) {
+        for (size_t cell = 0; cell < c->cells; ++cell) {
+            R o[4];
+            SFX(cell_output)(c, cell, (R)out_scale, o);
+            for (int k = 0; k < 4; ++k) out[cell * (size_t)c->out_ch + k] = (float)o[k];
+            for (int k = 4; k < c->out_ch; ++k) out[cell * (size_t)c->out_ch + k] = 0.0f;
+        }
+    }
+    c->step_counter += 1;
+    return 1;
+}
+
+ORACLE_API int SFX(oracle_run)(void* h, const float* packet, int steps) {
+    for (int s = 0; s < steps; ++s)
+        if (!SFX(oracle_step)(h, s == 0 ? packet : NULL, NULL, 1.0f)) return 0;
+    return 1;
+}
+
+/* extract_flow_atlas on the CPU path (jni.cpp:9378-9448); stride 1 = get_flow_state */
+ORACLE_API void SFX(oracle_extract_atlas)(void* h, int stride, float* out) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    const int sx = (c->nx + stride - 1) / stride, sy = (c->ny + stride - 1) / stride, sz = (c->nz + stride - 1) / stride;
+    size_t dst = 0;
+    for (int ax = 0; ax < sx; ++ax) {
+        const int gx = (ax * stride < c->nx - 1) ? ax * stride : c->nx - 1;
+        for (int ay = 0; ay < sy; ++ay) {
+            const int gy = (ay * stride < c->ny - 1) ? ay * stride : c->ny - 1;
+            for (int az = 0; az < sz; ++az) {
+                const int gz = (az * stride < c->nz - 1) ? az * stride : c->nz - 1;
+                R o[4];
+                SFX(cell_output)(c, SFX(cell_index)(c, gx, gy, gz), K(1.0), o);
+                for (int k = 0; k < 4; ++k) out[dst + k] = (float)o[k];
+                dst += 4;
+            }
+        }
+    }
+}
+
+/* raw state access for tests: post-stream populations f and macro fields */
+ORACLE_API void SFX(oracle_get_f)(void* h, R* f) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    memcpy(f, c->f, 27 * c->cells * sizeof(R));
+}
+ORACLE_API void SFX(oracle_set_f)(void* h, const R* f) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    memcpy(c->f, f, 27 * c->cells * sizeof(R));
+}
+ORACLE_API void SFX(oracle_get_velocity)(void* h, R* u) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    for (size_t i = 0; i < c->cells; ++i) {
+        u[3 * i] = c->ux[i]; u[3 * i + 1] = c->uy[i]; u[3 * i + 2] = c->uz[i];
+    }
+}
+ORACLE_API void SFX(oracle_get_temperature_r)(void* h, R* t) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    memcpy(t, c->T, c->cells * sizeof(R));
+}
+ORACLE_API void SFX(oracle_get_params)(void* h, R* out6) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    out6[0] = c->tau_shear_base; out6[1] = c->tau_normal_base; out6[2] = c->nu_base;
+    out6[3] = c->alpha; out6[4] = c->beta; out6[5] = c->sgs_c2;
+}
+
+/* Single-cell collision for unit pins: f (27 populations), T, returns f_post.
+ * Uses a 1x1x1 context's parameters (ctx `h`). */
+ORACLE_API void SFX(oracle_collide_cell)(void* h, const R* f_in, R temperature, R* f_out) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    /* operate on cell 0 of the context storage */
+    for (int q = 0; q < 27; ++q) c->f[(size_t)q * c->cells] = f_in[q];
+    c->T[0] = temperature;
+    const unsigned char ob = c->obstacle[0];
+    c->obstacle[0] = 0;
+    /* collide all cells; only cell 0 matters to the caller */
+    SFX(collide)(c);
+    c->obstacle[0] = ob;
+    for (int q = 0; q < 27; ++q) f_out[q] = c->f_post[(size_t)q * c->cells];
+}
+
+/* feq with the oracle's weights (for table/equilibrium pins against the reference core) */
+ORACLE_API R SFX(oracle_feq)(int q, R rho, R ux, R uy, R uz) {
+    SFX(Ctx) c;
+    SFX(init_weights)(c.w);
+    return SFX(feq)(&c, q, rho, ux, uy, uz);
+}
+ORACLE_API void SFX(oracle_weights)(R* w) { SFX(init_weights)(w); }

@@ -1,0 +1,166 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker. The product (libaero_lbm.so and its Python binding) never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_float, c_int, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref_core.so")
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = ctypes.CDLL(LIB)
+    for p, real in (("o32_", c_float), ("o64_", c_double)):
+        R = POINTER(real)
+        getattr(lib, p + "oracle_create").restype = c_void_p
+        getattr(lib, p + "oracle_create").argtypes = [c_int] * 5 + [c_void_p, c_void_p]
+        getattr(lib, p + "oracle_destroy").argtypes = [c_void_p]
+        getattr(lib, p + "oracle_set_temperature").argtypes = [c_void_p, POINTER(c_float)]
+        getattr(lib, p + "oracle_get_temperature").argtypes = [c_void_p, POINTER(c_float)]
+        getattr(lib, p + "oracle_step").argtypes = [c_void_p, POINTER(c_float), POINTER(c_float), c_float]
+        getattr(lib, p + "oracle_step").restype = c_int
+        getattr(lib, p + "oracle_run").argtypes = [c_void_p, POINTER(c_float), c_int]
+        getattr(lib, p + "oracle_run").restype = c_int
+        getattr(lib, p + "oracle_extract_atlas").argtypes = [c_void_p, c_int, POINTER(c_float)]
+        getattr(lib, p + "oracle_get_f").argtypes = [c_void_p, R]
+        getattr(lib, p + "oracle_set_f").argtypes = [c_void_p, R]
+        getattr(lib, p + "oracle_get_velocity").argtypes = [c_void_p, R]
+        getattr(lib, p + "oracle_get_temperature_r").argtypes = [c_void_p, R]
+        getattr(lib, p + "oracle_get_params").argtypes = [c_void_p, R]
+        getattr(lib, p + "oracle_collide_cell").argtypes = [c_void_p, R, real, R]
+        getattr(lib, p + "oracle_feq").argtypes = [c_int, real, real, real, real]
+        getattr(lib, p + "oracle_feq").restype = real
+        getattr(lib, p + "oracle_weights").argtypes = [R]
+    lib.oracle_sanitize_config.argtypes = [c_void_p]
+    _lib = lib
+    return lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+class Oracle:
+    """CPU restatement of the reference step. precision: 32 or 64."""
+
+    def __init__(self, nx, ny, nz, cfg, overrides=None, precision=32, in_ch=11, out_ch=4):
+        self.lib = load()
+        self.p = "o32_" if precision == 32 else "o64_"
+        self.real = np.float32 if precision == 32 else np.float64
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.cells = nx * ny * nz
+        self.out_ch = out_ch
+        ovp = ctypes.cast(ctypes.byref(overrides), c_void_p) if overrides is not None else None
+        self.h = getattr(self.lib, self.p + "oracle_create")(nx, ny, nz, in_ch, out_ch,
+                                                             ctypes.cast(ctypes.byref(cfg), c_void_p), ovp)
+        if not self.h:
+            raise ValueError("oracle does not restate this configuration")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self.p + "oracle_destroy")(self.h)
+            self.h = None
+
+    def _f(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def set_temperature(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        self._f("oracle_set_temperature")(self.h, _fp(t))
+
+    def temperature(self):
+        out = np.zeros(self.cells, dtype=np.float32)
+        self._f("oracle_get_temperature")(self.h, _fp(out))
+        return out
+
+    def temperature_r(self):
+        out = np.zeros(self.cells, dtype=self.real)
+        self._f("oracle_get_temperature_r")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
+        return out
+
+    def step(self, packet=None, out=None, scale=1.0):
+        pk = None if packet is None else _fp(np.ascontiguousarray(packet, dtype=np.float32))
+        ok = self._f("oracle_step")(self.h, pk, None if out is None else _fp(out), scale)
+        if not ok:
+            raise RuntimeError("oracle step failed")
+
+    def run(self, packet, steps):
+        pk = np.ascontiguousarray(packet, dtype=np.float32)
+        if not self._f("oracle_run")(self.h, _fp(pk), steps):
+            raise RuntimeError("oracle run failed")
+
+    def atlas(self, stride=1):
+        sx, sy, sz = (-(-n // stride) for n in (self.nx, self.ny, self.nz))
+        out = np.zeros(sx * sy * sz * 4, dtype=np.float32)
+        self._f("oracle_extract_atlas")(self.h, stride, _fp(out))
+        return out
+
+    def flow(self):
+        return self.atlas(1)
+
+    def populations(self):
+        out = np.zeros(27 * self.cells, dtype=self.real)
+        self._f("oracle_get_f")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
+        return out.reshape(27, self.cells)
+
+    def set_populations(self, f):
+        f = np.ascontiguousarray(f, dtype=self.real).reshape(-1)
+        self._f("oracle_set_f")(self.h, f.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
+
+    def velocity(self):
+        out = np.zeros(3 * self.cells, dtype=self.real)
+        self._f("oracle_get_velocity")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
+        return out.reshape(self.cells, 3)
+
+    def params(self):
+        out = np.zeros(6, dtype=self.real)
+        self._f("oracle_get_params")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
+        return dict(zip(("tau_shear", "tau_normal", "nu", "alpha", "beta", "sgs_c2"), out.tolist()))
+
+    def collide_cell(self, f, temperature=0.0):
+        """Post-collision populations of one fluid cell (context must be 1x1x1)."""
+        ct = np.ctypeslib.as_ctypes_type(self.real)
+        fin = np.ascontiguousarray(f, dtype=self.real)
+        out = np.zeros(27, dtype=self.real)
+        self._f("oracle_collide_cell")(self.h, fin.ctypes.data_as(POINTER(ct)), temperature,
+                                       out.ctypes.data_as(POINTER(ct)))
+        return out
+
+
+def load_ref():
+    """The reference's own hydro/thermal core (oracle/_ref), or None when absent."""
+    if not os.path.exists(REF_LIB):
+        return None
+    lib = ctypes.CDLL(REF_LIB)
+    lib.ref_d3q27_tables.argtypes = [POINTER(c_int)] * 4 + [POINTER(c_float)]
+    lib.ref_feq_d3q27.restype = c_float
+    lib.ref_feq_d3q27.argtypes = [c_int] + [c_float] * 4
+    lib.ref_feq_d3q7.restype = c_float
+    lib.ref_feq_d3q7.argtypes = [c_int] + [c_float] * 4
+    lib.ref_derive_transport.argtypes = [c_int] * 3 + [c_float] * 5 + [POINTER(c_float)]
+    lib.ref_boussinesq_beta.restype = c_float
+    lib.ref_boussinesq_beta.argtypes = [c_float] * 5
+    lib.ref_lattice_diffusivity.restype = c_float
+    lib.ref_lattice_diffusivity.argtypes = [c_float] * 3
+    return lib
