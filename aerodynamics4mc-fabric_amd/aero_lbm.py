@@ -28,6 +28,7 @@ SOLVER_CLASSIC_D3Q27 = 1
 SOLVER_COMPACT_EXPERIMENTAL = 2
 SOLVER_D3Q27_FP16_INPLACE = 3
 SOLVER_D3Q27_BF16_INPLACE = 4  # extension
+SOLVER_D3Q27_FP32_INPLACE = 5  # extension
 
 PRESET_NONE = 0
 PRESET_TAYLOR_GREEN_3D = 1

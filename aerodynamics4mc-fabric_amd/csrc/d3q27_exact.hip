@@ -1,0 +1,709 @@
+// Exact FP32 A-B path: one thread per cell, fused pull-stream + temperature
+// update + cumulant collision, in the operation order of the reference's CPU
+// step so FP32 results match it bit for bit.
+//
+// Compiled with -ffp-contract=off (no FMA contraction) and IEEE division and
+// square root, like the reference's g++ x86-64 build. Terms the reference adds
+// that are structurally zero (a population times a zero lattice coefficient)
+// are left out: the accumulators start at +0.0f and can never become -0.0f, so
+// adding a signed zero never changes them.
+//
+// Step semantics (ref: native/src/aero_lbm_jni.cpp:8480-8507): the reference
+// runs temperature update -> collide -> stream per step and keeps post-stream
+// populations f. This kernel keeps post-collision populations f* instead: a
+// step gathers f_n = stream(f*_n) (pull, with the reference's link rules), then
+// updates T and collides to f*_{n+1}. The very first step after initialisation
+// collides the initial populations in place (local_mode = 1). Outputs gather
+// f_n on the fly (launch_exact_readout), which is how the reference's
+// write_output sees post-stream populations.
+#include "kernels.h"
+
+namespace aero {
+namespace exact {
+
+struct W27 {
+    float w[27];
+};
+__host__ __device__ constexpr W27 make_weights() {
+    W27 r{};
+    const float w1[3] = {1.0f / 6.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+    int q = 0;
+    for (int ix = 0; ix < 3; ++ix)
+        for (int iy = 0; iy < 3; ++iy)
+            for (int iz = 0; iz < 3; ++iz) r.w[q++] = w1[ix] * w1[iy] * w1[iz];
+    return r;
+}
+__device__ constexpr W27 kW = make_weights();
+
+__device__ __forceinline__ float rmax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float rmin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float clampr(float v, float lo, float hi) { return rmin(hi, rmax(lo, v)); }
+__device__ __forceinline__ bool fin(float v) { return __builtin_isfinite(v); }
+
+// ref: native/src/aero_lbm_hydro_core.cpp:364-368
+__device__ __forceinline__ float feq(int q, float rho, float ux, float uy, float uz) {
+    const float cu = 3.0f * ((float)lat_cx(q) * ux + (float)lat_cy(q) * uy + (float)lat_cz(q) * uz);
+    const float uu = ux * ux + uy * uy + uz * uz;
+    return kW.w[q] * rho * (1.0f + cu + 0.5f * cu * cu - 1.5f * uu);
+}
+
+__device__ __forceinline__ int cell_of(const StepParams& P, int x, int y, int z) { return (x * P.ny + y) * P.nz + z; }
+
+__device__ __forceinline__ bool wrap_hydro(const StepParams& P, int& sx, int& sy, int& sz) {
+    if (sx < 0) { if (P.hydro_wrap_lo[0]) sx += P.nx; }
+    else if (sx >= P.nx) { if (P.hydro_wrap_hi[0]) sx -= P.nx; }
+    if (sy < 0) { if (P.hydro_wrap_lo[1]) sy += P.ny; }
+    else if (sy >= P.ny) { if (P.hydro_wrap_hi[1]) sy -= P.ny; }
+    if (sz < 0) { if (P.hydro_wrap_lo[2]) sz += P.nz; }
+    else if (sz >= P.nz) { if (P.hydro_wrap_hi[2]) sz -= P.nz; }
+    return sx >= 0 && sy >= 0 && sz >= 0 && sx < P.nx && sy < P.ny && sz < P.nz;
+}
+
+// index of the face the reference picks for an out-of-domain source (ref: jni.cpp:743-751)
+__device__ __forceinline__ int oob_face(const StepParams& P, int x, int y, int z) {
+    if (x < 0) return 0;
+    if (x >= P.nx) return 1;
+    if (y < 0) return 2;
+    if (y >= P.ny) return 3;
+    if (z < 0) return 4;
+    return 5;
+}
+
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ FaceParams face_at(const StepParams& P, int i) { return P.faces_dev[i]; }
+
+// ---------------------------------------------------------------- link rules
+// All take the "previous post-collision" buffer fp (f*), read-only in a step.
+struct Src {
+    const float* fp;
+    const uint8_t* flags;
+    const float* ref_p;
+    size_t N;
+};
+
+__device__ __forceinline__ float bounce(const Src& s, int cell, int q) {
+    // kObstacleBounceBlend = 0 (ref: jni.cpp:140, 1475-1492)
+    return s.fp[(size_t)lat_opp(q) * s.N + cell];
+}
+
+// ref: jni.cpp:1494-1516
+__device__ __forceinline__ float convective(const StepParams& P, const Src& s, int cell, int x, int y, int z, int q, int sx, int sy,
+                            int sz) {
+    const int dx = (sx < 0 || sx >= P.nx) ? lat_cx(q) : 0;
+    const int dy = (sy < 0 || sy >= P.ny) ? lat_cy(q) : 0;
+    const int dz = (sz < 0 || sz >= P.nz) ? lat_cz(q) : 0;
+    const int s1 = cell_of(P, iclamp(x + dx, 0, P.nx - 1), iclamp(y + dy, 0, P.ny - 1), iclamp(z + dz, 0, P.nz - 1));
+    if (s.flags[s1] & kCellSolid) return bounce(s, cell, q);
+    const float f1 = s.fp[(size_t)q * s.N + s1];
+    const int s2 =
+        cell_of(P, iclamp(x + 2 * dx, 0, P.nx - 1), iclamp(y + 2 * dy, 0, P.ny - 1), iclamp(z + 2 * dz, 0, P.nz - 1));
+    if (s.flags[s2] & kCellSolid) return f1;
+    const float f2 = s.fp[(size_t)q * s.N + s2];
+    return f1 + 0.15f * (f1 - f2);
+}
+
+// ref: jni.cpp:1518-1561 (cylinder preset is not supported by this build)
+__device__ __forceinline__ float equilibrium_face(const StepParams& P, const Src& s, int cell, int q, const FaceParams& fc) {
+    float rho = 1.0f + clampr(s.ref_p[cell], -0.03f, 0.03f);
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (fc.hydro_kind == 3 || fc.hydro_kind == 4) {
+        ux = fc.vel[0];
+        uy = fc.vel[1];
+        uz = fc.vel[2];
+    } else if (fc.hydro_kind == 5) {
+        rho = 1.0f + clampr(fc.pressure, -0.03f, 0.03f);
+    }
+    rho = clampr(rho, 0.97f, 1.03f);
+    const float s2 = ux * ux + uy * uy + uz * uz;
+    if (s2 > P.kmax_speed * P.kmax_speed && s2 > 0.0f) {
+        const float sc = P.kmax_speed / __builtin_sqrtf(s2);
+        ux *= sc;
+        uy *= sc;
+        uz *= sc;
+    }
+    return feq(q, rho, ux, uy, uz);
+}
+
+// ref: jni.cpp:1563-1602
+__device__ __forceinline__ float face_value(const StepParams& P, const Src& s, int cell, int x, int y, int z, int q, int sx, int sy,
+                            int sz) {
+    const FaceParams fc = face_at(P, oob_face(P, sx, sy, sz));
+    switch (fc.hydro_kind) {
+        case 2:
+            return bounce(s, cell, q);
+        case 3:
+        case 4:
+        case 5:
+            return equilibrium_face(P, s, cell, q, fc);
+        case 7: {
+            const int src = cell_of(P, iclamp(sx, 0, P.nx - 1), iclamp(sy, 0, P.ny - 1), iclamp(sz, 0, P.nz - 1));
+            return (s.flags[src] & kCellSolid) ? bounce(s, cell, q) : s.fp[(size_t)q * s.N + src];
+        }
+        case 6:
+        case 1:
+            return convective(P, s, cell, x, y, z, q, sx, sy, sz);
+        default:
+            if (P.convective_disabled) return bounce(s, cell, q);
+            return convective(P, s, cell, x, y, z, q, sx, sy, sz);
+    }
+}
+
+// f_n at one cell: pull of f* with the reference's link rules (jni.cpp:2415-2452)
+__device__ __forceinline__ void gather(const StepParams& P, const Src& s, int cell, int x, int y, int z, bool boundary,
+                                       float f[27]) {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        int sx = x - lat_cx(q), sy = y - lat_cy(q), sz = z - lat_cz(q);
+        const bool in = wrap_hydro(P, sx, sy, sz);
+        if (!boundary) {
+            f[q] = s.fp[(size_t)q * s.N + cell_of(P, sx, sy, sz)];
+        } else if (!in) {
+            f[q] = face_value(P, s, cell, x, y, z, q, sx, sy, sz);
+        } else {
+            const int src = cell_of(P, sx, sy, sz);
+            f[q] = (s.flags[src] & kCellSolid) ? bounce(s, cell, q) : s.fp[(size_t)q * s.N + src];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- temperature (jni.cpp:1799-1830, 1910-1945, 2145-2188)
+__device__ __forceinline__ float t_neighbor(const StepParams& P, const float* T, const uint8_t* flags, int x, int y,
+                                            int z, float self) {
+    // periodic wrap as in the reference (coord %= dim, then + dim if negative)
+    if ((x < 0 && P.thermal_wrap_lo[0]) || (x >= P.nx && P.thermal_wrap_hi[0])) { x %= P.nx; if (x < 0) x += P.nx; }
+    if ((y < 0 && P.thermal_wrap_lo[1]) || (y >= P.ny && P.thermal_wrap_hi[1])) { y %= P.ny; if (y < 0) y += P.ny; }
+    if ((z < 0 && P.thermal_wrap_lo[2]) || (z >= P.nz && P.thermal_wrap_hi[2])) { z %= P.nz; if (z < 0) z += P.nz; }
+    if (x < 0 || y < 0 || z < 0 || x >= P.nx || y >= P.ny || z >= P.nz) {
+        const FaceParams fc = face_at(P, oob_face(P, x, y, z));
+        if (fc.thermal_kind == 4) return clampr(fc.temperature, -1.0f, 1.0f);
+        if (fc.thermal_kind == 5) return clampr(self + fc.heat_flux, -1.0f, 1.0f);
+        return self;
+    }
+    const int c = cell_of(P, x, y, z);
+    if (flags[c] & kCellSolid) return self;
+    return T[c];
+}
+
+__device__ __forceinline__ float update_temperature(const StepParams& P, const float* T, const uint8_t* flags, const float* src, int cell,
+                                    int x, int y, int z, float ux, float uy, float uz) {
+    const float dt = 1.0f;
+    const float tc = T[cell];
+    const float px = (float)x - dt * ux, py = (float)y - dt * uy, pz = (float)z - dt * uz;
+    const int x0 = (int)floorf(px), y0 = (int)floorf(py), z0 = (int)floorf(pz);
+    const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+    const float fx = px - (float)x0, fy = py - (float)y0, fz = pz - (float)z0;
+    const float c000 = t_neighbor(P, T, flags, x0, y0, z0, tc);
+    const float c100 = t_neighbor(P, T, flags, x1, y0, z0, tc);
+    const float c010 = t_neighbor(P, T, flags, x0, y1, z0, tc);
+    const float c110 = t_neighbor(P, T, flags, x1, y1, z0, tc);
+    const float c001 = t_neighbor(P, T, flags, x0, y0, z1, tc);
+    const float c101 = t_neighbor(P, T, flags, x1, y0, z1, tc);
+    const float c011 = t_neighbor(P, T, flags, x0, y1, z1, tc);
+    const float c111 = t_neighbor(P, T, flags, x1, y1, z1, tc);
+    const float c00 = c000 + (c100 - c000) * fx;
+    const float c10 = c010 + (c110 - c010) * fx;
+    const float c01 = c001 + (c101 - c001) * fx;
+    const float c11 = c011 + (c111 - c011) * fx;
+    const float c0 = c00 + (c10 - c00) * fy;
+    const float c1 = c01 + (c11 - c01) * fy;
+    const float adv = c0 + (c1 - c0) * fz;
+    const float tsum = t_neighbor(P, T, flags, x + 1, y, z, tc) + t_neighbor(P, T, flags, x - 1, y, z, tc) +
+                       t_neighbor(P, T, flags, x, y + 1, z, tc) + t_neighbor(P, T, flags, x, y - 1, z, tc) +
+                       t_neighbor(P, T, flags, x, y, z + 1, tc) + t_neighbor(P, T, flags, x, y, z - 1, tc);
+    const float lap = tsum - 6.0f * tc;
+    float source = 0.0f;
+    if (P.source_on && P.in_ch > 9) source = clampr(src[cell], -0.006f, 0.006f);
+    const float tn = adv + dt * (P.alpha * lap + source - 0.0f * adv);
+    return clampr(tn, -1.0f, 1.0f);
+}
+
+// ---------------------------------------------------------------- cumulant collision (jni.cpp:1335-1473, 2209-2413)
+template <int A, int B, int C>
+__device__ __forceinline__ float raw_moment(const float f[27]) {
+    // raw[a][b][c] = sum_q f_q * cx^a * cy^b * cz^c in q order; zero coefficients dropped
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const int px = A == 0 ? 1 : (A == 1 ? lat_cx(q) : lat_cx(q) * lat_cx(q));
+        const int py = B == 0 ? 1 : (B == 1 ? lat_cy(q) : lat_cy(q) * lat_cy(q));
+        const int pz = C == 0 ? 1 : (C == 1 ? lat_cz(q) : lat_cz(q) * lat_cz(q));
+        if (px * py * pz != 0) s += f[q] * (float)px * (float)py * (float)pz;
+    }
+    return s;
+}
+
+__device__ __forceinline__ int binom2(int n, int k) { return (n <= 1 || k == 0 || k == n) ? 1 : 2; }
+
+__device__ __forceinline__ void raw_moments(const float f[27], float r[27]) {
+#define RM(a, b, c) r[(a * 3 + b) * 3 + c] = raw_moment<a, b, c>(f);
+    RM(0, 0, 0) RM(0, 0, 1) RM(0, 0, 2) RM(0, 1, 0) RM(0, 1, 1) RM(0, 1, 2) RM(0, 2, 0) RM(0, 2, 1) RM(0, 2, 2)
+    RM(1, 0, 0) RM(1, 0, 1) RM(1, 0, 2) RM(1, 1, 0) RM(1, 1, 1) RM(1, 1, 2) RM(1, 2, 0) RM(1, 2, 1) RM(1, 2, 2)
+    RM(2, 0, 0) RM(2, 0, 1) RM(2, 0, 2) RM(2, 1, 0) RM(2, 1, 1) RM(2, 1, 2) RM(2, 2, 0) RM(2, 2, 1) RM(2, 2, 2)
+#undef RM
+}
+
+// central[a][b][c] = sum_{p<=a,q<=b,r<=c} C(a,p)C(b,q)C(c,r) s_x^(a-p) s_y^(b-q) s_z^(c-r) m[p][q][r],
+// each term evaluated ((((coeff*sx)*sy)*sz)*m) as the reference does.
+__device__ __forceinline__ void shift_moments(const float in[27], const float sx[3], const float sy[3], const float sz[3],
+                                              float out[27]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float sum = 0.0f;
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            // constant trip counts so every index folds at compile time
+                            if (p <= a && q <= b && r <= c) {
+                                const float coeff = (float)(binom2(a, p) * binom2(b, q) * binom2(c, r));
+                                sum += coeff * sx[a - p] * sy[b - q] * sz[c - r] * in[(p * 3 + q) * 3 + r];
+                            }
+                        }
+                out[(a * 3 + b) * 3 + c] = sum;
+            }
+}
+
+__device__ __forceinline__ float inv1d(int i, int a) {
+    // kMomentInv1D (ref: jni.cpp:46-50): rows c=-1,0,+1; columns moment order 0,1,2
+    return i == 0 ? (a == 0 ? 0.0f : (a == 1 ? -0.5f : 0.5f))
+                  : (i == 1 ? (a == 0 ? 1.0f : (a == 1 ? 0.0f : -1.0f)) : (a == 0 ? 0.0f : 0.5f));
+}
+
+__device__ __forceinline__ void raw_to_pop(const float m[27], float f[27]) {
+#pragma unroll
+    for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+        for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+            for (int iz = 0; iz < 3; ++iz) {
+                float sum = 0.0f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const float k = inv1d(ix, a) * inv1d(iy, b) * inv1d(iz, c);
+                            if (inv1d(ix, a) != 0.0f && inv1d(iy, b) != 0.0f && inv1d(iz, c) != 0.0f)
+                                sum += k * m[(a * 3 + b) * 3 + c];
+                        }
+                f[(ix * 3 + iy) * 3 + iz] = sum;
+            }
+}
+
+__device__ __forceinline__ float tau_from_nu(float nu) {
+    const float nu_max = (0.95f - 0.5f) / 3.0f;
+    return 0.5f + 3.0f * clampr(nu, 1.0e-8f, nu_max);
+}
+
+struct CollideOut {
+    float ux, uy, uz;
+    bool reseeded;
+};
+
+__device__ __forceinline__ void reseed(const StepParams& P, const CellFields& C, int cell, float fo[27], CollideOut& o) {
+    // ref: jni.cpp:1724-1750
+    const size_t N = (size_t)P.cells;
+    const float p = clampr(fin(C.ref_p[cell]) ? C.ref_p[cell] : 0.0f, -0.03f, 0.03f);
+    const float rho = clampr(1.0f + p, 0.97f, 1.03f);
+    float ux = fin(C.ref_u[cell]) ? C.ref_u[cell] : 0.0f;
+    float uy = fin(C.ref_u[N + cell]) ? C.ref_u[N + cell] : 0.0f;
+    float uz = fin(C.ref_u[2 * N + cell]) ? C.ref_u[2 * N + cell] : 0.0f;
+    const float s2 = ux * ux + uy * uy + uz * uz;
+    if (!fin(s2) || s2 > P.kmax_speed * P.kmax_speed) {
+        if (!fin(s2) || s2 <= 0.0f) {
+            ux = uy = uz = 0.0f;
+        } else {
+            const float sc = P.kmax_speed / __builtin_sqrtf(s2);
+            ux *= sc;
+            uy *= sc;
+            uz *= sc;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) fo[q] = feq(q, rho, ux, uy, uz);
+    o.ux = ux;
+    o.uy = uy;
+    o.uz = uz;
+    o.reseeded = true;
+}
+
+__device__ __forceinline__ void collide(const StepParams& P, const CellFields& C, int cell, int x, int y, int z, const float fi[27],
+                        float T, float fo[27], CollideOut& o) {
+    o.reseeded = false;
+    float rho = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        if (!fin(fi[q])) bad = true;
+    }
+    if (!bad) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            rho += fi[q];
+            if (lat_cx(q) != 0) mx += fi[q] * (float)lat_cx(q);
+            if (lat_cy(q) != 0) my += fi[q] * (float)lat_cy(q);
+            if (lat_cz(q) != 0) mz += fi[q] * (float)lat_cz(q);
+        }
+    }
+    if (bad || !fin(rho) || !fin(mx) || !fin(my) || !fin(mz)) { reseed(P, C, cell, fo, o); return; }
+    const float rho_safe = rmax(1e-6f, rho);
+    const float inv_rho = 1.0f / rho_safe;
+    float ux = mx * inv_rho, uy = my * inv_rho, uz = mz * inv_rho;
+    if (!fin(rho_safe) || !fin(inv_rho) || !fin(ux) || !fin(uy) || !fin(uz)) { reseed(P, C, cell, fo, o); return; }
+    // buoyancy along the gravity axis; the other components stay +0.0f as in the reference
+    float F[3] = {0.0f, 0.0f, 0.0f};
+    if (P.buoyancy_on) {
+        const float b = clampr(P.beta * T, -0.02f, 0.02f);
+        const float fa = 0.0f + (P.g_sign < 0 ? rho_safe * b : rho_safe * (-b));
+        F[0] = P.g_axis == 0 ? fa : 0.0f;
+        F[1] = P.g_axis == 1 ? fa : 0.0f;
+        F[2] = P.g_axis == 2 ? fa : 0.0f;
+    }
+    ux += 0.5f * (F[0] * inv_rho);
+    uy += 0.5f * (F[1] * inv_rho);
+    uz += 0.5f * (F[2] * inv_rho);
+    // benchmark mode: the state nudge is zero and adds nothing (ref: jni.cpp:679-681, 2306-2314)
+    if (!fin(ux) || !fin(uy) || !fin(uz)) { reseed(P, C, cell, fo, o); return; }
+    const float s2 = ux * ux + uy * uy + uz * uz;
+    if (!fin(s2)) { reseed(P, C, cell, fo, o); return; }
+    if (s2 > P.kmax_speed * P.kmax_speed) {
+        const float sc = P.kmax_speed / __builtin_sqrtf(s2);
+        ux *= sc;
+        uy *= sc;
+        uz *= sc;
+    }
+    o.ux = ux;
+    o.uy = uy;
+    o.uz = uz;
+
+    float m[27], c[27];
+    raw_moments(fi, m);
+    {
+        const float sx[3] = {1.0f, -ux, ux * ux};
+        const float sy[3] = {1.0f, -uy, uy * uy};
+        const float sz[3] = {1.0f, -uz, uz * uz};
+        shift_moments(m, sx, sy, sz, c);
+    }
+    const float cs2 = 1.0f / 3.0f;
+    float ts = P.tau_shear_base, tn = P.tau_normal_base;
+    if (P.sgs_on) {
+        const float nu0 = rmax(1e-6f, P.nu_base);
+        const float nxx = c[18] - rho_safe * cs2;  // c200
+        const float nyy = c[6] - rho_safe * cs2;   // c020
+        const float nzz = c[2] - rho_safe * cs2;   // c002
+        const float nxy = c[12], nxz = c[10], nyz = c[4];
+        const float qn2 = nxx * nxx + nyy * nyy + nzz * nzz + 2.0f * (nxy * nxy + nxz * nxz + nyz * nyz);
+        const float qmag = __builtin_sqrtf(rmax(0.0f, qn2));
+        const float smag = qmag / rmax(1e-6f, 2.0f * rho_safe * nu0);
+        float nut = P.sgs_c2 * smag;
+        nut = rmin(nut, P.nut_cap_ratio * nu0);
+        const float tau_min = 0.5f + 3.0f * 1.0e-8f;
+        ts = clampr(tau_from_nu(nu0 + nut), tau_min, 0.95f);
+        tn = clampr(P.tau_normal_base + (ts - P.tau_shear_base) * 0.30f, tau_min, 0.95f);
+    }
+    const float om_diag = 1.0f / tn;
+    const float om_off = 1.0f / ts;
+    // relaxation + Gaussian closure (ref: jni.cpp:1384-1430); output reuses m[]
+    {
+        const float rr = rho_safe;
+        const float eq2 = rr * cs2;
+        const float sd = clampr(om_diag, 0.0f, 1.95f);
+        const float so = clampr(om_off, 0.0f, 1.95f);
+        const float c200 = c[18] + sd * (eq2 - c[18]);
+        const float c020 = c[6] + sd * (eq2 - c[6]);
+        const float c002 = c[2] + sd * (eq2 - c[2]);
+        const float c110 = (1.0f - so) * c[12];
+        const float c101 = (1.0f - so) * c[10];
+        const float c011 = (1.0f - so) * c[4];
+        const float ir = 1.0f / rmax(1e-6f, rr);
+#pragma unroll
+        for (int k = 0; k < 27; ++k) c[k] = 0.0f;
+        c[0] = rr;
+        c[18] = c200;
+        c[6] = c020;
+        c[2] = c002;
+        c[12] = c110;
+        c[10] = c101;
+        c[4] = c011;
+        c[24] = c200 * c020 * ir + 2.0f * c110 * c110 * ir;                       // c220
+        c[20] = c200 * c002 * ir + 2.0f * c101 * c101 * ir;                       // c202
+        c[8] = c020 * c002 * ir + 2.0f * c011 * c011 * ir;                        // c022
+        c[22] = (c200 * c011 + 2.0f * c110 * c101) * ir;                          // c211
+        c[16] = (c020 * c101 + 2.0f * c110 * c011) * ir;                          // c121
+        c[14] = (c002 * c110 + 2.0f * c101 * c011) * ir;                          // c112
+        c[26] = (c200 * c020 * c002 + 2.0f * c110 * c110 * c002 + 2.0f * c101 * c101 * c020 +
+                 2.0f * c011 * c011 * c200 + 8.0f * c110 * c101 * c011) * ir * ir;  // c222
+    }
+    {
+        const float px[3] = {1.0f, ux, ux * ux};
+        const float py[3] = {1.0f, uy, uy * uy};
+        const float pz[3] = {1.0f, uz, uz * uz};
+        shift_moments(c, px, py, pz, m);
+    }
+    raw_to_pop(m, fo);
+    // exact low-order conservation after inversion (ref: jni.cpp:2376-2394)
+    float rc = 0.0f, mxc = 0.0f, myc = 0.0f, mzc = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        rc += fo[q];
+        if (lat_cx(q) != 0) mxc += fo[q] * (float)lat_cx(q);
+        if (lat_cy(q) != 0) myc += fo[q] * (float)lat_cy(q);
+        if (lat_cz(q) != 0) mzc += fo[q] * (float)lat_cz(q);
+    }
+    const float rcs = rmax(1e-6f, rc);
+    const float uxc = mxc / rcs, uyc = myc / rcs, uzc = mzc / rcs;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) fo[q] += feq(q, rho_safe, ux, uy, uz) - feq(q, rcs, uxc, uyc, uzc);
+    // Guo source (ref: jni.cpp:1270-1281); identically +-0 when F = 0
+    if (F[0] != 0.0f || F[1] != 0.0f || F[2] != 0.0f) {
+        const float inv_cs2 = 1.0f / (1.0f / 3.0f);
+        const float inv_cs4 = inv_cs2 * inv_cs2;
+        const float pref = rmax(0.0f, 1.0f - 0.5f * clampr(om_off, 0.0f, 1.999f));
+        const float uf = ux * F[0] + uy * F[1] + uz * F[2];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const float cx = (float)lat_cx(q), cy = (float)lat_cy(q), cz = (float)lat_cz(q);
+            const float cu = cx * ux + cy * uy + cz * uz;
+            const float cf = cx * F[0] + cy * F[1] + cz * F[2];
+            fo[q] = fo[q] + pref * kW.w[q] * ((cf - uf) * inv_cs2 + cu * cf * inv_cs4);
+        }
+    }
+    if (P.sponge_on) {
+        // ref: jni.cpp:767-774, 2401-2411
+        int d = x;
+        d = y < d ? y : d;
+        d = z < d ? z : d;
+        d = (P.nx - 1 - x) < d ? (P.nx - 1 - x) : d;
+        d = (P.ny - 1 - y) < d ? (P.ny - 1 - y) : d;
+        d = (P.nz - 1 - z) < d ? (P.nz - 1 - z) : d;
+        if (d < 4) {
+            const float eta = (float)(4 - d) / (float)4;
+            const float al = clampr(0.03f * eta * eta, 0.0f, 0.95f);
+            if (al > 0.0f) {
+                const float keep = 1.0f - al;
+#pragma unroll
+                for (int q = 0; q < 27; ++q) fo[q] = keep * fo[q] + al * feq(q, 1.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void __launch_bounds__(256) init_kernel(StepParams P, CellFields C, float* __restrict__ f,
+                                                   float* __restrict__ T, float* __restrict__ u_state, float t_init) {
+    // ref: jni.cpp:1752-1781 (benchmark mode)
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.cells) return;
+    const size_t N = (size_t)P.cells;
+    const bool solid = C.flags[cell] & kCellSolid;
+    float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (!solid) {
+        rho = clampr(1.0f + C.ref_p[cell], 0.97f, 1.03f);
+        ux = C.ref_u[cell];
+        uy = C.ref_u[N + cell];
+        uz = C.ref_u[2 * N + cell];
+    }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[(size_t)q * N + cell] = feq(q, rho, ux, uy, uz);
+    float t = T[cell];
+    if (fabsf(t) < 1e-8f) t = clampr(t_init, -1.0f, 1.0f);
+    else if (!solid) t = clampr(C.ref_t[cell], -1.0f, 1.0f);
+    T[cell] = clampr(t, -1.0f, 1.0f);
+    u_state[cell] = ux;
+    u_state[N + cell] = uy;
+    u_state[2 * N + cell] = uz;
+}
+
+__global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const float* __restrict__ fin,
+                                                   float* __restrict__ fout, const float* __restrict__ t_in,
+                                                   float* __restrict__ t_out, float* __restrict__ u_state,
+                                                   int local_mode) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.cells) return;
+    const size_t N = (size_t)P.cells;
+    const uint8_t fl = C.flags[cell];
+    if (fl & kCellSolid) {
+        // collide copies, stream swaps to the opposite slot (ref: jni.cpp:2212-2218, 2421-2424)
+#pragma unroll
+        for (int q = 0; q < 27; ++q) fout[(size_t)q * N + cell] = fin[(size_t)(local_mode ? q : lat_opp(q)) * N + cell];
+        if (P.thermal_on) t_out[cell] = 0.0f;
+        u_state[cell] = 0.0f;
+        u_state[N + cell] = 0.0f;
+        u_state[2 * N + cell] = 0.0f;
+        return;
+    }
+    const int yz = P.ny * P.nz;
+    const int x = cell / yz;
+    const int rem = cell - x * yz;
+    const int y = rem / P.nz;
+    const int z = rem - y * P.nz;
+    float f[27];
+    if (local_mode) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) f[q] = fin[(size_t)q * N + cell];
+    } else {
+        const Src s{fin, C.flags, C.ref_p, N};
+        gather(P, s, cell, x, y, z, (fl & kCellBoundary) != 0, f);
+    }
+    float T = 0.0f;
+    if (P.thermal_on) {
+        T = update_temperature(P, t_in, C.flags, C.src, cell, x, y, z, u_state[cell], u_state[N + cell],
+                               u_state[2 * N + cell]);
+        t_out[cell] = T;
+    }
+    float fo[27];
+    CollideOut o;
+    collide(P, C, cell, x, y, z, f, T, fo, o);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) fout[(size_t)q * N + cell] = fo[q];
+    u_state[cell] = o.ux;
+    u_state[N + cell] = o.uy;
+    u_state[2 * N + cell] = o.uz;
+}
+
+// Stream only: f_n = stream(f*_n) into fout. Used when the payload changes between
+// steps, so the pending stream runs with the payload the reference streamed with.
+__global__ void __launch_bounds__(256) stream_kernel(StepParams P, CellFields C, const float* __restrict__ fin,
+                                                     float* __restrict__ fout) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.cells) return;
+    const size_t N = (size_t)P.cells;
+    const uint8_t fl = C.flags[cell];
+    if (fl & kCellSolid) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) fout[(size_t)q * N + cell] = fin[(size_t)lat_opp(q) * N + cell];
+        return;
+    }
+    const int yz = P.ny * P.nz;
+    const int x = cell / yz;
+    const int rem = cell - x * yz;
+    const int y = rem / P.nz;
+    const int z = rem - y * P.nz;
+    float f[27];
+    const Src s{fin, C.flags, C.ref_p, N};
+    gather(P, s, cell, x, y, z, (fl & kCellBoundary) != 0, f);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) fout[(size_t)q * N + cell] = f[q];
+}
+
+// moments of f_n -> (u*scale, clamp(rho-1)) with the reference's output rules (jni.cpp:2535-2581)
+__device__ __forceinline__ void cell_output(const float f[27], float scale, float o[4]) {
+    o[0] = o[1] = o[2] = o[3] = 0.0f;
+    float rho = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+        if (!fin(f[q])) return;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        rho += f[q];
+        if (lat_cx(q) != 0) ux += f[q] * (float)lat_cx(q);
+        if (lat_cy(q) != 0) uy += f[q] * (float)lat_cy(q);
+        if (lat_cz(q) != 0) uz += f[q] * (float)lat_cz(q);
+    }
+    if (!fin(rho) || !fin(ux) || !fin(uy) || !fin(uz)) return;
+    rho = clampr(rho, 0.97f, 1.03f);
+    if (!fin(rho) || rho <= 1e-6f) return;
+    ux /= rho;
+    uy /= rho;
+    uz /= rho;
+    if (!fin(ux) || !fin(uy) || !fin(uz)) ux = uy = uz = 0.0f;
+    if (fabsf(rho - 1.0f) < 1e-6f) rho = 1.0f;
+    if (fabsf(ux) < 1e-7f) ux = 0.0f;
+    if (fabsf(uy) < 1e-7f) uy = 0.0f;
+    if (fabsf(uz) < 1e-7f) uz = 0.0f;
+    o[0] = ux * scale;
+    o[1] = uy * scale;
+    o[2] = uz * scale;
+    o[3] = clampr(rho - 1.0f, -0.03f, 0.03f);
+}
+
+__device__ __forceinline__ void output_at(const StepParams& P, const CellFields& C, const float* fp, int post_collision, int x, int y,
+                          int z, float scale, float o[4]) {
+    const int cell = cell_of(P, x, y, z);
+    const uint8_t fl = C.flags[cell];
+    if (fl & kCellSolid) {
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        return;
+    }
+    float f[27];
+    if (post_collision) {
+        const Src s{fp, C.flags, C.ref_p, (size_t)P.cells};
+        gather(P, s, cell, x, y, z, (fl & kCellBoundary) != 0, f);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) f[q] = fp[(size_t)q * P.cells + cell];
+    }
+    cell_output(f, scale, o);
+}
+
+__global__ void __launch_bounds__(256) readout_kernel(StepParams P, CellFields C, const float* __restrict__ fp,
+                                                      int post_collision, int stride, int sx, int sy, int sz,
+                                                      int out_ch, float scale, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sx * sy * sz) return;
+    const int ax = i / (sy * sz);
+    const int rem = i - ax * sy * sz;
+    const int ay = rem / sz;
+    const int az = rem - ay * sz;
+    const int gx = min(P.nx - 1, ax * stride), gy = min(P.ny - 1, ay * stride), gz = min(P.nz - 1, az * stride);
+    float o[4];
+    output_at(P, C, fp, post_collision, gx, gy, gz, scale, o);
+    float* dst = out + (size_t)i * out_ch;
+    dst[0] = o[0];
+    dst[1] = o[1];
+    dst[2] = o[2];
+    dst[3] = o[3];
+    for (int k = 4; k < out_ch; ++k) dst[k] = 0.0f;
+}
+
+__global__ void sample_kernel(StepParams P, CellFields C, const float* __restrict__ fp, int post_collision, int x,
+                              int y, int z, float* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float o[4];
+    output_at(P, C, fp, post_collision, x, y, z, 1.0f, o);
+    out[0] = o[0];
+    out[1] = o[1];
+    out[2] = o[2];
+    out[3] = o[3];
+}
+
+}  // namespace exact
+
+static inline dim3 grid1(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
+void launch_exact_init(const StepParams& P, const CellFields& C, float* f, float* t, float* u_state,
+                       float initial_temperature, hipStream_t s) {
+    hipLaunchKernelGGL(exact::init_kernel, grid1(P.cells), dim3(256), 0, s, P, C, f, t, u_state, initial_temperature);
+}
+
+void launch_exact_step(const StepParams& P, const CellFields& C, const float* fin, float* fout, const float* t_in,
+                       float* t_out, float* u_state, int local_mode, hipStream_t s) {
+    hipLaunchKernelGGL(exact::step_kernel, grid1(P.cells), dim3(256), 0, s, P, C, fin, fout, t_in, t_out, u_state,
+                       local_mode);
+}
+
+void launch_exact_stream(const StepParams& P, const CellFields& C, const float* fin, float* fout, hipStream_t s) {
+    hipLaunchKernelGGL(exact::stream_kernel, grid1(P.cells), dim3(256), 0, s, P, C, fin, fout);
+}
+
+void launch_exact_readout(const StepParams& P, const CellFields& C, const float* f, int post_collision, int stride,
+                          int out_ch, float scale, float* out, hipStream_t s) {
+    const int sx = (P.nx + stride - 1) / stride, sy = (P.ny + stride - 1) / stride, sz = (P.nz + stride - 1) / stride;
+    hipLaunchKernelGGL(exact::readout_kernel, grid1(sx * sy * sz), dim3(256), 0, s, P, C, f, post_collision, stride, sx,
+                       sy, sz, out_ch, scale, out);
+}
+
+void launch_exact_sample(const StepParams& P, const CellFields& C, const float* f, int post_collision, int x, int y,
+                         int z, float* out4, hipStream_t s) {
+    hipLaunchKernelGGL(exact::sample_kernel, dim3(1), dim3(64), 0, s, P, C, f, post_collision, x, y, z, out4);
+}
+
+}  // namespace aero

@@ -1,0 +1,30 @@
+// Launchers of the fast D3Q27 kernels (d3q27_fast.hip).
+#pragma once
+
+#include "kernels.h"
+
+namespace aero {
+
+enum class FastStorage { F32, F16, BF16 };
+
+// per-cell link mask: bit q (0..26) = the pull source x - c_q is solid or out of the
+// domain after periodic wrapping; plus the two flags below
+constexpr uint32_t kMaskLinks = (1u << 27) - 1u;
+constexpr uint32_t kMaskSolid = 1u << 27;  // the cell itself is solid
+constexpr uint32_t kMaskFace = 1u << 28;   // at least one link leaves the domain
+
+void launch_fast_mask(const StepParams& P, const uint8_t* flags, uint32_t* mask, hipStream_t s);
+void launch_fast_init(const StepParams& P, const CellFields& C, FastStorage st, void* f, float* T, float* T_other, void* u_state,
+                      float t_init, hipStream_t s);
+// mode: 0 = A-B pull (local_first: collide the initial state in place),
+//       1 = A-A even (local), 2 = A-A odd (pull + push)
+void launch_fast_step(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* fin,
+                      void* fout, const float* t_in, float* t_out, void* u_state, const float* store_in,
+                      float* store_out, int mode, int local_first, hipStream_t s);
+// layout: 0 = A-B post-collision, 1 = A-A after an even step, 2 = A-A after an odd
+// step, 3 = initial state (incoming values in own slots, no face rules)
+void launch_fast_readout(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* f,
+                         const float* store, int layout, int stride, int out_ch, float scale, float* out,
+                         hipStream_t s);
+
+}  // namespace aero

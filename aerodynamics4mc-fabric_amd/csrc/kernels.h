@@ -1,0 +1,37 @@
+// Host-side launch wrappers for the HIP kernels (one translation unit per kernel
+// family so each can carry its own floating-point flags).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.h"
+
+namespace aero {
+
+// Per-context device state shared by all D3Q27 kernels.
+struct CellFields {
+    const uint8_t* flags;  // kCellSolid | kCellBoundary
+    const float* ref_p;    // payload channel 8 (clamped)
+    const float* ref_u;    // payload channels 5-7, three planes
+    const float* ref_t;    // payload channel 10 (clamped)
+    const float* src;      // payload channel 9 (clamped)
+};
+
+// ---- payload ingest and cell classification (d3q27_common.hip) ----
+void launch_ingest(const StepParams& P, const float* payload, uint8_t* flags, float* ref_p, float* ref_u,
+                   float* ref_t, float* src, hipStream_t s);
+void launch_classify(const StepParams& P, uint8_t* flags, hipStream_t s);
+
+// ---- exact FP32 A-B path (d3q27_exact.hip): operation order of the reference CPU step ----
+void launch_exact_init(const StepParams& P, const CellFields& C, float* f, float* t, float* u_state,
+                       float initial_temperature, hipStream_t s);
+void launch_exact_step(const StepParams& P, const CellFields& C, const float* fin, float* fout, const float* t_in,
+                       float* t_out, float* u_state, int local_mode, hipStream_t s);
+void launch_exact_stream(const StepParams& P, const CellFields& C, const float* fin, float* fout, hipStream_t s);
+void launch_exact_readout(const StepParams& P, const CellFields& C, const float* f, int post_collision, int stride,
+                          int out_ch, float scale, float* out, hipStream_t s);
+void launch_exact_sample(const StepParams& P, const CellFields& C, const float* f, int post_collision, int x, int y,
+                         int z, float* out4, hipStream_t s);
+
+}  // namespace aero

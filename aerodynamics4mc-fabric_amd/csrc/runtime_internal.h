@@ -1,0 +1,115 @@
+// Internal runtime state shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/aero_lbm_capi.h"
+#include "kernels.h"
+
+namespace aero {
+
+enum class Storage { F32Exact, F32Fast, F16, BF16, F32AA };
+
+struct Context {
+    int nx = 0, ny = 0, nz = 0;
+    size_t cells = 0;
+    Storage storage = Storage::F32Fast;
+    bool initialized = false;      // populations seeded from the first payload
+    bool post_collision = false;   // buffer holds post-collision populations
+    bool has_payload = false;
+    bool thermal_on = false;       // sticky: temperature can be non-zero
+    bool temperature_nonzero = false;
+    bool payload_src_nonzero = false;
+    bool payload_t_nonzero = false;
+    uint64_t steps = 0;
+    int f_cur = 0;                 // A-B: index of the buffer holding the state
+    int t_cur = 0;                 // temperature ping-pong index
+    int aa_layout = 3;             // A-A: 1 after an even step, 2 after an odd step, 3 initial state
+    int store_cur = 0;             // A-A: face store written by the last step
+    uint32_t* mask = nullptr;      // fast paths: per-cell link mask
+    // device buffers
+    uint8_t* flags = nullptr;
+    float* ref_p = nullptr;
+    float* ref_u = nullptr;
+    float* ref_t = nullptr;
+    float* src = nullptr;
+    float* T[2] = {nullptr, nullptr};
+    float* u_state = nullptr;
+    float* f32[2] = {nullptr, nullptr};  // FP32 A-B populations (27 planes)
+    uint16_t* h16 = nullptr;             // FP16/BF16 A-A populations (27 planes, shifted by w_q)
+    float* face_store[2] = {nullptr, nullptr};  // A-A: near-face post-collision values per parity
+    size_t face_store_floats = 0;
+    float* payload_dev = nullptr;
+    float* out_dev = nullptr;
+    float* sample_dev = nullptr;
+    FaceParams* faces_dev = nullptr;   // device copy of the face table
+    FaceParams faces_host[6] = {};     // last uploaded face table (source of the async copy)
+    bool faces_valid = false;
+    int class_wrap[6] = {-1, -1, -1, -1, -1, -1};  // periodic flags the cell classes were built with
+    std::vector<float> payload_host;
+    std::vector<void*> allocations;
+    uint64_t timing_upload_bytes = 0;
+
+    bool alloc(void** p, size_t bytes);
+    ~Context();
+};
+
+struct RuntimeConfig {
+    bool initialized = false;
+    int grid_size = 0, nx = 0, ny = 0, nz = 0, in_ch = 0, out_ch = 0;
+};
+
+struct Timing {
+    uint64_t ticks = 0;
+    double last_total = 0, last_solver = 0, last_readback = 0, last_copy = 0;
+    double sum_total = 0, sum_solver = 0, sum_readback = 0, sum_copy = 0;
+};
+
+AeroLbmBenchmarkConfig default_benchmark_config();
+AeroLbmSolverOverrides default_overrides();
+
+struct RuntimeState {
+    std::mutex mu;
+    RuntimeConfig cfg;
+    AeroLbmBenchmarkConfig bench = default_benchmark_config();
+    AeroLbmSolverOverrides overrides = default_overrides();
+    int solver_mode = AERO_LBM_REALTIME_SOLVER_AUTO;
+    std::string last_error;
+    std::string runtime_info;
+    std::string device_name, device_arch;
+    int cu_count = 0;
+    bool device_ready = false;
+    hipStream_t stream = nullptr;
+    std::unordered_map<long long, std::unique_ptr<Context>> contexts;
+    Timing timing;
+};
+
+RuntimeState& rt();
+void set_error(const std::string& e);
+void clear_error();
+bool hip_ok(hipError_t e, const char* what);
+bool ensure_device();
+bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P);
+// resolve_params + upload of the face table into the context's device copy
+bool prepare_params(Context& c, StepParams& P);
+void sanitize_benchmark_config(AeroLbmBenchmarkConfig& c);
+Context* find_context(long long key);
+Context* get_context(long long key, int nx, int ny, int nz, bool create);
+CellFields cell_fields(const Context& c);
+bool step_context(Context& c, const float* packet, float* out, int out_ch, float scale);
+
+// storage-specific operations (storage.hip)
+bool storage_alloc(Context& c);
+bool storage_step(Context& c, const StepParams& P);
+bool storage_readout(Context& c, const StepParams& P, int stride, int out_ch, float scale, float* out_dev);
+bool storage_sample(Context& c, const StepParams& P, int x, int y, int z, float* out_dev4);
+size_t context_device_bytes(const Context& c);
+void storage_rebuild_masks(Context& c, const StepParams& P);
+
+}  // namespace aero

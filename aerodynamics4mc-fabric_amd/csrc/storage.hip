@@ -1,0 +1,146 @@
+// Storage-mode dispatch: which kernels advance and read a context.
+//   F32Exact: FP32 A-B, reference operation order (parity path)
+//   F32Fast : FP32 A-B, factorised moment transforms (performance path)
+//   F16/BF16: equilibrium-shifted 16-bit populations, A-A in-place streaming
+#include "fast_kernels.h"
+#include "runtime_internal.h"
+
+namespace aero {
+
+static FastStorage fast_kind(Storage s) {
+    return s == Storage::F16 ? FastStorage::F16 : (s == Storage::BF16 ? FastStorage::BF16 : FastStorage::F32);
+}
+
+bool storage_alloc(Context& c) {
+    const size_t N = c.cells;
+    switch (c.storage) {
+        case Storage::F32Exact:
+            return c.alloc((void**)&c.f32[0], 27 * N * 4) && c.alloc((void**)&c.f32[1], 27 * N * 4);
+        case Storage::F32Fast:
+            return c.alloc((void**)&c.f32[0], 27 * N * 4) && c.alloc((void**)&c.f32[1], 27 * N * 4) &&
+                   c.alloc((void**)&c.mask, N * 4);
+        case Storage::F16:
+        case Storage::BF16:
+            return c.alloc((void**)&c.h16, 27 * N * 2) && c.alloc((void**)&c.mask, N * 4);
+        case Storage::F32AA:
+            return c.alloc((void**)&c.f32[0], 27 * N * 4) && c.alloc((void**)&c.mask, N * 4);
+    }
+    return false;
+}
+
+// population buffer of an A-A context
+static void* aa_buffer(Context& c) { return c.storage == Storage::F32AA ? (void*)c.f32[0] : (void*)c.h16; }
+
+size_t context_device_bytes(const Context& c) {
+    const size_t N = c.cells;
+    size_t b = N * (1 + 4 * 4 + 4 * 2 + 12 + 16) + 64;
+    if (c.f32[0]) b += 2 * 27 * N * 4;
+    if (c.h16) b += 27 * N * 2;
+    if (c.mask) b += N * 4;
+    b += 2 * c.face_store_floats * 4;
+    if (c.payload_dev) b += c.payload_host.size() * 4;
+    return b;
+}
+
+void storage_rebuild_masks(Context& c, const StepParams& P) {
+    if (c.mask) launch_fast_mask(P, c.flags, c.mask, rt().stream);
+}
+
+static bool ensure_face_store(Context& c, const StepParams& P) {
+    if ((size_t)P.face_store_floats <= c.face_store_floats) return true;
+    if (!hip_ok(hipStreamSynchronize(rt().stream), "sync")) return false;
+    for (int i = 0; i < 2; ++i) {
+        if (c.face_store[i]) {
+            hipFree(c.face_store[i]);
+            for (auto& p : c.allocations)
+                if (p == c.face_store[i]) p = nullptr;
+            c.face_store[i] = nullptr;
+        }
+        if (!c.alloc((void**)&c.face_store[i], (size_t)P.face_store_floats * 4)) return false;
+        if (!hip_ok(hipMemsetAsync(c.face_store[i], 0, (size_t)P.face_store_floats * 4, rt().stream), "memset"))
+            return false;
+    }
+    c.face_store_floats = (size_t)P.face_store_floats;
+    return true;
+}
+
+bool storage_step(Context& c, const StepParams& P) {
+    hipStream_t st = rt().stream;
+    const CellFields C = cell_fields(c);
+    if (c.storage == Storage::F32Exact) {
+        if (!c.initialized) {
+            launch_exact_init(P, C, c.f32[c.f_cur], c.T[c.t_cur], c.u_state, rt().bench.initial_temperature, st);
+            c.initialized = true;
+            c.post_collision = false;
+        }
+        const int local = c.post_collision ? 0 : 1;
+        launch_exact_step(P, C, c.f32[c.f_cur], c.f32[1 - c.f_cur], c.T[c.t_cur], c.T[1 - c.t_cur], c.u_state, local,
+                          st);
+        c.f_cur ^= 1;
+        if (P.thermal_on) c.t_cur ^= 1;
+        c.post_collision = true;
+        return hip_ok(hipGetLastError(), "exact step");
+    }
+    const FastStorage fk = fast_kind(c.storage);
+    if (!ensure_face_store(c, P)) return false;
+    if (!c.initialized) {
+        void* f0 = c.storage == Storage::F32Fast ? (void*)c.f32[c.f_cur] : aa_buffer(c);
+        launch_fast_init(P, C, fk, f0, c.T[c.t_cur], c.T[1 - c.t_cur], c.u_state, rt().bench.initial_temperature, st);
+        c.initialized = true;
+        c.post_collision = false;
+        c.aa_layout = 3;
+    }
+    if (c.storage == Storage::F32Fast) {
+        launch_fast_step(P, C, fk, c.mask, c.f32[c.f_cur], c.f32[1 - c.f_cur], c.T[c.t_cur], c.T[1 - c.t_cur],
+                         c.u_state, nullptr, nullptr, 0, c.post_collision ? 0 : 1, st);
+        c.f_cur ^= 1;
+        c.post_collision = true;
+    } else {
+        const bool even = c.aa_layout == 3 || c.aa_layout == 2;
+        const int mode = even ? 1 : 2;
+        launch_fast_step(P, C, fk, c.mask, aa_buffer(c), aa_buffer(c), c.T[c.t_cur], c.T[1 - c.t_cur], c.u_state,
+                         c.face_store[c.store_cur], c.face_store[1 - c.store_cur], mode, c.aa_layout == 3 ? 1 : 0,
+                         st);
+        c.store_cur ^= 1;
+        c.aa_layout = even ? 1 : 2;
+        c.post_collision = true;
+    }
+    if (P.thermal_on) c.t_cur ^= 1;
+    return hip_ok(hipGetLastError(), "fast step");
+}
+
+bool storage_readout(Context& c, const StepParams& P, int stride, int out_ch, float scale, float* out_dev) {
+    hipStream_t st = rt().stream;
+    const CellFields C = cell_fields(c);
+    switch (c.storage) {
+        case Storage::F32Exact:
+            launch_exact_readout(P, C, c.f32[c.f_cur], c.post_collision ? 1 : 0, stride, out_ch, scale, out_dev, st);
+            break;
+        case Storage::F32Fast:
+            launch_fast_readout(P, C, FastStorage::F32, c.mask, c.f32[c.f_cur], nullptr, c.post_collision ? 0 : 3,
+                                stride, out_ch, scale, out_dev, st);
+            break;
+        case Storage::F16:
+        case Storage::BF16:
+        case Storage::F32AA:
+            launch_fast_readout(P, C, fast_kind(c.storage), c.mask, aa_buffer(c), c.face_store[c.store_cur],
+                                c.aa_layout, stride, out_ch, scale, out_dev, st);
+            break;
+    }
+    return hip_ok(hipGetLastError(), "readout");
+}
+
+bool storage_sample(Context& c, const StepParams& P, int x, int y, int z, float* out_dev4) {
+    hipStream_t st = rt().stream;
+    const CellFields C = cell_fields(c);
+    if (c.storage == Storage::F32Exact) {
+        launch_exact_sample(P, C, c.f32[c.f_cur], c.post_collision ? 1 : 0, x, y, z, out_dev4, st);
+        return hip_ok(hipGetLastError(), "sample");
+    }
+    // the fast paths read the whole field once and copy the requested cell
+    if (!storage_readout(c, P, 1, 4, 1.0f, c.out_dev)) return false;
+    const size_t cell = ((size_t)x * c.ny + y) * c.nz + z;
+    return hip_ok(hipMemcpyAsync(out_dev4, c.out_dev + cell * 4, 16, hipMemcpyDeviceToDevice, st), "sample copy");
+}
+
+}  // namespace aero

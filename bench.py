@@ -1,0 +1,187 @@
+"""Throughput benchmark of the D3Q27 cumulant step on one MI355X.
+
+Workload (one "step" = one lattice update of every cell of one brick): D3Q27
+cumulant + Smagorinsky-type SGS (C = 0.1) + Boussinesq (beta 0.1, gravity -z)
+on one 128^3 brick with ~10% solid boxes and a Gaussian hot blob on the floor,
+inflow 0.05 -- the 128^3 fp16_inplace workload. The headline `value` is the
+GLUPS of the FP16 equilibrium-shifted A-A path; the FP32 A-B path runs on the
+same inputs for the FP16/FP32 ratio. Inputs are resident on the device before
+timing; no readback inside the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes fp16,fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "aerodynamics4mc-fabric_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "GLUPS + % HBM roofline, D3Q27 128³ FP16-shifted vs FP32 plain, 1 MI355X"
+
+MODES = {
+    # name: (solver mode, kernel variant, bytes per population)
+    "fp16": ("fp16", 2),
+    "bf16": ("bf16", 2),
+    "fp32": ("fp32_fast", 4),
+    "fp32_exact": ("fp32_exact", 4),
+}
+
+
+def algorithmic_bytes(mode: str, solid: np.ndarray, thermal: bool) -> dict:
+    """Minimal HBM bytes of one step launch: every fluid population read once and
+    written once, one flag byte per cell, and with the thermal scalar on the
+    temperature (read + write) and the cell velocity carried to the next step's
+    advection (read + write). Solid cells move no populations except in the
+    exact path, which swaps them like the reference."""
+    cells = solid.size
+    fluid = int((solid == 0).sum())
+    pop = MODES[mode][1]
+    per_fluid = 27 * pop * 2
+    if thermal:
+        u_bytes = 6 if pop == 2 else 12  # velocity carried as 3 x f16 or 3 x f32
+        per_fluid += 4 * 2 + u_bytes * 2
+    total = fluid * per_fluid + cells * 1
+    if mode == "fp32_exact":
+        total += (cells - fluid) * 27 * 4 * 2
+    return {"bytes": total, "per_lup": total / cells, "fluid_fraction": fluid / cells}
+
+
+def set_mode(lib, A, W, mode: str, ov):
+    ov.kernel_variant = A.KERNEL_EXACT if mode == "fp32_exact" else A.KERNEL_FAST
+    A.check(lib.aero_lbm_set_solver_overrides(ctypes.byref(ov)), "set_solver_overrides")
+    lib.aero_lbm_set_realtime_solver_mode({"fp16": A.SOLVER_D3Q27_FP16_INPLACE, "bf16": A.SOLVER_D3Q27_BF16_INPLACE}
+                                          .get(mode, A.SOLVER_CLASSIC_D3Q27))
+
+
+def run_mode(lib, A, W, mode, n, steps, warmup, inputs):
+    import torch
+    cfg, ov, packet, temp, solid = inputs
+    A.check(lib.aero_lbm_init_rect(n, n, n, 11, 4), "init_rect")
+    A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "benchmark_set_config")
+    set_mode(lib, A, W, mode, ov)
+    ctx = A.Context(n, n, n, key=100)
+    ctx.set_temperature(temp)
+    ctx.step(packet)  # upload + initialise (untimed)
+    for _ in range(max(0, warmup - 1)):
+        ctx.step(None)
+    A.check(lib.aero_lbm_finish(), "finish")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    device_ms = ctx.run_timed(steps)  # HIP events on the library's stream around the step launches
+    A.check(lib.aero_lbm_finish(), "finish")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    flow = ctx.flow().reshape(-1, 4)
+    finite = bool(np.isfinite(flow).all())
+    ctx.release()
+    cells = n ** 3
+    return {"mode": mode, "wall_s": wall, "device_ms": device_ms, "glups_wall": cells * steps / wall / 1e9,
+            "glups_device": cells * steps / (device_ms / 1e3) / 1e9, "kernel_ms": device_ms / steps,
+            "finite": finite, "max_speed": float(np.abs(flow[:, :3]).max())}
+
+
+def cpu_baseline(n, inputs, sample_steps=2):
+    """The FP32 oracle (plain-C restatement of the reference's single-threaded CPU
+    step) on this host, one core, on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cfg, ov, packet, temp, solid = inputs
+    o = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=32)
+    o.set_temperature(temp)
+    o.run(packet, 1)  # warm-up step (includes initialisation)
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        o.step(None)
+    dt = time.perf_counter() - t0
+    cpu = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": n ** 3 * sample_steps / dt / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": f"{sample_steps} steps of the same {n}^3 workload after 1 warm-up step, FP32 oracle "
+                      f"(oracle/liboracle.so), single thread on {cpu}; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--modes", default="fp16,fp32")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the solver has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    import aero_lbm as A
+    import workloads as W
+    lib = A.load()
+    n = args.n
+    inputs = W.config3(n, beta=0.1, cs=0.1)
+    solid = inputs[4]
+    modes = [m for m in args.modes.split(",") if m]
+    results = {}
+    for m in modes:
+        print(f"bench: mode {m} n={n} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+        results[m] = run_mode(lib, A, W, m, n, args.steps, args.warmup, inputs)
+        print(f"bench: {m}: {results[m]['glups_wall']:.2f} GLUPS (wall), kernel {results[m]['kernel_ms']:.4f} ms",
+              file=sys.stderr, flush=True)
+    head = modes[0]
+    r = results[head]
+    ab = algorithmic_bytes(head, solid, thermal=True)
+    achieved = ab["bytes"] / (r["kernel_ms"] / 1e3)
+    line = {
+        "metric": METRIC,
+        "value": round(r["glups_wall"], 4),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"d3q27_cumulant_sgs_boussinesq_{n}^3_{head}", "cells": n ** 3,
+                   "storage": head, "solid_fraction": round(1 - ab["fluid_fraction"], 4),
+                   "sgs_c": 0.1, "beta": 0.1, "gravity": "-z", "inflow": 0.05},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes_per_launch": ab["bytes"], "bytes_per_lup": round(ab["per_lup"], 2),
+                     "kernel_ms": r["kernel_ms"]},
+        "modes": {m: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res.items()}
+                  for m, res in results.items()},
+    }
+    if "fp32" in results and head != "fp32":
+        line["fp16_over_fp32"] = round(results[head]["glups_wall"] / results["fp32"]["glups_wall"], 3)
+    if not args.no_cpu:
+        print("bench: cpu baseline (oracle, 1 core)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(n, inputs)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

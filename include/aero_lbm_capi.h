@@ -43,7 +43,8 @@ enum {
     AERO_LBM_REALTIME_SOLVER_CLASSIC_D3Q27 = 1,
     AERO_LBM_REALTIME_SOLVER_COMPACT_EXPERIMENTAL = 2,
     AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL = 3,
-    AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE = 4
+    AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE = 4,
+    AERO_LBM_REALTIME_SOLVER_D3Q27_FP32_INPLACE = 5  /* extension: FP32 unshifted A-A */
 };
 
 /* ref: native/include/aero_lbm_capi.h:35-43 */

@@ -29,6 +29,20 @@
 
 #define ORACLE_API __attribute__((visibility("default")))
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Worker threads for the per-cell loops (1 = the reference's single thread).
+ * Every loop writes disjoint cells, so results do not depend on the count. */
+ORACLE_API void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
 /* hydro_core.h:122-205: q = (ix*3 + iy)*3 + iz, c = i - 1, opp(q) = 26 - q */
 static const int kCX[27] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1};
 static const int kCY[27] = {-1, -1, -1, 0, 0, 0, 1, 1, 1, -1, -1, -1, 0, 0, 0, 1, 1, 1, -1, -1, -1, 0, 0, 0, 1, 1, 1};

@@ -337,6 +337,8 @@ static void SFX(collide)(SFX(Ctx) * c) {
     const int sgs_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_SGS);
     const int buoy_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_BUOYANCY);
     const R cs2 = K(1.0) / K(3.0);
+    /* cells are independent: threads change nothing but wall time */
+#pragma omp parallel for schedule(static)
     for (size_t cell = 0; cell < N; ++cell) {
         if (c->obstacle[cell]) {
             c->rho[cell] = K(1.0);
@@ -525,6 +527,7 @@ static R SFX(face_value)(const SFX(Ctx) * c, size_t cell, int x, int y, int z, i
 
 static void SFX(stream)(SFX(Ctx) * c) {
     const size_t N = c->cells;
+#pragma omp parallel for schedule(static)
     for (int x = 0; x < c->nx; ++x)
         for (int y = 0; y < c->ny; ++y)
             for (int z = 0; z < c->nz; ++z) {
@@ -592,6 +595,7 @@ static R SFX(trilinear)(const SFX(Ctx) * c, const R* field, R x, R y, R z, R fb)
 static void SFX(update_temperature)(SFX(Ctx) * c) {
     const R dt = (R)1; /* benchmark stride 1 (jni.cpp:901-909) */
     const int src_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_INTERNAL_THERMAL_SOURCE);
+#pragma omp parallel for schedule(static)
     for (size_t cell = 0; cell < c->cells; ++cell) {
         if (c->obstacle[cell]) { c->T_next[cell] = K(0.0); continue; }
         const int yz = c->ny * c->nz;

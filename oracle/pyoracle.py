@@ -53,8 +53,15 @@ def load():
         getattr(lib, p + "oracle_feq").restype = real
         getattr(lib, p + "oracle_weights").argtypes = [R]
     lib.oracle_sanitize_config.argtypes = [c_void_p]
+    lib.oracle_set_threads.argtypes = [c_int]
+    lib.oracle_set_threads(min(16, os.cpu_count() or 1))
     _lib = lib
     return lib
+
+
+def set_threads(n: int) -> None:
+    """Threads for the oracle's per-cell loops; results are identical for any count."""
+    load().oracle_set_threads(n)
 
 
 def _fp(a):

@@ -1,0 +1,94 @@
+"""Writes the oracle fixtures under tests/golden/ (run in the build container).
+
+Long oracle runs are too slow to repeat inside a GPU test, so their results are
+stored as fixtures: SHA-256 of the FP32 output bytes (the exact GPU path must
+reproduce them bit for bit), 512 seeded probe cells, per-x-plane sums, and for
+accuracy checks the FP64 oracle's velocity field at the probes.
+
+    python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py config1    # one
+"""
+import hashlib
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+for p in (ROOT, os.path.join(ROOT, "aerodynamics4mc-fabric_amd"), os.path.join(ROOT, "oracle")):
+    sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+import pyoracle  # noqa: E402
+import workloads as W  # noqa: E402
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float32).tobytes()).hexdigest()
+
+
+def probes(cells: int, count: int = 512, seed: int = 99) -> list:
+    return sorted(set(int(v) for v in (W.splitmix_array(seed, count) * cells).astype(np.int64)))
+
+
+def summarize(flow: np.ndarray, temp: np.ndarray, nx: int, idx: list) -> dict:
+    f4 = flow.reshape(-1, 4)
+    planes = f4.reshape(nx, -1, 4).astype(np.float64).sum(axis=1)
+    return {
+        "flow_sha256": digest(flow),
+        "temperature_sha256": digest(temp),
+        "probe_cells": idx,
+        "probe_flow": f4[idx].astype(float).tolist(),
+        "probe_temperature": temp[idx].astype(float).tolist(),
+        "plane_sums": planes.tolist(),
+    }
+
+
+def config1(steps=1000, n=32):
+    cfg, pk, solid = W.config1(n)
+    out = {"name": "config1", "n": n, "steps": steps,
+           "desc": "D3Q27 cumulant 32^3, 15% solid boxes (seed 2), u=(0.05,0,0)+-1e-3 (seed 1), "
+                   "nu 1e-3 (Re 50), reference SGS C=0.025, buoyancy off, velocity inlet x-, convective x+, "
+                   "periodic y/z"}
+    idx = probes(n ** 3)
+    for prec in (32, 64):
+        t0 = time.time()
+        o = pyoracle.Oracle(n, n, n, cfg, precision=prec)
+        o.run(pk, steps)
+        out[f"fp{prec}"] = summarize(o.flow(), o.temperature(), n, idx)
+        if prec == 64:
+            u = o.velocity()
+            out["fp64"]["probe_velocity_f64"] = u[idx].tolist()
+        out[f"fp{prec}"]["seconds"] = time.time() - t0
+        print(f"config1 fp{prec}: {time.time() - t0:.1f} s", flush=True)
+    return out
+
+
+def config3_small(steps=300, n=32):
+    cfg, ov, pk, t, solid = W.config3(n, beta=0.1, cs=0.1)
+    out = {"name": "config3_small", "n": n, "steps": steps,
+           "desc": "D3Q27 cumulant + SGS (C=0.1) + Boussinesq (beta 0.1, gravity -z) at 32^3: ~10% solid "
+                   "boxes (seed 5), Gaussian hot blob dT 0.02 sigma 8 on the z=0 floor (seed 4), inflow 0.05"}
+    idx = probes(n ** 3)
+    for prec in (32, 64):
+        t0 = time.time()
+        o = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=prec)
+        o.set_temperature(t)
+        o.run(pk, steps)
+        out[f"fp{prec}"] = summarize(o.flow(), o.temperature(), n, idx)
+        out[f"fp{prec}"]["seconds"] = time.time() - t0
+        print(f"config3_small fp{prec}: {time.time() - t0:.1f} s", flush=True)
+    return out
+
+
+JOBS = {"config1": config1, "config3_small": config3_small}
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(JOBS)
+    for name in names:
+        res = JOBS[name]()
+        with open(os.path.join(HERE, f"{name}.json"), "w") as fh:
+            json.dump(res, fh, indent=0)
+        print("wrote", name, flush=True)

@@ -1,0 +1,154 @@
+"""GPU checks of the fast paths (FP32 A-B, FP32/FP16/BF16 A-A) through the C ABI.
+
+* FP32 A-A and FP32 A-B run the same collision code; the A-A indexing, parity
+  flip, bounce-back slots and face store must reproduce the A-B result bit for
+  bit after every step count.
+* FP32 fast / FP16 / BF16 against the FP64 oracle: velocity error relative to
+  the inflow speed. Bar: FP16 shifted storage <= 1e-3 of
+  the inflow speed; FP32 <= 1e-4 of the inflow speed.
+"""
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import aero_lbm as A
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return A.load()
+
+
+def make_ctx(lib, dims, cfg, ov, mode, key):
+    A.check(lib.aero_lbm_init_rect(*dims, 11, 4), "init_rect")
+    A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "set_config")
+    A.check(lib.aero_lbm_set_solver_overrides(ctypes.byref(ov)), "set_overrides")
+    lib.aero_lbm_set_realtime_solver_mode(mode)
+    return A.Context(*dims, key=key)
+
+
+def run(lib, dims, cfg, ov, mode, packet, steps, temperature=None, key=31):
+    ctx = make_ctx(lib, dims, cfg, ov, mode, key)
+    if temperature is not None:
+        ctx.set_temperature(temperature)
+    ctx.step(packet)
+    for _ in range(steps - 1):
+        ctx.step(None)
+    flow, temp = ctx.flow(), ctx.temperature()
+    ctx.release()
+    return flow, temp
+
+
+def variant_cfg(variant):
+    cfg = W.channel_config(0.04, 40.0, sgs=True, buoyancy=True)
+    if variant == "symmetry":
+        for f in (cfg.y_min, cfg.y_max, cfg.z_min, cfg.z_max):
+            f.hydrodynamic_kind = A.HYDRO_SYMMETRY
+            f.thermal_kind = A.THERMAL_ADIABATIC
+    elif variant == "walls":
+        cfg.y_min.hydrodynamic_kind = A.HYDRO_BOUNCE_BACK
+        cfg.y_max.hydrodynamic_kind = A.HYDRO_MOVING_WALL
+        cfg.y_max.velocity[0] = 0.03
+        cfg.z_min.hydrodynamic_kind = A.HYDRO_PRESSURE
+        cfg.z_max.hydrodynamic_kind = A.HYDRO_INHERIT
+    elif variant == "sponge":
+        cfg.flags &= ~A.FLAG_DISABLE_SPONGE
+    return cfg
+
+
+def variant_inputs(dims):
+    solid = W.solid_boxes(*dims, 0.12, seed=21, edge=(1, 3), keep_clear_x=1)
+    t = W.hot_blob(*dims, 0.05, 3.0, seed=4, axis=1)
+    pk = W.make_packet(solid, u0=(0.04, 0.0, 0.0), perturb=2e-3, seed_u=5, temperature=t,
+                       source=0.001 * np.ones(dims, np.float32))
+    return pk, t
+
+
+@pytest.mark.parametrize("variant", ["channel", "symmetry", "walls", "sponge"])
+def test_aa_fp32_equals_ab_fp32_bitwise(lib, variant):
+    dims = (10, 9, 8)
+    cfg = variant_cfg(variant)
+    ov = W.reference_overrides(A.KERNEL_FAST)
+    ov.boussinesq_beta = 0.05
+    pk, t = variant_inputs(dims)
+    for steps in (1, 2, 3, 6, 9):
+        log(f"aa-vs-ab {variant} steps={steps}")
+        fa, ta = run(lib, dims, cfg, ov, A.SOLVER_D3Q27_FP32_INPLACE, pk, steps, t)
+        fb, tb = run(lib, dims, cfg, ov, A.SOLVER_CLASSIC_D3Q27, pk, steps, t)
+        assert np.array_equal(fa, fb), (steps, np.max(np.abs(fa - fb)))
+        assert np.array_equal(ta, tb), steps
+
+
+def velocity_error(flow, ref_u, solid, u_in):
+    f = flow.reshape(-1, 4)[:, :3].astype(np.float64)
+    fluid = solid.reshape(-1) == 0
+    return float(np.max(np.abs(f[fluid] - ref_u[fluid]))) / u_in
+
+
+@pytest.mark.parametrize("mode,bar", [(A.SOLVER_CLASSIC_D3Q27, 1e-4), (A.SOLVER_D3Q27_FP16_INPLACE, 1e-3),
+                                      (A.SOLVER_D3Q27_FP32_INPLACE, 1e-4)])
+def test_fast_paths_vs_fp64_oracle_channel(oracle_mod, lib, mode, bar):
+    n, steps = 16, 200
+    log(f"fast vs fp64 channel mode={mode}")
+    cfg, pk, solid = W.config1(n)
+    o = oracle_mod.Oracle(n, n, n, cfg, precision=64)
+    o.run(pk, steps)
+    ref = o.flow().reshape(-1, 4)[:, :3].astype(np.float64)
+    flow, _ = run(lib, (n, n, n), cfg, W.reference_overrides(A.KERNEL_FAST), mode, pk, steps)
+    err = velocity_error(flow, ref, solid, 0.05)
+    log(f"  velocity error / U_in = {err:.3e}")
+    assert err <= bar
+
+
+def test_config3_small_fp16_bf16_vs_fp64_fixture(lib):
+    """32^3 Boussinesq workload (C=0.1, beta 0.1, gravity -z), 300 steps, against the
+    FP64 oracle fixture tests/golden/config3_small.json."""
+    gold = json.load(open(os.path.join(HERE, "golden", "config3_small.json")))
+    n, steps = gold["n"], gold["steps"]
+    cfg, ov, pk, t, solid = W.config3(n, beta=0.1, cs=0.1)
+    idx = gold["fp64"]["probe_cells"]
+    ref = np.array(gold["fp64"]["probe_flow"])[:, :3]
+    fluid = solid.reshape(-1)[idx] == 0
+    errs = {}
+    for name, mode in (("fp32", A.SOLVER_CLASSIC_D3Q27), ("fp16", A.SOLVER_D3Q27_FP16_INPLACE),
+                       ("bf16", A.SOLVER_D3Q27_BF16_INPLACE)):
+        log(f"config3 small {name}")
+        flow, temp = run(lib, (n, n, n), cfg, ov, mode, pk, steps, t)
+        got = flow.reshape(-1, 4)[idx, :3].astype(np.float64)
+        errs[name] = float(np.max(np.abs(got[fluid] - ref[fluid]))) / 0.05
+        tref = np.array(gold["fp64"]["probe_temperature"])
+        errs[name + "_T"] = float(np.max(np.abs(temp[idx] - tref)))
+    log(f"  errors: {errs}")
+    assert errs["fp32"] <= 1e-4
+    assert errs["fp16"] <= 1e-3
+    assert errs["fp32_T"] <= 1e-5 and errs["fp16_T"] <= 1e-4
+    assert errs["bf16"] < 5e-2  # reported, BF16's 8-bit mantissa is not held to the FP16 bar
+
+
+def test_fp16_mass_drift_closed_box(lib):
+    """Closed box, bounce-back on all faces: mass drift stays at FP16 rounding level."""
+    n = 16
+    cfg = W.channel_config(0.03, 30.0, sgs=True, buoyancy=False)
+    for f in (cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.z_min, cfg.z_max):
+        f.hydrodynamic_kind = A.HYDRO_BOUNCE_BACK
+    solid = np.zeros((n, n, n), np.uint8)
+    pk = W.make_packet(solid, u0=(0.03, 0.0, 0.0), perturb=5e-3, seed_u=2)
+    ov = W.reference_overrides(A.KERNEL_FAST)
+    f1, _ = run(lib, (n, n, n), cfg, ov, A.SOLVER_D3Q27_FP16_INPLACE, pk, 1)
+    f2, _ = run(lib, (n, n, n), cfg, ov, A.SOLVER_D3Q27_FP16_INPLACE, pk, 400)
+    m1 = np.sum(f1.reshape(-1, 4)[:, 3].astype(np.float64))
+    m2 = np.sum(f2.reshape(-1, 4)[:, 3].astype(np.float64))
+    assert abs(m2 - m1) / n ** 3 < 2e-5
