@@ -1,0 +1,573 @@
+// Device-side building blocks of the fast D3Q27 kernels: storage traits (FP32,
+// FP16 and BF16 with the equilibrium shift g = f - w), neighbour offsets with
+// periodic wrap, the reference's face rules, the temperature update and the
+// factorised cumulant collision. Included by d3q27_fast.hip only.
+#pragma once
+
+#include "fast_kernels.h"
+
+namespace aero {
+namespace fast {
+
+// ---------------------------------------------------------------- storage traits
+struct StF32 {
+    using T = float;
+    static constexpr bool kShifted = false;
+    static __device__ __forceinline__ float ld(const T* p) { return __builtin_nontemporal_load(p); }
+    static __device__ __forceinline__ void st(T* p, float v) { __builtin_nontemporal_store(v, p); }
+};
+struct StF16 {
+    using T = uint16_t;
+    static constexpr bool kShifted = true;
+    static __device__ __forceinline__ float ld(const T* p) {
+        const uint16_t b = __builtin_nontemporal_load(p);
+        return (float)__builtin_bit_cast(_Float16, b);
+    }
+    static __device__ __forceinline__ void st(T* p, float v) {
+        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, (_Float16)v), p);
+    }
+};
+struct StBF16 {
+    using T = uint16_t;
+    static constexpr bool kShifted = true;
+    static __device__ __forceinline__ float ld(const T* p) {
+        const uint32_t b = __builtin_nontemporal_load(p);
+        return __uint_as_float(b << 16);
+    }
+    static __device__ __forceinline__ void st(T* p, float v) {
+        uint32_t u = __float_as_uint(v);
+        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+        __builtin_nontemporal_store((uint16_t)(u >> 16), p);
+    }
+};
+
+// velocity carried to the next step's temperature advection
+struct UF32 {
+    using T = float;
+    static __device__ __forceinline__ float ld(const T* p) { return *p; }
+    static __device__ __forceinline__ void st(T* p, float v) { *p = v; }
+};
+struct UF16 {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(const T* p) { return (float)__builtin_bit_cast(_Float16, *p); }
+    static __device__ __forceinline__ void st(T* p, float v) { *p = __builtin_bit_cast(uint16_t, (_Float16)v); }
+};
+
+__device__ constexpr float wq(int q) {
+    return (lat_cx(q) * lat_cx(q) + lat_cy(q) * lat_cy(q) + lat_cz(q) * lat_cz(q)) == 0   ? 8.0f / 27.0f
+           : (lat_cx(q) * lat_cx(q) + lat_cy(q) * lat_cy(q) + lat_cz(q) * lat_cz(q)) == 1 ? 2.0f / 27.0f
+           : (lat_cx(q) * lat_cx(q) + lat_cy(q) * lat_cy(q) + lat_cz(q) * lat_cz(q)) == 2 ? 1.0f / 54.0f
+                                                                                         : 1.0f / 216.0f;
+}
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(hi, fmaxf(lo, v)); }
+__device__ __forceinline__ bool finite(float v) { return __builtin_isfinite(v); }
+
+// equilibrium (ref: native/src/aero_lbm_hydro_core.cpp:364-368); `shifted` returns feq - w
+template <bool Shifted>
+__device__ __forceinline__ float feq_s(int q, float rho, float ux, float uy, float uz) {
+    const float cu = 3.0f * ((float)lat_cx(q) * ux + (float)lat_cy(q) * uy + (float)lat_cz(q) * uz);
+    const float uu = ux * ux + uy * uy + uz * uz;
+    const float poly = cu + 0.5f * cu * cu - 1.5f * uu;
+    if (Shifted) return wq(q) * ((rho - 1.0f) + rho * poly);
+    return wq(q) * rho * (1.0f + poly);
+}
+
+// ---------------------------------------------------------------- neighbour offsets
+// Offsets (in cells) to the neighbour at +-1 along each axis with periodic wrap;
+// an out-of-domain neighbour on a non-periodic face maps to the cell itself so
+// every computed index stays in bounds (those links are flagged special).
+struct Nbr {
+    int xm, xp, ym, yp, zm, zp;
+    __device__ __forceinline__ int ox(int d) const { return d < 0 ? xm : (d > 0 ? xp : 0); }
+    __device__ __forceinline__ int oy(int d) const { return d < 0 ? ym : (d > 0 ? yp : 0); }
+    __device__ __forceinline__ int oz(int d) const { return d < 0 ? zm : (d > 0 ? zp : 0); }
+    __device__ __forceinline__ int off(int dx, int dy, int dz) const { return ox(dx) + oy(dy) + oz(dz); }
+};
+
+__device__ __forceinline__ Nbr make_nbr(const StepParams& P, int x, int y, int z) {
+    const int yz = P.ny * P.nz;
+    Nbr n;
+    n.xm = x > 0 ? -yz : (P.hydro_wrap_lo[0] ? (P.nx - 1) * yz : 0);
+    n.xp = x < P.nx - 1 ? yz : (P.hydro_wrap_hi[0] ? -(P.nx - 1) * yz : 0);
+    n.ym = y > 0 ? -P.nz : (P.hydro_wrap_lo[1] ? (P.ny - 1) * P.nz : 0);
+    n.yp = y < P.ny - 1 ? P.nz : (P.hydro_wrap_hi[1] ? -(P.ny - 1) * P.nz : 0);
+    n.zm = z > 0 ? -1 : (P.hydro_wrap_lo[2] ? P.nz - 1 : 0);
+    n.zp = z < P.nz - 1 ? 1 : (P.hydro_wrap_hi[2] ? -(P.nz - 1) : 0);
+    return n;
+}
+
+// ---------------------------------------------------------------- face rules
+__device__ __forceinline__ int wrapc(int c, int n, int lo, int hi) {
+    if (c < 0 && lo) return c + n;
+    if (c >= n && hi) return c - n;
+    return c;
+}
+__device__ __forceinline__ int icl(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// first out-of-domain axis in the reference's order x-, x+, y-, y+, z-, z+ (jni.cpp:743-751)
+__device__ __forceinline__ int oob_face(const StepParams& P, int sx, int sy, int sz) {
+    if (sx < 0) return 0;
+    if (sx >= P.nx) return 1;
+    if (sy < 0) return 2;
+    if (sy >= P.ny) return 3;
+    if (sz < 0) return 4;
+    return 5;
+}
+
+// Post-collision value (stored form) of direction q at in-domain cell (sx,sy,sz)
+// as the previous step left it: direct A-B read, or the A-A face store of `face`.
+template <class St, bool AA>
+struct PrevPost {
+    const typename St::T* f;  // A-B: previous post-collision buffer
+    const float* store;       // A-A: face store of the previous step
+    size_t N;
+    __device__ __forceinline__ float get(const StepParams& P, int face, int q, int sx, int sy, int sz) const {
+        if (!AA) return St::ld(f + (size_t)q * N + (size_t)((sx * P.ny + sy) * P.nz + sz));
+        const int axis = face >> 1;
+        const int layer = (face & 1) ? ((axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz) - 1 -
+                                        (axis == 0 ? sx : axis == 1 ? sy : sz))
+                                     : (axis == 0 ? sx : axis == 1 ? sy : sz);
+        const int ix = q / 9, iy = (q / 3) % 3, iz = q % 3;
+        const int j = axis == 0 ? iy * 3 + iz : axis == 1 ? ix * 3 + iz : ix * 3 + iy;
+        const int A = axis == 0 ? P.ny : P.nx;
+        const int B = axis == 2 ? P.ny : P.nz;
+        const int a = axis == 0 ? sy : sx;
+        const int b = axis == 2 ? sy : sz;
+        return store[P.faces_dev[face].store_off + ((size_t)(layer * 9 + j) * A + a) * B + b];
+    }
+};
+
+// Incoming value for an out-of-domain link q at fluid cell (x,y,z); `bounce` is the
+// cell's own post-collision value of opp(q) (the reference's halfway bounce-back).
+template <class St, bool AA>
+__device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* flags, const float* ref_p, const PrevPost<St, AA>& prev,
+                           int cell, int x, int y, int z, int q, float bounce) {
+    const int cx = lat_cx(q), cy = lat_cy(q), cz = lat_cz(q);
+    const int sx = wrapc(x - cx, P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
+    const int sy = wrapc(y - cy, P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);
+    const int sz = wrapc(z - cz, P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]);
+    const int face = oob_face(P, sx, sy, sz);
+    const FaceParams fc = P.faces_dev[face];
+    int kind = fc.hydro_kind;
+    if (kind == 0 || kind == 1) kind = P.convective_disabled && kind == 0 ? 2 : 6;
+    if (kind == 2) return bounce;
+    if (kind >= 3 && kind <= 5) {
+        // ref: jni.cpp:1518-1561
+        float rho = 1.0f + clampf(ref_p[cell], -0.03f, 0.03f);
+        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+        if (kind != 5) {
+            ux = fc.vel[0];
+            uy = fc.vel[1];
+            uz = fc.vel[2];
+        } else {
+            rho = 1.0f + clampf(fc.pressure, -0.03f, 0.03f);
+        }
+        rho = clampf(rho, 0.97f, 1.03f);
+        const float s2 = ux * ux + uy * uy + uz * uz;
+        if (s2 > P.kmax_speed * P.kmax_speed && s2 > 0.0f) {
+            const float sc = P.kmax_speed / sqrtf(s2);
+            ux *= sc;
+            uy *= sc;
+            uz *= sc;
+        }
+        return feq_s<St::kShifted>(q, rho, ux, uy, uz);
+    }
+    if (kind == 7) {
+        // symmetry: copy the clamped source (ref: jni.cpp:1582-1587)
+        const int ix = icl(sx, 0, P.nx - 1), iy = icl(sy, 0, P.ny - 1), iz = icl(sz, 0, P.nz - 1);
+        if (flags[(ix * P.ny + iy) * P.nz + iz] & kCellSolid) return bounce;
+        return prev.get(P, face, q, ix, iy, iz);
+    }
+    // convective outflow f1 + 0.15 (f1 - f2) (ref: jni.cpp:1494-1516)
+    const int dx = (sx < 0 || sx >= P.nx) ? cx : 0;
+    const int dy = (sy < 0 || sy >= P.ny) ? cy : 0;
+    const int dz = (sz < 0 || sz >= P.nz) ? cz : 0;
+    const int x1 = icl(x + dx, 0, P.nx - 1), y1 = icl(y + dy, 0, P.ny - 1), z1 = icl(z + dz, 0, P.nz - 1);
+    if (flags[(x1 * P.ny + y1) * P.nz + z1] & kCellSolid) return bounce;
+    const float f1 = prev.get(P, face, q, x1, y1, z1);
+    const int x2 = icl(x + 2 * dx, 0, P.nx - 1), y2 = icl(y + 2 * dy, 0, P.ny - 1), z2 = icl(z + 2 * dz, 0, P.nz - 1);
+    if (flags[(x2 * P.ny + y2) * P.nz + z2] & kCellSolid) return f1;
+    const float f2 = prev.get(P, face, q, x2, y2, z2);
+    return f1 + 0.15f * (f1 - f2);  // the w shift cancels: coefficients sum to one
+}
+
+__device__ __forceinline__ bool link_oob(const StepParams& P, int x, int y, int z, int q) {
+    const int sx = wrapc(x - lat_cx(q), P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
+    const int sy = wrapc(y - lat_cy(q), P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);
+    const int sz = wrapc(z - lat_cz(q), P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]);
+    return sx < 0 || sy < 0 || sz < 0 || sx >= P.nx || sy >= P.ny || sz >= P.nz;
+}
+
+// ---------------------------------------------------------------- temperature
+__device__ __forceinline__ float t_at(const StepParams& P, const float* T, const uint8_t* flags, int x, int y, int z,
+                                      float self) {
+    if ((x < 0 && P.thermal_wrap_lo[0]) || (x >= P.nx && P.thermal_wrap_hi[0])) { x %= P.nx; if (x < 0) x += P.nx; }
+    if ((y < 0 && P.thermal_wrap_lo[1]) || (y >= P.ny && P.thermal_wrap_hi[1])) { y %= P.ny; if (y < 0) y += P.ny; }
+    if ((z < 0 && P.thermal_wrap_lo[2]) || (z >= P.nz && P.thermal_wrap_hi[2])) { z %= P.nz; if (z < 0) z += P.nz; }
+    if (x < 0 || y < 0 || z < 0 || x >= P.nx || y >= P.ny || z >= P.nz) {
+        const FaceParams fc = P.faces_dev[oob_face(P, x, y, z)];
+        if (fc.thermal_kind == 4) return clampf(fc.temperature, -1.0f, 1.0f);
+        if (fc.thermal_kind == 5) return clampf(self + fc.heat_flux, -1.0f, 1.0f);
+        return self;
+    }
+    const int c = (x * P.ny + y) * P.nz + z;
+    if (flags[c] & kCellSolid) return self;
+    return T[c];
+}
+
+// first-order semi-Lagrangian advection + explicit diffusion + source
+// (ref: native/src/aero_lbm_jni.cpp:2145-2188)
+__device__ __forceinline__ float temperature_step(const StepParams& P, const float* __restrict__ T,
+                                                  const uint8_t* __restrict__ flags, const float* __restrict__ src,
+                                                  int cell, int x, int y, int z, const Nbr& nb, bool interior,
+                                                  float ux, float uy, float uz) {
+    const float tc = T[cell];
+    const float px = (float)x - ux, py = (float)y - uy, pz = (float)z - uz;
+    const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
+    const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
+    float c000, c100, c010, c110, c001, c101, c011, c111, tsum;
+    if (interior) {
+        // all 26 neighbours are fluid and in the domain: plain indexed reads
+        const int ax = (int)fx0 < x ? nb.xm : 0, bx = (int)fx0 < x ? 0 : nb.xp;
+        const int ay = (int)fy0 < y ? nb.ym : 0, by = (int)fy0 < y ? 0 : nb.yp;
+        const int az = (int)fz0 < z ? nb.zm : 0, bz = (int)fz0 < z ? 0 : nb.zp;
+        c000 = T[cell + ax + ay + az];
+        c100 = T[cell + bx + ay + az];
+        c010 = T[cell + ax + by + az];
+        c110 = T[cell + bx + by + az];
+        c001 = T[cell + ax + ay + bz];
+        c101 = T[cell + bx + ay + bz];
+        c011 = T[cell + ax + by + bz];
+        c111 = T[cell + bx + by + bz];
+        tsum = T[cell + nb.xp] + T[cell + nb.xm] + T[cell + nb.yp] + T[cell + nb.ym] + T[cell + nb.zp] +
+               T[cell + nb.zm];
+    } else {
+        const int x0 = (int)fx0, y0 = (int)fy0, z0 = (int)fz0;
+        c000 = t_at(P, T, flags, x0, y0, z0, tc);
+        c100 = t_at(P, T, flags, x0 + 1, y0, z0, tc);
+        c010 = t_at(P, T, flags, x0, y0 + 1, z0, tc);
+        c110 = t_at(P, T, flags, x0 + 1, y0 + 1, z0, tc);
+        c001 = t_at(P, T, flags, x0, y0, z0 + 1, tc);
+        c101 = t_at(P, T, flags, x0 + 1, y0, z0 + 1, tc);
+        c011 = t_at(P, T, flags, x0, y0 + 1, z0 + 1, tc);
+        c111 = t_at(P, T, flags, x0 + 1, y0 + 1, z0 + 1, tc);
+        tsum = t_at(P, T, flags, x + 1, y, z, tc) + t_at(P, T, flags, x - 1, y, z, tc) +
+               t_at(P, T, flags, x, y + 1, z, tc) + t_at(P, T, flags, x, y - 1, z, tc) +
+               t_at(P, T, flags, x, y, z + 1, tc) + t_at(P, T, flags, x, y, z - 1, tc);
+    }
+    const float c00 = c000 + (c100 - c000) * fx;
+    const float c10 = c010 + (c110 - c010) * fx;
+    const float c01 = c001 + (c101 - c001) * fx;
+    const float c11 = c011 + (c111 - c011) * fx;
+    const float c0 = c00 + (c10 - c00) * fy;
+    const float c1 = c01 + (c11 - c01) * fy;
+    const float adv = c0 + (c1 - c0) * fz;
+    const float lap = tsum - 6.0f * tc;
+    const float source = (P.source_on && P.in_ch > 9) ? clampf(src[cell], -0.006f, 0.006f) : 0.0f;
+    return clampf(adv + (P.alpha * lap + source), -1.0f, 1.0f);
+}
+
+// ---------------------------------------------------------------- collision
+__device__ __forceinline__ void fwd_line(float& a, float& b, float& c) {
+    // populations (c=-1, c=0, c=+1) -> raw moments (order 0, 1, 2)
+    const float s = a + c, t = c - a;
+    a = s + b;
+    b = t;
+    c = s;
+}
+__device__ __forceinline__ void shift_line(float& m0, float& m1, float& m2, float u, float u2) {
+    // raw -> central about u
+    const float k1 = fmaf(-u, m0, m1);
+    m2 = fmaf(-2.0f * u, m1, fmaf(u2, m0, m2));
+    m1 = k1;
+}
+__device__ __forceinline__ void unshift_line(float& k0, float& k1, float& k2, float u, float u2) {
+    // central about u -> raw
+    k2 = fmaf(2.0f * u, k1, fmaf(u2, k0, k2));
+    k1 = fmaf(u, k0, k1);
+}
+__device__ __forceinline__ void inv_line(float& m0, float& m1, float& m2) {
+    // raw moments -> populations (c=-1, c=0, c=+1)
+    const float h2 = 0.5f * m2, h1 = 0.5f * m1;
+    const float fm = h2 - h1, fp = h2 + h1;
+    m0 = m0 - m2;
+    m1 = fm;  // temporarily c=-1 in slot 1
+    m2 = fp;
+}
+
+#define IDX(a, b, c) (((a) * 3 + (b)) * 3 + (c))
+
+__device__ __forceinline__ void forward_raw(float f[27]) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) fwd_line(f[i * 3 + 0], f[i * 3 + 1], f[i * 3 + 2]);  // z
+#pragma unroll
+    for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) fwd_line(f[IDX(ix, 0, kz)], f[IDX(ix, 1, kz)], f[IDX(ix, 2, kz)]);  // y
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) fwd_line(f[IDX(0, ky, kz)], f[IDX(1, ky, kz)], f[IDX(2, ky, kz)]);  // x
+}
+
+__device__ __forceinline__ void inverse_raw(float m[27]) {
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) {
+            inv_line(m[IDX(0, ky, kz)], m[IDX(1, ky, kz)], m[IDX(2, ky, kz)]);
+            const float t = m[IDX(0, ky, kz)];  // c=0 value
+            m[IDX(0, ky, kz)] = m[IDX(1, ky, kz)];
+            m[IDX(1, ky, kz)] = t;
+        }
+#pragma unroll
+    for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) {
+            inv_line(m[IDX(ix, 0, kz)], m[IDX(ix, 1, kz)], m[IDX(ix, 2, kz)]);
+            const float t = m[IDX(ix, 0, kz)];
+            m[IDX(ix, 0, kz)] = m[IDX(ix, 1, kz)];
+            m[IDX(ix, 1, kz)] = t;
+        }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        inv_line(m[i * 3 + 0], m[i * 3 + 1], m[i * 3 + 2]);
+        const float t = m[i * 3 + 0];
+        m[i * 3 + 0] = m[i * 3 + 1];
+        m[i * 3 + 1] = t;
+    }
+}
+
+// 1-D lattice moments of the weights: sum_c w(c) c^n = 1, 0, 1/3, 0, 1/3 for n = 0..4
+__device__ constexpr float mu1(int n) { return n == 0 ? 1.0f : ((n & 1) ? 0.0f : 1.0f / 3.0f); }
+__device__ constexpr float emom(int a, int b, int c) { return mu1(a) * mu1(b) * mu1(c); }
+
+// raw moment (a,b,c) of the Guo source S_q = w_q [3 (c-u).F + 9 (c.u)(c.F)] / P
+template <int A, int B, int C>
+__device__ __forceinline__ float guo_moment(const float F[3], float uF, const float uaFb[3][3]) {
+    float v = 0.0f;
+    constexpr int e[3] = {A, B, C};
+#pragma unroll
+    for (int beta = 0; beta < 3; ++beta) {
+        const float k = 3.0f * emom(A + (beta == 0), B + (beta == 1), C + (beta == 2));
+        if (k != 0.0f) v = fmaf(k, F[beta], v);
+    }
+    {
+        const float k = -3.0f * emom(A, B, C);
+        if (k != 0.0f) v = fmaf(k, uF, v);
+    }
+#pragma unroll
+    for (int al = 0; al < 3; ++al)
+#pragma unroll
+        for (int beta = 0; beta < 3; ++beta) {
+            const float k = 9.0f * emom(A + (al == 0) + (beta == 0), B + (al == 1) + (beta == 1),
+                                        C + (al == 2) + (beta == 2));
+            if (k != 0.0f) v = fmaf(k, uaFb[al][beta], v);
+        }
+    (void)e;
+    return v;
+}
+
+struct CollideResult {
+    float ux, uy, uz;
+    bool ok;
+};
+
+// g: populations in stored form (f - w when Shifted), overwritten with the post-collision values
+template <bool Shifted>
+__device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27], float T) {
+    CollideResult r;
+    forward_raw(g);
+    float rho;
+    if (Shifted) {
+        const float drho = g[IDX(0, 0, 0)];
+        rho = 1.0f + drho;
+        // add the weights' own raw moments (1, 1/3, 1/9, 1/27 on the even ones)
+        g[IDX(0, 0, 0)] = rho;
+        g[IDX(2, 0, 0)] += 1.0f / 3.0f;
+        g[IDX(0, 2, 0)] += 1.0f / 3.0f;
+        g[IDX(0, 0, 2)] += 1.0f / 3.0f;
+        g[IDX(2, 2, 0)] += 1.0f / 9.0f;
+        g[IDX(2, 0, 2)] += 1.0f / 9.0f;
+        g[IDX(0, 2, 2)] += 1.0f / 9.0f;
+        g[IDX(2, 2, 2)] += 1.0f / 27.0f;
+    } else {
+        rho = g[IDX(0, 0, 0)];
+    }
+    const float jx = g[IDX(1, 0, 0)], jy = g[IDX(0, 1, 0)], jz = g[IDX(0, 0, 1)];
+    r.ok = finite(rho) && finite(jx) && finite(jy) && finite(jz);
+    if (!r.ok) return r;
+    const float rho_s = fmaxf(1e-6f, rho);
+    const float inv = 1.0f / rho_s;
+    float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    // Boussinesq force along the gravity axis; F/rho = clamp(beta T)
+    float b = 0.0f;
+    if (P.buoyancy_on) {
+        b = clampf(P.beta * T, -0.02f, 0.02f);
+        if (P.g_sign > 0) b = -b;
+        if (P.g_axis == 0) ux = fmaf(0.5f, b, ux);
+        else if (P.g_axis == 1) uy = fmaf(0.5f, b, uy);
+        else uz = fmaf(0.5f, b, uz);
+    }
+    {
+        const float s2 = ux * ux + uy * uy + uz * uz;
+        if (s2 > P.kmax_speed * P.kmax_speed) {
+            const float sc = P.kmax_speed / sqrtf(s2);
+            ux *= sc;
+            uy *= sc;
+            uz *= sc;
+        }
+    }
+    r.ux = ux;
+    r.uy = uy;
+    r.uz = uz;
+    const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    // raw -> central
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) shift_line(g[IDX(0, ky, kz)], g[IDX(1, ky, kz)], g[IDX(2, ky, kz)], ux, ux2);
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) shift_line(g[IDX(kx, 0, kz)], g[IDX(kx, 1, kz)], g[IDX(kx, 2, kz)], uy, uy2);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) shift_line(g[i * 3 + 0], g[i * 3 + 1], g[i * 3 + 2], uz, uz2);
+
+    const float cs2 = 1.0f / 3.0f;
+    const float c200 = g[IDX(2, 0, 0)], c020 = g[IDX(0, 2, 0)], c002 = g[IDX(0, 0, 2)];
+    const float c110 = g[IDX(1, 1, 0)], c101 = g[IDX(1, 0, 1)], c011 = g[IDX(0, 1, 1)];
+    float ts = P.tau_shear_base, tn = P.tau_normal_base;
+    if (P.sgs_on) {
+        // ref: jni.cpp:2346-2366
+        const float nu0 = fmaxf(1e-6f, P.nu_base);
+        const float e = rho_s * cs2;
+        const float nxx = c200 - e, nyy = c020 - e, nzz = c002 - e;
+        const float qn2 = nxx * nxx + nyy * nyy + nzz * nzz + 2.0f * (c110 * c110 + c101 * c101 + c011 * c011);
+        const float smag = sqrtf(fmaxf(0.0f, qn2)) / fmaxf(1e-6f, 2.0f * rho_s * nu0);
+        const float nut = fminf(P.sgs_c2 * smag, P.nut_cap_ratio * nu0);
+        const float tau_min = 0.5f + 3.0f * 1.0e-8f;
+        ts = clampf(0.5f + 3.0f * clampf(nu0 + nut, 1.0e-8f, (0.95f - 0.5f) / 3.0f), tau_min, 0.95f);
+        tn = clampf(P.tau_normal_base + (ts - P.tau_shear_base) * 0.30f, tau_min, 0.95f);
+    }
+    const float sd = clampf(1.0f / tn, 0.0f, 1.95f);
+    const float so = clampf(1.0f / ts, 0.0f, 1.95f);
+    // relaxation + Gaussian closure (ref: jni.cpp:1384-1430)
+    const float eq2 = rho_s * cs2;
+    const float d200 = fmaf(sd, eq2 - c200, c200);
+    const float d020 = fmaf(sd, eq2 - c020, c020);
+    const float d002 = fmaf(sd, eq2 - c002, c002);
+    const float om = 1.0f - so;
+    const float d110 = om * c110, d101 = om * c101, d011 = om * c011;
+    const float ir = 1.0f / rho_s;
+    const float t110 = 2.0f * d110, t101 = 2.0f * d101, t011 = 2.0f * d011;
+    const float d220 = (d200 * d020 + t110 * d110) * ir;
+    const float d202 = (d200 * d002 + t101 * d101) * ir;
+    const float d022 = (d020 * d002 + t011 * d011) * ir;
+    const float d211 = (d200 * d011 + t110 * d101) * ir;
+    const float d121 = (d020 * d101 + t110 * d011) * ir;
+    const float d112 = (d002 * d110 + t101 * d011) * ir;
+    const float d222 = (d200 * d020 * d002 + t110 * d110 * d002 + t101 * d101 * d020 + t011 * d011 * d200 +
+                        8.0f * d110 * d101 * d011) * ir * ir;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) g[k] = 0.0f;
+    g[IDX(0, 0, 0)] = rho_s;
+    g[IDX(2, 0, 0)] = d200;
+    g[IDX(0, 2, 0)] = d020;
+    g[IDX(0, 0, 2)] = d002;
+    g[IDX(1, 1, 0)] = d110;
+    g[IDX(1, 0, 1)] = d101;
+    g[IDX(0, 1, 1)] = d011;
+    g[IDX(2, 2, 0)] = d220;
+    g[IDX(2, 0, 2)] = d202;
+    g[IDX(0, 2, 2)] = d022;
+    g[IDX(2, 1, 1)] = d211;
+    g[IDX(1, 2, 1)] = d121;
+    g[IDX(1, 1, 2)] = d112;
+    g[IDX(2, 2, 2)] = d222;
+    // central -> raw
+#pragma unroll
+    for (int i = 0; i < 9; ++i) unshift_line(g[i * 3 + 0], g[i * 3 + 1], g[i * 3 + 2], uz, uz2);
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) unshift_line(g[IDX(kx, 0, kz)], g[IDX(kx, 1, kz)], g[IDX(kx, 2, kz)], uy, uy2);
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) unshift_line(g[IDX(0, ky, kz)], g[IDX(1, ky, kz)], g[IDX(2, ky, kz)], ux, ux2);
+    // Guo source as raw moments (ref: jni.cpp:1270-1281, added after the collision)
+    if (P.buoyancy_on && b != 0.0f) {
+        const float pref = fmaxf(0.0f, 1.0f - 0.5f * clampf(1.0f / ts, 0.0f, 1.999f));
+        const float fa = pref * rho_s * b;  // P * F along the axis
+        const float F[3] = {P.g_axis == 0 ? fa : 0.0f, P.g_axis == 1 ? fa : 0.0f, P.g_axis == 2 ? fa : 0.0f};
+        const float uF = ux * F[0] + uy * F[1] + uz * F[2];
+        const float u3[3] = {ux, uy, uz};
+        float uaFb[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) uaFb[a][c] = u3[a] * F[c];
+#define GM(a, b, c) g[IDX(a, b, c)] += guo_moment<a, b, c>(F, uF, uaFb);
+        GM(0, 0, 0) GM(0, 0, 1) GM(0, 0, 2) GM(0, 1, 0) GM(0, 1, 1) GM(0, 1, 2) GM(0, 2, 0) GM(0, 2, 1) GM(0, 2, 2)
+        GM(1, 0, 0) GM(1, 0, 1) GM(1, 0, 2) GM(1, 1, 0) GM(1, 1, 1) GM(1, 1, 2) GM(1, 2, 0) GM(1, 2, 1) GM(1, 2, 2)
+        GM(2, 0, 0) GM(2, 0, 1) GM(2, 0, 2) GM(2, 1, 0) GM(2, 1, 1) GM(2, 1, 2) GM(2, 2, 0) GM(2, 2, 1) GM(2, 2, 2)
+#undef GM
+    }
+    if (Shifted) {
+        g[IDX(0, 0, 0)] -= 1.0f;
+        g[IDX(2, 0, 0)] -= 1.0f / 3.0f;
+        g[IDX(0, 2, 0)] -= 1.0f / 3.0f;
+        g[IDX(0, 0, 2)] -= 1.0f / 3.0f;
+        g[IDX(2, 2, 0)] -= 1.0f / 9.0f;
+        g[IDX(2, 0, 2)] -= 1.0f / 9.0f;
+        g[IDX(0, 2, 2)] -= 1.0f / 9.0f;
+        g[IDX(2, 2, 2)] -= 1.0f / 27.0f;
+    }
+    inverse_raw(g);
+    return r;
+}
+
+// reference reseed for a non-finite cell (jni.cpp:1724-1750)
+template <bool Shifted>
+__device__ __forceinline__ void reseed(const StepParams& P, const CellFields& C, int cell, float g[27], CollideResult& r) {
+    const size_t N = (size_t)P.cells;
+    const float p = clampf(finite(C.ref_p[cell]) ? C.ref_p[cell] : 0.0f, -0.03f, 0.03f);
+    const float rho = clampf(1.0f + p, 0.97f, 1.03f);
+    float ux = C.ref_u[cell], uy = C.ref_u[N + cell], uz = C.ref_u[2 * N + cell];
+    ux = finite(ux) ? ux : 0.0f;
+    uy = finite(uy) ? uy : 0.0f;
+    uz = finite(uz) ? uz : 0.0f;
+    const float s2 = ux * ux + uy * uy + uz * uz;
+    if (!finite(s2) || s2 > P.kmax_speed * P.kmax_speed) {
+        if (!finite(s2) || s2 <= 0.0f) ux = uy = uz = 0.0f;
+        else {
+            const float sc = P.kmax_speed / sqrtf(s2);
+            ux *= sc;
+            uy *= sc;
+            uz *= sc;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) g[q] = feq_s<Shifted>(q, rho, ux, uy, uz);
+    r.ux = ux;
+    r.uy = uy;
+    r.uz = uz;
+    r.ok = true;
+}
+
+template <bool Shifted>
+__device__ __forceinline__ void sponge(const StepParams& P, int x, int y, int z, float g[27]) {
+    int d = min(min(x, y), z);
+    d = min(d, min(min(P.nx - 1 - x, P.ny - 1 - y), P.nz - 1 - z));
+    if (d >= 4) return;
+    const float eta = (float)(4 - d) * 0.25f;
+    const float al = clampf(0.03f * eta * eta, 0.0f, 0.95f);
+    const float keep = 1.0f - al;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) g[q] = keep * g[q] + al * (Shifted ? 0.0f : wq(q));
+}
+
+}  // namespace fast
+}  // namespace aero

@@ -11,7 +11,8 @@ enum class FastStorage { F32, F16, BF16 };
 // domain after periodic wrapping; plus the two flags below
 constexpr uint32_t kMaskLinks = (1u << 27) - 1u;
 constexpr uint32_t kMaskSolid = 1u << 27;  // the cell itself is solid
-constexpr uint32_t kMaskFace = 1u << 28;   // at least one link leaves the domain
+constexpr uint32_t kMaskFace = 1u << 28;   // a link leaves through a face whose rule is not plain bounce-back
+constexpr uint32_t kMaskTherm = 1u << 29;  // a stencil neighbour is outside the domain at a Dirichlet/Neumann thermal face
 
 void launch_fast_mask(const StepParams& P, const uint8_t* flags, uint32_t* mask, hipStream_t s);
 void launch_fast_init(const StepParams& P, const CellFields& C, FastStorage st, void* f, float* T, float* T_other, void* u_state,

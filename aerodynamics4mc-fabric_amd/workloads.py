@@ -191,3 +191,77 @@ def reference_overrides(kernel_variant: int = 0):
     ov.gravity_sign = 0
     ov.kernel_variant = kernel_variant
     return ov
+
+
+# ---------------------------------------------------------------- mesoscale (D3Q19) workloads
+# 25 forcing channels per cell (ref: native/src/aero_lbm_mesoscale.cpp:110-139); y is vertical.
+MESO_CHANNELS = 25
+
+
+def value_noise_2d(nx: int, nz: int, seed: int, octaves: int = 4, base_cells: int = 64) -> np.ndarray:
+    """Seeded multi-octave value noise in [0, 1) on an nx x nz grid (bilinear lattice noise)."""
+    out = np.zeros((nx, nz))
+    amp_total = 0.0
+    amp = 1.0
+    period = base_cells
+    for o in range(octaves):
+        gx, gz = nx // period + 2, nz // period + 2
+        lattice = splitmix_array(seed * 1000 + o, gx * gz).reshape(gx, gz)
+        xs = np.arange(nx) / period
+        zs = np.arange(nz) / period
+        x0 = np.floor(xs).astype(int)
+        z0 = np.floor(zs).astype(int)
+        fx = (xs - x0)[:, None]
+        fz = (zs - z0)[None, :]
+        a = lattice[x0][:, z0]
+        b = lattice[x0 + 1][:, z0]
+        c = lattice[x0][:, z0 + 1]
+        d = lattice[x0 + 1][:, z0 + 1]
+        out += amp * ((a * (1 - fx) + b * fx) * (1 - fz) + (c * (1 - fx) + d * fx) * fz)
+        amp_total += amp
+        amp *= 0.5
+        period = max(1, period // 2)
+    return out / amp_total
+
+
+def meso_forcing(nx: int, ny: int, nz: int, wind_lattice: float = 0.04, angle_deg: float = 30.0,
+                 velocity_scale: float = 12.8, max_height: float = 24.0, seed: int = 3, z0: float = 0.1,
+                 ambient: float = 288.15, deep: float = 289.65, roughness: float = 0.1) -> np.ndarray:
+    """Terrain-following mesoscale forcing: heightmap from 4-octave value noise (heights
+    0..max_height cells) as solid cells below the surface, geostrophic wind of
+    `wind_lattice` lattice units rotated `angle_deg` from +x with a log-law profile in
+    the vertical (y), ambient/surface targets at `ambient`, deep ground at `deep`."""
+    h = value_noise_2d(nx, nz, seed) * max_height
+    y = np.arange(ny)[None, :, None].astype(np.float64)
+    hh = h[:, None, :]
+    solid = (y < np.floor(hh)).astype(np.float32)
+    above = np.maximum(y - np.floor(hh) + 0.5, 0.0)
+    profile = np.log1p(above / z0) / np.log1p((ny - 0.5) / z0)
+    u_ms = wind_lattice * velocity_scale
+    ang = np.deg2rad(angle_deg)
+    F = np.zeros((nx, ny, nz, MESO_CHANNELS), dtype=np.float32)
+    F[..., 0] = np.broadcast_to(hh, (nx, ny, nz))
+    F[..., 2] = ambient
+    F[..., 3] = deep
+    F[..., 4] = ambient
+    F[..., 5] = roughness
+    F[..., 6] = (u_ms * np.cos(ang) * profile).astype(np.float32)
+    F[..., 7] = (u_ms * np.sin(ang) * profile).astype(np.float32)
+    F[..., 24] = solid
+    return F.reshape(-1)
+
+
+def meso_config(nx: int, ny: int, nz: int, nu_lattice: float = 5e-4, dx: float = 64.0, dt: float = 5.0):
+    """Mesoscale config whose molecular viscosity gives lattice viscosity `nu_lattice`."""
+    cfg = A.MesoscaleConfig()
+    cfg.abi_version = 3
+    cfg.struct_size = 96
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.dx_m, cfg.dt_s = dx, dt
+    cfg.molecular_nu_m2_s = nu_lattice * dx * dx / dt
+    cfg.prandtl_air = 0.71
+    cfg.turbulent_prandtl = 0.85
+    cfg.reference_density_kg_m3 = 1.225
+    cfg.ambient_air_temperature_k = 288.15
+    cfg.deep_ground_temperature_k = 289.65
+    return cfg

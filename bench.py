@@ -97,6 +97,7 @@ def cpu_baseline(n, inputs, sample_steps=2):
     step) on this host, one core, on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
+    pyoracle.set_threads(1)  # the reference CPU step is single-threaded
     cfg, ov, packet, temp, solid = inputs
     o = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=32)
     o.set_temperature(temp)
@@ -126,7 +127,19 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--modes", default="fp16,fp32")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) five times, "
+                         "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_each_way": src.numel() * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the solver has no multi-GPU path", file=sys.stderr)

@@ -171,3 +171,43 @@ def load_ref():
     lib.ref_lattice_diffusivity.restype = c_float
     lib.ref_lattice_diffusivity.argtypes = [c_float] * 3
     return lib
+
+
+class MesoOracle:
+    """CPU restatement of the reference's D3Q19 mesoscale step (precision 32 or 64)."""
+
+    def __init__(self, nx, ny, nz, dx=64.0, dt=5.0, nu=1.5e-5, pr=0.71, prt=0.85, tau_min=0.58, precision=32):
+        self.lib = load()
+        self.p = "m32_" if precision == 32 else "m64_"
+        self.real = np.float32 if precision == 32 else np.float64
+        ct = np.ctypeslib.as_ctypes_type(self.real)
+        f = getattr(self.lib, self.p + "meso_create")
+        f.restype = c_void_p
+        f.argtypes = [c_int] * 3 + [c_float] * 6
+        getattr(self.lib, self.p + "meso_destroy").argtypes = [c_void_p]
+        getattr(self.lib, self.p + "meso_step").argtypes = [c_void_p, POINTER(c_float), POINTER(c_float)]
+        getattr(self.lib, self.p + "meso_step").restype = c_int
+        getattr(self.lib, self.p + "meso_get_f").argtypes = [c_void_p, POINTER(ct), POINTER(ct)]
+        self.cells = nx * ny * nz
+        self.h = f(nx, ny, nz, dx, dt, nu, pr, prt, tau_min)
+        if not self.h:
+            raise ValueError("invalid mesoscale domain")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self.p + "meso_destroy")(self.h)
+            self.h = None
+
+    def step(self, forcing):
+        forcing = np.ascontiguousarray(forcing, dtype=np.float32).reshape(-1)
+        out = np.zeros(self.cells * 6, dtype=np.float32)
+        if not getattr(self.lib, self.p + "meso_step")(self.h, _fp(forcing), _fp(out)):
+            raise RuntimeError("mesoscale oracle step failed")
+        return out
+
+    def distributions(self):
+        ct = np.ctypeslib.as_ctypes_type(self.real)
+        f = np.zeros(self.cells * 19, dtype=self.real)
+        g = np.zeros(self.cells * 19, dtype=self.real)
+        getattr(self.lib, self.p + "meso_get_f")(self.h, f.ctypes.data_as(POINTER(ct)), g.ctypes.data_as(POINTER(ct)))
+        return f.reshape(self.cells, 19), g.reshape(self.cells, 19)
