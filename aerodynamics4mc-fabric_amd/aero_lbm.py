@@ -20,7 +20,8 @@ from ctypes import (POINTER, Structure, c_char_p, c_float, c_int, c_longlong,
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libaero_lbm.so")
+# AERO_LBM_LIBRARY selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("AERO_LBM_LIBRARY") or os.path.join(_HERE, "lib", "libaero_lbm.so")
 
 # ---- enums (native/include/aero_lbm_capi.h:16-74) ----
 SOLVER_AUTO = 0

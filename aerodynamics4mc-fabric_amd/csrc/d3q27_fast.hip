@@ -21,20 +21,28 @@
 // take a path with no per-link selects; the face-rule path runs only for cells
 // at faces whose rule is not plain bounce-back, and the general temperature
 // stencil only next to Dirichlet / Neumann thermal faces.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
 #include "fast_core.h"
 
 namespace aero {
 namespace fast {
 
-constexpr int kTileZ = 64;
+constexpr int kTileZ = 64;  // one wavefront = 64 consecutive z cells of one (x, y) row
 constexpr int kTileY = 4;
+static_assert(kTileZ == 64, "the step kernel treats x and y as wave-uniform");
 
 struct TileCoord {
-    int x, y, z;
+    int x, y, z;       // x and y are wave-uniform (scalar registers)
+    bool row_valid;    // wave-uniform
     bool valid;
 };
 
-// XCD-aware decode of the cell handled by this thread
+// XCD-aware decode of the cell handled by this thread. Workgroups are dispatched
+// round-robin over the 8 XCDs; logical tile (L & 7) * per + (L >> 3) gives each
+// XCD one contiguous eighth of the brick.
 __device__ __forceinline__ TileCoord tile_cell(const StepParams& P) {
     const int nbz = (P.nz + kTileZ - 1) / kTileZ;
     const int nby = (P.ny + kTileY - 1) / kTileY;
@@ -42,17 +50,71 @@ __device__ __forceinline__ TileCoord tile_cell(const StepParams& P) {
     const int per = (nb + 7) >> 3;
     const int L = blockIdx.x;
     const int logical = (L & 7) * per + (L >> 3);
-    TileCoord t;
-    t.valid = logical < nb;
     const int bx = logical / (nbz * nby);
     const int r = logical - bx * nbz * nby;
     const int by = r / nbz;
     const int bz = r - by * nbz;
+    TileCoord t;
     t.x = bx;
-    t.y = by * kTileY + (int)threadIdx.y;
+    // a wavefront holds threads 64w..64w+63 of the 64 x 4 workgroup: one threadIdx.y
+    t.y = by * kTileY + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     t.z = bz * kTileZ + (int)threadIdx.x;
-    t.valid = t.valid && t.y < P.ny && t.z < P.nz;
+    t.row_valid = logical < nb && t.y < P.ny;
+    t.valid = t.row_valid && t.z < P.nz;
     return t;
+}
+
+// Wave-uniform neighbour rows of the wave's (x, y) row: element offset of row
+// (x + dx, y + dy) relative to row (x, y) after periodic wrap, and whether that
+// row is inside the domain.
+struct RowNbr {
+    int row;  // element index of cell (x, y, 0)
+    int xo[3], yo[3];
+    int okbits;  // bit (dx + 1) * 3 + (dy + 1): row (x + dx, y + dy) is inside the domain
+    __device__ __forceinline__ int off(int dx, int dy) const { return row + xo[dx + 1] + yo[dy + 1]; }
+    __device__ __forceinline__ bool ok(int dx, int dy) const { return (okbits >> ((dx + 1) * 3 + dy + 1)) & 1; }
+};
+
+__device__ __forceinline__ RowNbr row_nbr(const StepParams& P, int x, int y) {
+    RowNbr R;
+    const int yz = P.ny * P.nz;
+    R.row = (x * P.ny + y) * P.nz;
+    R.xo[1] = R.yo[1] = 0;
+    R.xo[0] = x > 0 ? -yz : (P.hydro_wrap_lo[0] ? (P.nx - 1) * yz : 0);
+    R.xo[2] = x < P.nx - 1 ? yz : (P.hydro_wrap_hi[0] ? -(P.nx - 1) * yz : 0);
+    R.yo[0] = y > 0 ? -P.nz : (P.hydro_wrap_lo[1] ? (P.ny - 1) * P.nz : 0);
+    R.yo[2] = y < P.ny - 1 ? P.nz : (P.hydro_wrap_hi[1] ? -(P.ny - 1) * P.nz : 0);
+    const int xok = 2 | ((x > 0 || P.hydro_wrap_lo[0]) ? 1 : 0) | ((x < P.nx - 1 || P.hydro_wrap_hi[0]) ? 4 : 0);
+    const int yok = 2 | ((y > 0 || P.hydro_wrap_lo[1]) ? 1 : 0) | ((y < P.ny - 1 || P.hydro_wrap_hi[1]) ? 4 : 0);
+    R.okbits = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if ((xok >> i) & 1) R.okbits |= yok << (3 * i);
+    return R;
+}
+
+// An opaque copy of a wave-uniform value: derived values are recomputed from it
+// instead of being kept live in scalar registers across the collision.
+__device__ __forceinline__ int fresh_s(int v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+// Per-lane byte offsets of z - 1, z, z + 1 inside a row (periodic wrap); a
+// neighbour outside the domain gets kDropByte.
+template <class T>
+__device__ __forceinline__ void z_bytes(const StepParams& P, int z, int zb[3]) {
+    constexpr int s = (int)sizeof(T);
+    zb[1] = z * s;
+    zb[0] = z > 0 ? (z - 1) * s : (P.hydro_wrap_lo[2] ? (P.nz - 1) * s : kDropByte);
+    zb[2] = z < P.nz - 1 ? (z + 1) * s : (P.hydro_wrap_hi[2] ? 0 : kDropByte);
+}
+
+// wave-uniform byte offset of row `row_off` of population plane q inside the
+// population array (below 2 GiB)
+template <class T>
+__device__ __forceinline__ int plane_row(int N, int q, int row_off) {
+    return (q * N + row_off) * (int)sizeof(T);
 }
 
 __host__ inline int tile_grid(const StepParams& P) {
@@ -96,44 +158,84 @@ __device__ __forceinline__ float temperature_fast(const StepParams& P, const flo
     return clampf(adv + fmaf(P.alpha, lap, source), -1.0f, 1.0f);
 }
 
-template <class St>
-__device__ __forceinline__ typename St::T* plane(typename St::T* base, size_t N, int q) {
-    return base + (size_t)q * N;
+// ---------------------------------------------------------------- rare paths
+// Rarely taken branches read their parameters through the device copy of the
+// step parameters, so the bulk path keeps no scalar registers for them. (Calls
+// to out-of-line functions would give every wave a scratch segment, which costs
+// more than the branches themselves.)
+__device__ __forceinline__ void copy27(float* d, const float* s) {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) d[q] = s[q];
 }
 
-// Byte offsets of a cell and of its +-1 neighbours along each axis (periodic wrap,
-// out-of-domain neighbours clamped to the cell). A population access is then a
-// wave-uniform plane base plus one per-lane 32-bit byte offset, and the pull
-// source of link q is also the push target of opp(q), so 27 offsets serve both.
-struct ByteNbr {
-    uint32_t cell;
-    int32_t xm, xp, ym, yp, zm, zp;
-    __device__ __forceinline__ uint32_t at(int dx, int dy, int dz) const {
-        return ((cell + (uint32_t)(dx < 0 ? xm : (dx > 0 ? xp : 0))) + (uint32_t)(dy < 0 ? ym : (dy > 0 ? yp : 0))) +
-               (uint32_t)(dz < 0 ? zm : (dz > 0 ? zp : 0));
+// links through faces whose rule is not plain bounce-back (jni.cpp:743-785)
+template <class St, int Mode>
+__device__ __forceinline__ void rare_face_links(const StepParams* __restrict__ Pd, const uint8_t* flags,
+                                             const float* ref_p, const typename St::T* fin, const float* store_in,
+                                             int cell, int x, int y, int z, uint32_t links, float* g) {
+    const StepParams& P = *Pd;
+    const PrevPost<St, Mode != 0> prev{fin, store_in, (size_t)P.plane};
+    for (int q = 0; q < 27; ++q)
+        if (((links >> q) & 1u) && link_oob(P, x, y, z, q))
+            g[q] = face_link<St, Mode != 0>(P, flags, ref_p, prev, cell, x, y, z, q, g[q]);
+}
+
+// temperature next to a Dirichlet / Neumann thermal face, or with thermal
+// periodicity that differs from the hydrodynamic one
+__device__ __forceinline__ float rare_temperature(const StepParams* __restrict__ Pd, const float* T, const uint8_t* flags,
+                                               const float* src, int cell, int x, int y, int z, float ux, float uy,
+                                               float uz) {
+    const StepParams& P = *Pd;
+    return temperature_step(P, T, flags, src, cell, x, y, z, make_nbr(P, x, y, z), false, ux, uy, uz);
+}
+
+// reseed of a non-finite cell, then the edge sponge
+template <bool Shifted>
+__device__ __forceinline__ void rare_after_collide(const StepParams* __restrict__ Pd, CellFields C, int cell, int x, int y,
+                                                int z, float* g, CollideResult* r) {
+    const StepParams& P = *Pd;
+    if (!r->ok) reseed<Shifted>(P, C, cell, g, *r);
+    if (P.sponge_on) sponge<Shifted>(P, x, y, z, g);
+}
+
+// A-A face store: post-collision values of the first three layers of faces whose
+// rule reads the previous step (symmetry, convective)
+__device__ __forceinline__ void rare_face_store(const StepParams* __restrict__ Pd, int x, int y, int z, const float* g,
+                                             float* store_out) {
+    const StepParams& P = *Pd;
+    for (int face = 0; face < 6; ++face) {
+        if (!((P.face_store_faces >> face) & 1)) continue;
+        const int axis = face >> 1;
+        const int coord = axis == 0 ? x : axis == 1 ? y : z;
+        const int n = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
+        const int layer = (face & 1) ? n - 1 - coord : coord;
+        if (layer > 2) continue;
+        const int A = axis == 0 ? P.ny : P.nx;
+        const int B = axis == 2 ? P.ny : P.nz;
+        const int a = axis == 0 ? y : x;
+        const int bb = axis == 2 ? y : z;
+        const int cdir = (face & 1) ? 0 : 2;  // incoming through the min face: c = +1 (index 2)
+        const long long off = P.face_store_off[face];
+        for (int j = 0; j < 9; ++j) {
+            const int q = axis == 0 ? cdir * 9 + j : axis == 1 ? (j / 3) * 9 + cdir * 3 + (j % 3) : j * 3 + cdir;
+            store_out[off + ((size_t)(layer * 9 + j) * A + a) * B + bb] = g[q];
+        }
     }
-};
-
-template <class T>
-__device__ __forceinline__ ByteNbr byte_nbr(int cell, const Nbr& n) {
-    constexpr int s = (int)sizeof(T);
-    return ByteNbr{(uint32_t)(cell * s), n.xm * s, n.xp * s, n.ym * s, n.yp * s, n.zm * s, n.zp * s};
-}
-
-template <class St>
-__device__ __forceinline__ float ld_at(const typename St::T* plane_base, uint32_t byte_off) {
-    return St::ld((const typename St::T*)((const char*)plane_base + byte_off));
-}
-template <class St>
-__device__ __forceinline__ void st_at(typename St::T* plane_base, uint32_t byte_off, float v) {
-    St::st((typename St::T*)((char*)plane_base + byte_off), v);
 }
 
 // ---------------------------------------------------------------- step kernel
 // Mode: 0 = A-B pull, 1 = A-A even (local), 2 = A-A odd (pull + push).
 // local_first: the state is the initial one (incoming values in own slots, no
 // stream pending): collide in place and apply no face rules.
-template <class St, class Uv, int Mode>
+//
+// Streaming runs in two passes. The bulk pass moves every link as if all
+// neighbours were fluid: stores to a neighbour row outside the domain are
+// skipped (a wave-uniform branch) and a neighbour z outside the domain has a
+// dropped byte offset, while accesses to solid neighbours touch their slots,
+// which no fluid cell reads. The fix-up pass then applies
+// halfway bounce-back to the links the cell's mask flags, for only those link
+// directions that some lane of the wave needs (a wave-uniform branch each).
+template <class St, class Uv, int Mode, int StPol>
 __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
                                                    const typename St::T* __restrict__ fin, typename St::T* fout,
                                                    const float* __restrict__ t_in, float* __restrict__ t_out,
@@ -142,66 +244,68 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     const TileCoord tc = tile_cell(P);
     if (!tc.valid) return;
     const int x = tc.x, y = tc.y, z = tc.z;
-    const int cell = (x * P.ny + y) * P.nz + z;
+    const RowNbr R = row_nbr(P, x, y);
+    const int cell = R.row + z;
     const uint32_t m = mask[cell];
     if (m & kMaskSolid) return;  // solid cells hold no populations in the fast paths
     const size_t N = (size_t)P.cells;
-    const Nbr nb = make_nbr(P, x, y, z);
     const uint32_t links = m & kMaskLinks;
     using T = typename St::T;
-    const bool wave_bulk = __ballot(links != 0) == 0;  // no lane of this wave has a special link
+    int zb[3];
+    z_bytes<T>(P, z, zb);
+    const T* const src = Mode == 0 ? fin : fout;
+    const int Ni = P.plane;
+    const Rsrc rin = make_rsrc(src, 27 * Ni * (int)sizeof(T));
+    const Rsrc rout = make_rsrc(fout, 27 * Ni * (int)sizeof(T));
     float g[27];
     // ---- incoming populations
     if (Mode == 1 || local_first) {
-        const T* src = Mode == 0 ? fin : fout;
 #pragma unroll
-        for (int q = 0; q < 27; ++q) g[q] = St::ld(plane<St>((T*)src, N, q) + cell);
-    } else if (wave_bulk) {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-            const T* p = Mode == 0 ? plane<St>((T*)fin, N, q) : plane<St>(fout, N, lat_opp(q));
-            g[q] = St::ld(p + cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q)));
-        }
+        for (int q = 0; q < 27; ++q) g[q] = St::template bld<kPolDefault>(rin, zb[1], plane_row<T>(Ni, q, R.row));
     } else {
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
+            const int dx = -lat_cx(q), dy = -lat_cy(q), dz = -lat_cz(q);
+            const int qs = Mode == 0 ? q : lat_opp(q);
+            // a row outside the domain is read at the cell's own row; the fix-up replaces it
+            const int row = R.ok(dx, dy) ? R.off(dx, dy) : R.row;
+            g[q] = St::template bld<kPolDefault>(rin, zb[dz + 1], plane_row<T>(Ni, qs, row));
+        }
+        // halfway bounce-back: the cell's own post-collision value of opp(q)
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
             const bool sp = (links >> q) & 1u;
-            // special link: the cell's own post-collision value of opp(q) (halfway bounce-back)
-            const int qs = Mode == 0 ? (sp ? lat_opp(q) : q) : (sp ? q : lat_opp(q));
-            const int src = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
-            g[q] = St::ld((Mode == 0 ? fin : fout) + (size_t)qs * N + src);
+            if (__builtin_amdgcn_ballot_w64(sp)) {
+                const int qs = Mode == 0 ? lat_opp(q) : q;
+                if (sp) g[q] = St::template bld<kPolDefault>(rin, zb[1], plane_row<T>(Ni, qs, R.row));
+            }
         }
     }
     // ---- links through faces with a rule other than bounce-back
     if ((m & kMaskFace) && !local_first) {
-        const PrevPost<St, Mode != 0> prev{fin, store_in, N};
-        for (int q = 0; q < 27; ++q) {
-            if (((links >> q) & 1u) && link_oob(P, x, y, z, q)) {
-                float bnc = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 27; ++k)
-                    if (k == q) bnc = g[k];
-                const float v = face_link<St, Mode != 0>(P, C.flags, C.ref_p, prev, cell, x, y, z, q, bnc);
-#pragma unroll
-                for (int k = 0; k < 27; ++k)
-                    if (k == q) g[k] = v;
-            }
-        }
+        float gt[27];
+        copy27(gt, g);
+        rare_face_links<St, Mode>(P.self_dev, C.flags, C.ref_p, fin, store_in, cell, x, y, z, links, gt);
+        copy27(g, gt);
     }
     // ---- temperature
     float temp = 0.0f;
     if (P.thermal_on) {
         const float ux0 = Uv::ld(u_state + cell), uy0 = Uv::ld(u_state + N + cell), uz0 = Uv::ld(u_state + 2 * N + cell);
         if ((m & kMaskTherm) || !P.thermal_wrap_matches)
-            temp = temperature_step(P, t_in, C.flags, C.src, cell, x, y, z, nb, false, ux0, uy0, uz0);
+            temp = rare_temperature(P.self_dev, t_in, C.flags, C.src, cell, x, y, z, ux0, uy0, uz0);
         else
-            temp = temperature_fast(P, t_in, C.src, cell, x, y, z, nb, links, ux0, uy0, uz0);
+            temp = temperature_fast(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0);
         t_out[cell] = temp;
     }
     // ---- collision
     CollideResult r = collide<St::kShifted>(P, g, temp);
-    if (!r.ok) reseed<St::kShifted>(P, C, cell, g, r);
-    if (P.sponge_on) sponge<St::kShifted>(P, x, y, z, g);
+    if (!r.ok || P.sponge_on) {
+        float gt[27];
+        copy27(gt, g);
+        rare_after_collide<St::kShifted>(P.self_dev, C, cell, x, y, z, gt, &r);
+        copy27(g, gt);
+    }
     if (P.thermal_on) {
         Uv::st(u_state + cell, r.ux);
         Uv::st(u_state + N + cell, r.uy);
@@ -209,43 +313,36 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     }
     // ---- face store of this step's post-collision values (A-A only)
     if (Mode != 0 && P.face_store_faces) {
-#pragma unroll
-        for (int face = 0; face < 6; ++face) {
-            if (!((P.face_store_faces >> face) & 1)) continue;
-            const int axis = face >> 1;
-            const int coord = axis == 0 ? x : axis == 1 ? y : z;
-            const int n = axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz;
-            const int layer = (face & 1) ? n - 1 - coord : coord;
-            if (layer > 2) continue;
-            const int A = axis == 0 ? P.ny : P.nx;
-            const int B = axis == 2 ? P.ny : P.nz;
-            const int a = axis == 0 ? y : x;
-            const int bb = axis == 2 ? y : z;
-            const int cdir = (face & 1) ? 0 : 2;  // incoming through the min face: c = +1 (index 2)
-            const long long off = P.face_store_off[face];
-#pragma unroll
-            for (int j = 0; j < 9; ++j) {
-                const int q = axis == 0 ? cdir * 9 + j : axis == 1 ? (j / 3) * 9 + cdir * 3 + (j % 3) : j * 3 + cdir;
-                store_out[off + ((size_t)(layer * 9 + j) * A + a) * B + bb] = g[q];
-            }
+        const int fs = P.face_store_faces;
+        const bool near = ((fs & 1) && x < 3) || ((fs & 2) && x >= P.nx - 3) || ((fs & 4) && y < 3) ||
+                          ((fs & 8) && y >= P.ny - 3) || ((fs & 16) && z < 3) || ((fs & 32) && z >= P.nz - 3);
+        if (near) {
+            float gt[27];
+            copy27(gt, g);
+            rare_face_store(P.self_dev, x, y, z, gt, store_out);
         }
     }
     // ---- write back
-    if (Mode == 0) {
+    const RowNbr R2 = row_nbr(P, fresh_s(x), fresh_s(y));
+    if (Mode != 2) {
 #pragma unroll
-        for (int q = 0; q < 27; ++q) St::st(plane<St>(fout, N, q) + cell, g[q]);
-    } else if (Mode == 1) {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) St::st(plane<St>(fout, N, lat_opp(q)) + cell, g[q]);
-    } else if (wave_bulk) {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) St::st(plane<St>(fout, N, q) + cell + nb.off(lat_cx(q), lat_cy(q), lat_cz(q)), g[q]);
+        for (int q = 0; q < 27; ++q)
+            St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, Mode == 0 ? q : lat_opp(q), R2.row), g[q]);
     } else {
+        // push to x + c_q; a target outside the domain is dropped, a solid target
+        // receives a value no fluid cell reads
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
-            const bool sp = (links >> lat_opp(q)) & 1u;  // target x + c_q solid or out of the domain
-            const int dst = sp ? cell : cell + nb.off(lat_cx(q), lat_cy(q), lat_cz(q));
-            St::st(fout + (size_t)(sp ? lat_opp(q) : q) * N + dst, g[q]);
+            const int dx = lat_cx(q), dy = lat_cy(q), dz = lat_cz(q);
+            if (R2.ok(dx, dy)) St::template bst<StPol>(rout, zb[dz + 1], plane_row<T>(Ni, q, R2.off(dx, dy)), g[q]);
+        }
+        // target solid or outside: bounce back into the cell's own slot opp(q)
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const bool sp = (links >> lat_opp(q)) & 1u;
+            if (__builtin_amdgcn_ballot_w64(sp)) {
+                if (sp) St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, lat_opp(q), R2.row), g[q]);
+            }
         }
     }
 }
@@ -261,7 +358,7 @@ __device__ __forceinline__ void cell_output_fast(const StepParams& P, const Cell
     const int cell = (x * P.ny + y) * P.nz + z;
     const uint32_t m = mask[cell];
     if (m & kMaskSolid) return;
-    const size_t N = (size_t)P.cells;
+    const size_t Np = (size_t)P.plane;
     const uint32_t links = m & kMaskLinks;
     const Nbr nb = make_nbr(P, x, y, z);
     float g[27];
@@ -270,17 +367,17 @@ __device__ __forceinline__ void cell_output_fast(const StepParams& P, const Cell
         const bool sp = (links >> q) & 1u;
         if (layout == 0) {
             const int src = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
-            g[q] = St::ld(f + (size_t)(sp ? lat_opp(q) : q) * N + src);
+            g[q] = St::ld(f + (size_t)(sp ? lat_opp(q) : q) * Np + src);
         } else if (layout == 1) {
             const int src = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
-            g[q] = St::ld(f + (size_t)(sp ? q : lat_opp(q)) * N + src);
+            g[q] = St::ld(f + (size_t)(sp ? q : lat_opp(q)) * Np + src);
         } else {
-            g[q] = St::ld(f + (size_t)q * N + cell);
+            g[q] = St::ld(f + (size_t)q * Np + cell);
         }
     }
     if ((m & kMaskFace) && layout != 3) {
-        const PrevPost<St, true> prev_aa{f, store, N};
-        const PrevPost<St, false> prev_ab{f, store, N};
+        const PrevPost<St, true> prev_aa{f, store, Np};
+        const PrevPost<St, false> prev_ab{f, store, Np};
         for (int q = 0; q < 27; ++q) {
             if (((links >> q) & 1u) && link_oob(P, x, y, z, q)) {
                 float bnc = 0.0f;
@@ -361,7 +458,7 @@ __global__ void __launch_bounds__(256) init_kernel(StepParams P, CellFields C, t
         uz = C.ref_u[2 * N + cell];
     }
 #pragma unroll
-    for (int q = 0; q < 27; ++q) St::st(f + (size_t)q * N + cell, feq_s<St::kShifted>(q, rho, ux, uy, uz));
+    for (int q = 0; q < 27; ++q) St::st(f + (size_t)q * P.plane + cell, feq_s<St::kShifted>(q, rho, ux, uy, uz));
     float t = T[cell];
     if (fabsf(t) < 1e-8f) t = clampf(t_init, -1.0f, 1.0f);
     else if (!solid) t = clampf(C.ref_t[cell], -1.0f, 1.0f);
@@ -412,6 +509,14 @@ __global__ void __launch_bounds__(256) mask_kernel(StepParams P, const uint8_t* 
 }  // namespace fast
 
 // ---------------------------------------------------------------- launchers
+int fast_plane_stride(int cells) {
+    static const int pad = [] {
+        const char* e = std::getenv("AERO_LBM_PLANE_PAD");
+        return e ? std::max(0, std::atoi(e)) : 0;
+    }();
+    return cells + pad;
+}
+
 static inline dim3 g1(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 void launch_fast_mask(const StepParams& P, const uint8_t* flags, uint32_t* mask, hipStream_t s) {
@@ -437,22 +542,55 @@ void launch_fast_init(const StepParams& P, const CellFields& C, FastStorage st, 
     }
 }
 
-template <class St, class Uv>
-static void launch_step_t(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
+// Store policy: a brick whose working set fits the 256 MB last-level cache
+// (Infinity Cache) keeps the default policy, so populations stay resident from
+// one step to the next; a larger one streams its stores past the caches.
+// AERO_LBM_CACHE_POLICY=default|stream overrides the choice.
+constexpr size_t kResidentBytes = 192ull << 20;
+
+static int store_policy(size_t working_set) {
+    static const int forced = [] {
+        const char* e = std::getenv("AERO_LBM_CACHE_POLICY");
+        if (e && std::strcmp(e, "stream") == 0) return fast::kPolStream;
+        if (e && std::strcmp(e, "default") == 0) return fast::kPolDefault;
+        return -1;
+    }();
+    if (forced >= 0) return forced;
+    return working_set > kResidentBytes ? fast::kPolStream : fast::kPolDefault;
+}
+
+template <class St, class Uv, int StPol>
+static void launch_step_p(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
                           const float* t_in, float* t_out, void* u_state, const float* store_in, float* store_out,
                           int mode, int local_first, hipStream_t s) {
     using T = typename St::T;
     using U = typename Uv::T;
     const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
     if (mode == 0)
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 0>), grid, block, 0, s, P, C, mask, (const T*)fin, (T*)fout,
-                           t_in, t_out, (U*)u_state, store_in, store_out, local_first);
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 0, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
+                           (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, local_first);
     else if (mode == 1)
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 1>), grid, block, 0, s, P, C, mask, (const T*)fin, (T*)fout,
-                           t_in, t_out, (U*)u_state, store_in, store_out, local_first);
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 1, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
+                           (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, local_first);
     else
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 2>), grid, block, 0, s, P, C, mask, (const T*)fin, (T*)fout,
-                           t_in, t_out, (U*)u_state, store_in, store_out, 0);
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 2, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
+                           (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, 0);
+}
+
+template <class St, class Uv>
+static void launch_step_t(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
+                          const float* t_in, float* t_out, void* u_state, const float* store_in, float* store_out,
+                          int mode, int local_first, hipStream_t s) {
+    const size_t N = (size_t)P.cells;
+    // populations (two arrays for A-B), temperature pair, carried velocity, link mask, flags
+    const size_t ws = (mode == 0 ? 2 : 1) * 27 * sizeof(typename St::T) * (size_t)P.plane +
+                      N * (8 + 3 * sizeof(typename Uv::T) + 5);
+    if (store_policy(ws) == fast::kPolStream)
+        launch_step_p<St, Uv, fast::kPolStream>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out,
+                                                mode, local_first, s);
+    else
+        launch_step_p<St, Uv, fast::kPolDefault>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out,
+                                                 mode, local_first, s);
 }
 
 void launch_fast_step(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* fin,

@@ -9,12 +9,39 @@
 namespace aero {
 namespace fast {
 
+// ---------------------------------------------------------------- buffer resources
+// The step kernel reaches populations through one buffer resource over the
+// whole population array. The byte offset of an access is split into a
+// wave-uniform scalar part (population plane + neighbour row) and a per-lane
+// vector part (z inside the row), and the hardware drops an access whose vector
+// offset is beyond the resource's range (a load returns 0, a store writes
+// nothing). The array must stay below 2 GiB so that kDropByte is out of range.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+constexpr int kDropByte = INT32_MIN;    // vector byte offset outside every resource
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, bytes, kRsrcWord3);
+}
+
+// cache policy bits of a buffer access: 0 = default, 2 = non-temporal (streaming)
+constexpr int kPolDefault = 0;
+constexpr int kPolStream = 2;
+
 // ---------------------------------------------------------------- storage traits
 struct StF32 {
     using T = float;
     static constexpr bool kShifted = false;
     static __device__ __forceinline__ float ld(const T* p) { return __builtin_nontemporal_load(p); }
     static __device__ __forceinline__ void st(T* p, float v) { __builtin_nontemporal_store(v, p); }
+    template <int Pol>
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, Pol));
+    }
+    template <int Pol>
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, soff, Pol);
+    }
 };
 struct StF16 {
     using T = uint16_t;
@@ -26,18 +53,37 @@ struct StF16 {
     static __device__ __forceinline__ void st(T* p, float v) {
         __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, (_Float16)v), p);
     }
+    template <int Pol>
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, Pol);
+        return (float)__builtin_bit_cast(_Float16, b);
+    }
+    template <int Pol>
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), r, off, soff, Pol);
+    }
 };
 struct StBF16 {
     using T = uint16_t;
     static constexpr bool kShifted = true;
+    static __device__ __forceinline__ uint16_t pack(float v) {
+        uint32_t u = __float_as_uint(v);
+        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+        return (uint16_t)(u >> 16);
+    }
     static __device__ __forceinline__ float ld(const T* p) {
         const uint32_t b = __builtin_nontemporal_load(p);
         return __uint_as_float(b << 16);
     }
-    static __device__ __forceinline__ void st(T* p, float v) {
-        uint32_t u = __float_as_uint(v);
-        u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-        __builtin_nontemporal_store((uint16_t)(u >> 16), p);
+    static __device__ __forceinline__ void st(T* p, float v) { __builtin_nontemporal_store(pack(v), p); }
+    template <int Pol>
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+        const uint32_t b = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, Pol);
+        return __uint_as_float(b << 16);
+    }
+    template <int Pol>
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+        __builtin_amdgcn_raw_buffer_store_b16(pack(v), r, off, soff, Pol);
     }
 };
 

@@ -14,6 +14,15 @@ constexpr uint32_t kMaskSolid = 1u << 27;  // the cell itself is solid
 constexpr uint32_t kMaskFace = 1u << 28;   // a link leaves through a face whose rule is not plain bounce-back
 constexpr uint32_t kMaskTherm = 1u << 29;  // a stencil neighbour is outside the domain at a Dirichlet/Neumann thermal face
 
+// largest population array the fast step kernel addresses (one buffer resource,
+// per-lane byte offsets below 2^31)
+constexpr long long kFastMaxBufferBytes = (1ll << 31) - 1;
+
+// plane stride (cells) of the fast kernels' population arrays for a brick of
+// `cells` cells: padded so that the 27 planes do not start at addresses that are
+// equal modulo a large power of two
+int fast_plane_stride(int cells);
+
 void launch_fast_mask(const StepParams& P, const uint8_t* flags, uint32_t* mask, hipStream_t s);
 void launch_fast_init(const StepParams& P, const CellFields& C, FastStorage st, void* f, float* T, float* T_other, void* u_state,
                       float t_init, hipStream_t s);

@@ -42,6 +42,7 @@ struct FaceParams {
 struct StepParams {
     int nx, ny, nz;
     int cells;
+    int plane;  // population plane stride of the fast kernels, in cells (cells + padding)
     FaceParams face[6];
     int hydro_periodic[3];    // axis wraps (both faces periodic on that side handled per side below)
     int hydro_wrap_lo[3];     // face kind of the min side is PERIODIC
@@ -70,6 +71,9 @@ struct StepParams {
     // thermal periodic flags equal the hydrodynamic ones, so a cell whose 26
     // neighbours are fluid and in the domain can read its temperature stencil directly
     int thermal_wrap_matches;
+    // device copy of this whole struct: rarely taken kernel paths read their
+    // parameters from it, so the bulk path keeps no scalar registers for them
+    const StepParams* self_dev;
 };
 
 }  // namespace aero

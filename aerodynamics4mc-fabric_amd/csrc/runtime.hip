@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/aero_lbm_capi.h"
+#include "fast_kernels.h"
 #include "kernels.h"
 #include "runtime_internal.h"
 
@@ -246,6 +247,7 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
     P.ny = ny;
     P.nz = nz;
     P.cells = nx * ny * nz;
+    P.plane = fast_plane_stride(P.cells);
     P.in_ch = in_ch;
     const AeroLbmBoundaryFaceConfig* faces[6] = {&g.x_min, &g.x_max, &g.y_min, &g.y_max, &g.z_min, &g.z_max};
     for (int i = 0; i < 6; ++i) {
@@ -350,12 +352,21 @@ bool prepare_params(Context& c, StepParams& P) {
             std::memcpy(c.class_wrap, w, sizeof(w));
         }
     }
+    P.self_dev = c.params_dev;
+    if (!c.params_valid || std::memcmp(&c.params_host, &P, sizeof(P)) != 0) {
+        if (!hip_ok(hipStreamSynchronize(rt().stream), "sync")) return false;
+        std::memcpy(&c.params_host, &P, sizeof(P));
+        if (!hip_ok(hipMemcpyAsync(c.params_dev, &c.params_host, sizeof(P), hipMemcpyHostToDevice, rt().stream),
+                    "upload step parameters"))
+            return false;
+        c.params_valid = true;
+    }
     return true;
 }
 
 // ---- contexts ----
 static void free_context_buffers(Context& c) {
-    for (void* p : c.allocations) hipFree(p);
+    for (void* p : c.allocations) (void)hipFree(p);
     c.allocations.clear();
 }
 
@@ -395,6 +406,7 @@ static bool allocate_context(Context& c, int nx, int ny, int nz) {
     ok &= c.alloc((void**)&c.out_dev, N * 4 * 4);
     ok &= c.alloc((void**)&c.sample_dev, 64);
     ok &= c.alloc((void**)&c.faces_dev, sizeof(FaceParams) * 6);
+    ok &= c.alloc((void**)&c.params_dev, sizeof(StepParams));
     if (!ok) return false;
     if (!storage_alloc(c)) return false;
     hipStream_t st = rt().stream;
@@ -853,7 +865,7 @@ AERO_LBM_CAPI_EXPORT const char* aero_lbm_memory_info(void) {
     size_t bytes = 0;
     for (auto& kv : rt().contexts) bytes += context_device_bytes(*kv.second);
     size_t free_b = 0, total_b = 0;
-    if (rt().device_ready) hipMemGetInfo(&free_b, &total_b);
+    if (rt().device_ready) (void)hipMemGetInfo(&free_b, &total_b);
     std::ostringstream o;
     o << "contexts=" << rt().contexts.size() << " context_device_bytes=" << bytes << " device_free=" << free_b
       << " device_total=" << total_b;

@@ -51,6 +51,9 @@ struct Context {
     FaceParams* faces_dev = nullptr;   // device copy of the face table
     FaceParams faces_host[6] = {};     // last uploaded face table (source of the async copy)
     bool faces_valid = false;
+    StepParams* params_dev = nullptr;  // device copy of the step parameters
+    StepParams params_host = {};       // last uploaded parameters (source of the async copy)
+    bool params_valid = false;
     int class_wrap[6] = {-1, -1, -1, -1, -1, -1};  // periodic flags the cell classes were built with
     std::vector<float> payload_host;
     std::vector<void*> allocations;

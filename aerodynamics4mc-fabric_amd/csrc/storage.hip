@@ -13,17 +13,27 @@ static FastStorage fast_kind(Storage s) {
 
 bool storage_alloc(Context& c) {
     const size_t N = c.cells;
+    const size_t Np = (size_t)fast_plane_stride((int)c.cells);  // fast kernels' plane stride
+    if (c.storage != Storage::F32Exact) {
+        // the fast kernels address each population array through one buffer resource
+        const size_t bytes = 27 * Np * (c.storage == Storage::F16 || c.storage == Storage::BF16 ? 2 : 4);
+        if (bytes > (size_t)kFastMaxBufferBytes) {
+            set_error("brick too large for the fast kernels (population array above 2 GiB); "
+                      "use the exact kernel variant or smaller bricks");
+            return false;
+        }
+    }
     switch (c.storage) {
         case Storage::F32Exact:
             return c.alloc((void**)&c.f32[0], 27 * N * 4) && c.alloc((void**)&c.f32[1], 27 * N * 4);
         case Storage::F32Fast:
-            return c.alloc((void**)&c.f32[0], 27 * N * 4) && c.alloc((void**)&c.f32[1], 27 * N * 4) &&
+            return c.alloc((void**)&c.f32[0], 27 * Np * 4) && c.alloc((void**)&c.f32[1], 27 * Np * 4) &&
                    c.alloc((void**)&c.mask, N * 4);
         case Storage::F16:
         case Storage::BF16:
-            return c.alloc((void**)&c.h16, 27 * N * 2) && c.alloc((void**)&c.mask, N * 4);
+            return c.alloc((void**)&c.h16, 27 * Np * 2) && c.alloc((void**)&c.mask, N * 4);
         case Storage::F32AA:
-            return c.alloc((void**)&c.f32[0], 27 * N * 4) && c.alloc((void**)&c.mask, N * 4);
+            return c.alloc((void**)&c.f32[0], 27 * Np * 4) && c.alloc((void**)&c.mask, N * 4);
     }
     return false;
 }
@@ -33,10 +43,15 @@ static void* aa_buffer(Context& c) { return c.storage == Storage::F32AA ? (void*
 
 size_t context_device_bytes(const Context& c) {
     const size_t N = c.cells;
+    const size_t Np = (size_t)fast_plane_stride((int)c.cells);
     size_t b = N * (1 + 4 * 4 + 4 * 2 + 12 + 16) + 64;
-    if (c.f32[0]) b += 2 * 27 * N * 4;
-    if (c.h16) b += 27 * N * 2;
-    if (c.mask) b += N * 4;
+    switch (c.storage) {
+        case Storage::F32Exact: b += 2 * 27 * N * 4; break;
+        case Storage::F32Fast: b += 2 * 27 * Np * 4 + N * 4; break;
+        case Storage::F16:
+        case Storage::BF16: b += 27 * Np * 2 + N * 4; break;
+        case Storage::F32AA: b += 27 * Np * 4 + N * 4; break;
+    }
     b += 2 * c.face_store_floats * 4;
     if (c.payload_dev) b += c.payload_host.size() * 4;
     return b;
