@@ -509,10 +509,12 @@ __global__ void __launch_bounds__(256) mask_kernel(StepParams P, const uint8_t* 
 }  // namespace fast
 
 // ---------------------------------------------------------------- launchers
+// 64 cells of padding between planes (measured: +4% on the 128^3 FP32 A-B step,
+// neutral for FP16); AERO_LBM_PLANE_PAD overrides it
 int fast_plane_stride(int cells) {
     static const int pad = [] {
         const char* e = std::getenv("AERO_LBM_PLANE_PAD");
-        return e ? std::max(0, std::atoi(e)) : 0;
+        return e ? std::max(0, std::atoi(e)) : 64;
     }();
     return cells + pad;
 }

@@ -39,23 +39,27 @@ MODES = {
 }
 
 
-def algorithmic_bytes(mode: str, solid: np.ndarray, thermal: bool) -> dict:
-    """Minimal HBM bytes of one step launch: every fluid population read once and
-    written once, one flag byte per cell, and with the thermal scalar on the
-    temperature (read + write) and the cell velocity carried to the next step's
-    advection (read + write). Solid cells move no populations except in the
-    exact path, which swaps them like the reference."""
-    cells = solid.size
-    fluid = int((solid == 0).sum())
-    pop = MODES[mode][1]
-    per_fluid = 27 * pop * 2
-    if thermal:
-        u_bytes = 6 if pop == 2 else 12  # velocity carried as 3 x f16 or 3 x f32
-        per_fluid += 4 * 2 + u_bytes * 2
-    total = fluid * per_fluid + cells * 1
-    if mode == "fp32_exact":
-        total += (cells - fluid) * 27 * 4 * 2
-    return {"bytes": total, "per_lup": total / cells, "fluid_fraction": fluid / cells}
+# Algorithmic bytes per lattice update: every population read once and written
+# once, one flag byte per cell, the temperature scalar read and written once.
+B_LUP = {"fp16": 27 * 2 * 2 + 1 + 8, "bf16": 27 * 2 * 2 + 1 + 8, "fp32": 27 * 4 * 2 + 1 + 8,
+         "fp32_exact": 27 * 4 * 2 + 1 + 8}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "d3q27_step_128_traffic.json")
+
+
+def algorithmic_bytes(mode: str, cells: int) -> dict:
+    return {"bytes": cells * B_LUP[mode], "per_lup": B_LUP[mode]}
+
+
+def measured_traffic(mode: str):
+    """HBM bytes per step launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (scaled by the copy-kernel calibration recorded in the same file), or
+    None when the summary does not cover this storage mode."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+        return t["modes"][mode]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def set_mode(lib, A, W, mode: str, ov):
@@ -92,7 +96,7 @@ def run_mode(lib, A, W, mode, n, steps, warmup, inputs):
             "finite": finite, "max_speed": float(np.abs(flow[:, :3]).max())}
 
 
-def cpu_baseline(n, inputs, sample_steps=2):
+def cpu_baseline(n, inputs, sample_steps=8):
     """The FP32 oracle (plain-C restatement of the reference's single-threaded CPU
     step) on this host, one core, on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -163,8 +167,9 @@ def main():
               file=sys.stderr, flush=True)
     head = modes[0]
     r = results[head]
-    ab = algorithmic_bytes(head, solid, thermal=True)
+    ab = algorithmic_bytes(head, n ** 3)
     achieved = ab["bytes"] / (r["kernel_ms"] / 1e3)
+    solid_fraction = float(solid.mean())
     line = {
         "metric": METRIC,
         "value": round(r["glups_wall"], 4),
@@ -179,11 +184,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"d3q27_cumulant_sgs_boussinesq_{n}^3_{head}", "cells": n ** 3,
-                   "storage": head, "solid_fraction": round(1 - ab["fluid_fraction"], 4),
+                   "storage": head, "solid_fraction": round(solid_fraction, 4),
                    "sgs_c": 0.1, "beta": 0.1, "gravity": "-z", "inflow": 0.05},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "bytes_per_launch": ab["bytes"], "bytes_per_lup": round(ab["per_lup"], 2),
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(head),
+                     "bytes_per_launch": ab["bytes"], "bytes_per_lup": ab["per_lup"],
                      "kernel_ms": r["kernel_ms"]},
         "modes": {m: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in res.items()}
                   for m, res in results.items()},

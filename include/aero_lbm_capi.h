@@ -264,7 +264,8 @@ AERO_LBM_CAPI_EXPORT void aero_lbm_benchmark_reset_config(void);
 AERO_LBM_CAPI_EXPORT const char* aero_lbm_benchmark_info(void);
 
 /* ---- extension: overrides (no counterpart in the reference; for callers that
- * need the BASELINE-style parametrisation: Cs, beta, gravity axis, exact vs fast) ---- */
+ * re-parametrise the SGS constant, beta or the gravity axis, or pick the exact
+ * or the fast kernels) ---- */
 AERO_LBM_CAPI_EXPORT void aero_lbm_default_solver_overrides(AeroLbmSolverOverrides* out_overrides);
 AERO_LBM_CAPI_EXPORT int aero_lbm_set_solver_overrides(const AeroLbmSolverOverrides* overrides);
 AERO_LBM_CAPI_EXPORT int aero_lbm_get_solver_overrides(AeroLbmSolverOverrides* out_overrides);

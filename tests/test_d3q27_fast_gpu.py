@@ -4,8 +4,8 @@
   flip, bounce-back slots and face store must reproduce the A-B result bit for
   bit after every step count.
 * FP32 fast / FP16 / BF16 against the FP64 oracle: velocity error relative to
-  the inflow speed. Bar: FP16 shifted storage <= 1e-3 of
-  the inflow speed; FP32 <= 1e-4 of the inflow speed.
+  the inflow speed. Bar: FP16 shifted storage <= 1e-3 of the inflow speed;
+  FP32 <= 1e-4 of the inflow speed.
 """
 import ctypes
 import json
