@@ -382,3 +382,24 @@ ORACLE_API void MS(meso_get_f)(void* h, R* f, R* g) {
     memcpy(f, c->f, (size_t)c->cells * 19 * sizeof(R));
     memcpy(g, c->g, (size_t)c->cells * 19 * sizeof(R));
 }
+
+/* test hooks: overwrite the distributions (marks the context initialised) and
+ * read the macroscopic fields and the derived transport */
+ORACLE_API void MS(meso_set_f)(void* h, const R* f, const R* g) {
+    MS(Ctx)* c = (MS(Ctx)*)h;
+    memcpy(c->f, f, (size_t)c->cells * 19 * sizeof(R));
+    memcpy(c->g, g, (size_t)c->cells * 19 * sizeof(R));
+    c->initialized = 1;
+}
+
+ORACLE_API void MS(meso_get_fields)(void* h, R* ambient, R* surface, R* deep) {
+    MS(Ctx)* c = (MS(Ctx)*)h;
+    memcpy(ambient, c->ambient, (size_t)c->cells * sizeof(R));
+    memcpy(surface, c->surface, (size_t)c->cells * sizeof(R));
+    memcpy(deep, c->deep, (size_t)c->cells * sizeof(R));
+}
+
+ORACLE_API void MS(meso_transport)(void* h, R* out3) {
+    MS(Ctx)* c = (MS(Ctx)*)h;
+    MS(transport)(c, &out3[0], &out3[1], &out3[2]);
+}

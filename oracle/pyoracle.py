@@ -188,6 +188,9 @@ class MesoOracle:
         getattr(self.lib, self.p + "meso_step").argtypes = [c_void_p, POINTER(c_float), POINTER(c_float)]
         getattr(self.lib, self.p + "meso_step").restype = c_int
         getattr(self.lib, self.p + "meso_get_f").argtypes = [c_void_p, POINTER(ct), POINTER(ct)]
+        getattr(self.lib, self.p + "meso_set_f").argtypes = [c_void_p, POINTER(ct), POINTER(ct)]
+        getattr(self.lib, self.p + "meso_get_fields").argtypes = [c_void_p] + [POINTER(ct)] * 3
+        getattr(self.lib, self.p + "meso_transport").argtypes = [c_void_p, POINTER(ct)]
         self.cells = nx * ny * nz
         self.h = f(nx, ny, nz, dx, dt, nu, pr, prt, tau_min)
         if not self.h:
@@ -211,3 +214,23 @@ class MesoOracle:
         g = np.zeros(self.cells * 19, dtype=self.real)
         getattr(self.lib, self.p + "meso_get_f")(self.h, f.ctypes.data_as(POINTER(ct)), g.ctypes.data_as(POINTER(ct)))
         return f.reshape(self.cells, 19), g.reshape(self.cells, 19)
+
+    def set_distributions(self, f, g):
+        ct = np.ctypeslib.as_ctypes_type(self.real)
+        f = np.ascontiguousarray(f, dtype=self.real).reshape(-1)
+        g = np.ascontiguousarray(g, dtype=self.real).reshape(-1)
+        getattr(self.lib, self.p + "meso_set_f")(self.h, f.ctypes.data_as(POINTER(ct)), g.ctypes.data_as(POINTER(ct)))
+
+    def fields(self):
+        """(ambient, surface, deep) per cell."""
+        ct = np.ctypeslib.as_ctypes_type(self.real)
+        a, s, d = (np.zeros(self.cells, dtype=self.real) for _ in range(3))
+        getattr(self.lib, self.p + "meso_get_fields")(self.h, *(v.ctypes.data_as(POINTER(ct)) for v in (a, s, d)))
+        return a, s, d
+
+    def transport(self):
+        """(velocity scale m/s per lattice unit, tau_shear, tau_thermal) before clamping."""
+        ct = np.ctypeslib.as_ctypes_type(self.real)
+        out = np.zeros(3, dtype=self.real)
+        getattr(self.lib, self.p + "meso_transport")(self.h, out.ctypes.data_as(POINTER(ct)))
+        return tuple(out.tolist())
