@@ -156,6 +156,15 @@ PROTOTYPES = {
     "aero_lbm_set_solver_overrides": (c_int, [POINTER(SolverOverrides)]),
     "aero_lbm_get_solver_overrides": (c_int, [POINTER(SolverOverrides)]),
     "aero_lbm_run_cached_steps_timed": (c_int, [c_int, c_int, c_int, c_longlong, c_int, _F]),
+    "aero_lbm_get_conservation_rect": (c_int, [c_int, c_int, c_int, c_longlong, POINTER(c_double)]),
+    "aero_lbm_mesoscale_default_config": (None, [POINTER(MesoscaleConfig)]),
+    "aero_lbm_mesoscale_derive_transport": (c_int, [POINTER(MesoscaleConfig), POINTER(MesoscaleTransport)]),
+    "aero_lbm_mesoscale_create_context": (c_int, [POINTER(MesoscaleConfig), POINTER(c_longlong)]),
+    "aero_lbm_mesoscale_step_context": (c_int, [c_longlong, POINTER(MesoscaleConfig), _F, _F]),
+    "aero_lbm_mesoscale_release_context": (None, [c_longlong]),
+    "aero_lbm_mesoscale_set_tau_floor": (c_int, [c_float]),
+    "aero_lbm_mesoscale_run_steps_timed": (c_int, [c_longlong, POINTER(MesoscaleConfig), _F, c_int, _F]),
+    "aero_lbm_mesoscale_get_state": (c_int, [c_longlong, _F]),
 }
 
 
@@ -279,3 +288,41 @@ class Context:
 
     def release(self) -> None:
         self.lib.aero_lbm_release_context(self.key)
+
+
+class Mesoscale:
+    """One D3Q19 mesoscale context of the library (aero_lbm_mesoscale_*)."""
+
+    def __init__(self, cfg: MesoscaleConfig):
+        self.lib = load()
+        self.cfg = cfg
+        key = c_longlong(0)
+        check(self.lib.aero_lbm_mesoscale_create_context(ctypes.byref(cfg), ctypes.byref(key)), "mesoscale create")
+        self.key = key.value
+        self.cells = cfg.nx * cfg.ny * cfg.nz
+
+    def step(self, forcing):
+        forcing = np.ascontiguousarray(forcing, dtype=np.float32).reshape(-1)
+        if forcing.size != self.cells * 25:
+            raise ValueError("forcing must hold cells x 25 floats")
+        out = np.zeros(self.cells * 6, dtype=np.float32)
+        check(self.lib.aero_lbm_mesoscale_step_context(self.key, ctypes.byref(self.cfg), forcing.ctypes.data_as(_F),
+                                                        out.ctypes.data_as(_F)), "mesoscale step")
+        return out
+
+    def run_timed(self, forcing, steps):
+        forcing = np.ascontiguousarray(forcing, dtype=np.float32).reshape(-1)
+        ms = c_float(0.0)
+        check(self.lib.aero_lbm_mesoscale_run_steps_timed(self.key, ctypes.byref(self.cfg), forcing.ctypes.data_as(_F),
+                                                           steps, ctypes.byref(ms)), "mesoscale timed run")
+        return ms.value
+
+    def state(self):
+        out = np.zeros(self.cells * 6, dtype=np.float32)
+        check(self.lib.aero_lbm_mesoscale_get_state(self.key, out.ctypes.data_as(_F)), "mesoscale state")
+        return out
+
+    def release(self):
+        if self.key:
+            self.lib.aero_lbm_mesoscale_release_context(self.key)
+            self.key = 0

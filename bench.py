@@ -123,6 +123,53 @@ def cpu_baseline(n, inputs, sample_steps=8):
                       f"(oracle/liboracle.so), single thread on {cpu}; {dt:.1f} s"}
 
 
+MESO_DIMS = (256, 64, 256)  # x, y (vertical), z
+MESO_B_LUP = 2 * 19 * 4 * 2 + 1  # f and g read and written once, one flag byte
+
+
+def run_meso(steps, warmup, no_cpu):
+    """D3Q19 mesoscale step on 256 x 64 x 256 terrain (4-octave value noise, heights
+    0-24 cells, geostrophic wind 0.04 lattice units at 30 degrees from +x with a log-law
+    vertical profile, lattice viscosity 5e-4 clamped at the reference's tau floor 0.58)."""
+    import aero_lbm as A
+    import workloads as W
+    cfg = W.meso_config(*MESO_DIMS)
+    F = W.meso_forcing(*MESO_DIMS)
+    cells = int(np.prod(MESO_DIMS))
+    ctx = A.Mesoscale(cfg)
+    ctx.step(F)  # upload, coefficient planes, seeding (untimed)
+    if warmup > 0:
+        ctx.run_timed(F, warmup)
+    t0 = time.perf_counter()
+    device_ms = ctx.run_timed(F, steps)
+    wall = time.perf_counter() - t0
+    state = ctx.state().reshape(-1, 6)
+    ctx.release()
+    ms = device_ms / steps
+    out = {"workload": "d3q19_mesoscale_256x64x256_fp32", "cells": cells, "steps": steps,
+           "glups": cells * steps / wall / 1e9, "ms_per_step": wall * 1e3 / steps, "kernel_ms_per_step": ms,
+           "finite": bool(np.isfinite(state).all()),
+           "roofline": {"bound": "hbm", "achieved": cells * MESO_B_LUP / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": cells * MESO_B_LUP / (ms * 1e-3) / HBM_PEAK, "traffic": None,
+                        "bytes_per_lup": MESO_B_LUP}}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        pyoracle.set_threads(1)
+        o = pyoracle.MesoOracle(*MESO_DIMS, dx=cfg.dx_m, dt=cfg.dt_s, nu=cfg.molecular_nu_m2_s, pr=cfg.prandtl_air,
+                                prt=cfg.turbulent_prandtl, precision=32)
+        o.step(F)
+        n = 3
+        t0 = time.perf_counter()
+        for _ in range(n):
+            o.step(F)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cells * n / dt / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+                               "sample": f"{n} steps of the same 256x64x256 workload after 1 seeding step, FP32 "
+                                         f"mesoscale oracle, single thread; {dt:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +178,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--modes", default="fp16,fp32")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-meso", action="store_true", help="skip the D3Q19 mesoscale line")
+    ap.add_argument("--meso-steps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) five times, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -198,6 +247,10 @@ def main():
     if not args.no_cpu:
         print("bench: cpu baseline (oracle, 1 core)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n, inputs)
+    if not args.no_meso:
+        print(f"bench: mesoscale d3q19 {MESO_DIMS} steps={args.meso_steps}", file=sys.stderr, flush=True)
+        line["mesoscale"] = run_meso(args.meso_steps, min(args.warmup, 20), args.no_cpu)
+        print(f"bench: mesoscale {line['mesoscale']['glups']:.2f} GLUPS", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

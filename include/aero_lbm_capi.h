@@ -255,6 +255,39 @@ AERO_LBM_CAPI_EXPORT void aero_lbm_set_realtime_solver_mode(int solver_mode);
 AERO_LBM_CAPI_EXPORT int aero_lbm_finish(void);
 AERO_LBM_CAPI_EXPORT int aero_lbm_get_timing_snapshot(AeroLbmTimingSnapshot* out_snapshot);
 
+/* ---- extension: conservation diagnostics (no counterpart in the reference).
+ * Total mass and momentum {m, jx, jy, jz} of the fluid cells' populations as
+ * stored (lattice units), summed in FP64 on the device in a fixed order. ---- */
+AERO_LBM_CAPI_EXPORT int aero_lbm_get_conservation_rect(int nx, int ny, int nz, long long context_key, double* out4);
+
+/* ---- mesoscale D3Q19 step (ref: native/include/aero_lbm_capi.h:286-301) ----
+ * forcing: cells x 25 floats, state: cells x 6 floats (ambient, deep ground,
+ * surface, wind x, y, z), cell = (x*ny + y)*nz + z, y vertical. */
+AERO_LBM_CAPI_EXPORT void aero_lbm_mesoscale_default_config(AeroLbmMesoscaleConfig* out_config);
+AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_derive_transport(
+    const AeroLbmMesoscaleConfig* config,
+    AeroLbmMesoscaleTransport* out_transport);
+AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_create_context(
+    const AeroLbmMesoscaleConfig* config,
+    long long* out_context_key);
+AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_step_context(
+    long long context_key,
+    const AeroLbmMesoscaleConfig* config,
+    const float* forcing,
+    float* out_state);
+AERO_LBM_CAPI_EXPORT void aero_lbm_mesoscale_release_context(long long context_key);
+
+/* ---- extension: mesoscale (no counterpart in the reference) ----
+ * set_tau_floor: lower clamp of both relaxation times (the reference fixes 0.58;
+ *   NaN restores it), for low-viscosity runs such as tau = 0.5015;
+ * run_steps_timed: `steps` steps with the same forcing, no per-step readback,
+ *   device time in *out_ms;
+ * get_state: the 6-channel state after the last step. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_set_tau_floor(float tau_floor);
+AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_run_steps_timed(
+    long long context_key, const AeroLbmMesoscaleConfig* config, const float* forcing, int steps, float* out_ms);
+AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_get_state(long long context_key, float* out_state);
+
 /* ---- benchmark configuration (ref: native/include/aero_lbm_capi.h:698-703) ---- */
 AERO_LBM_CAPI_EXPORT void aero_lbm_benchmark_default_config(AeroLbmBenchmarkConfig* out_config);
 AERO_LBM_CAPI_EXPORT int aero_lbm_benchmark_default_preset_config(int preset, AeroLbmBenchmarkConfig* out_config);

@@ -791,6 +791,11 @@ ORACLE_API void SFX(oracle_get_f)(void* h, R* f) {
     SFX(Ctx)* c = (SFX(Ctx)*)h;
     memcpy(f, c->f, 27 * c->cells * sizeof(R));
 }
+/* post-collision populations of the last step (before streaming) */
+ORACLE_API void SFX(oracle_get_f_post)(void* h, R* f) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    memcpy(f, c->f_post, 27 * c->cells * sizeof(R));
+}
 ORACLE_API void SFX(oracle_set_f)(void* h, const R* f) {
     SFX(Ctx)* c = (SFX(Ctx)*)h;
     memcpy(c->f, f, 27 * c->cells * sizeof(R));

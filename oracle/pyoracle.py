@@ -45,6 +45,7 @@ def load():
         getattr(lib, p + "oracle_extract_atlas").argtypes = [c_void_p, c_int, POINTER(c_float)]
         getattr(lib, p + "oracle_get_f").argtypes = [c_void_p, R]
         getattr(lib, p + "oracle_set_f").argtypes = [c_void_p, R]
+        getattr(lib, p + "oracle_get_f_post").argtypes = [c_void_p, R]
         getattr(lib, p + "oracle_get_velocity").argtypes = [c_void_p, R]
         getattr(lib, p + "oracle_get_temperature_r").argtypes = [c_void_p, R]
         getattr(lib, p + "oracle_get_params").argtypes = [c_void_p, R]
@@ -129,6 +130,12 @@ class Oracle:
     def populations(self):
         out = np.zeros(27 * self.cells, dtype=self.real)
         self._f("oracle_get_f")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
+        return out.reshape(27, self.cells)
+
+    def post_collision(self):
+        """Post-collision populations of the last step, before streaming, [27, cells]."""
+        out = np.zeros(27 * self.cells, dtype=self.real)
+        self._f("oracle_get_f_post")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
         return out.reshape(27, self.cells)
 
     def set_populations(self, f):

@@ -79,3 +79,29 @@ def test_no_gpu_fails_loudly():
     assert lib.aero_lbm_init_rect(8, 8, 8, 11, 4) == 0
     assert "no HIP device" in A.last_error()
     assert lib.aero_lbm_runtime_info().decode().startswith(("hip|unavailable", "uninitialized"))
+
+
+def test_mesoscale_transport_and_config_validation(oracle_mod):
+    """Host-side mesoscale entry points: transport identical to the reference's own
+    hydro/thermal core (oracle/_ref), reference defaults, ABI-version check."""
+    lib = A.load()
+    cfg = A.MesoscaleConfig()
+    lib.aero_lbm_mesoscale_default_config(ctypes.byref(cfg))
+    assert (cfg.abi_version, cfg.struct_size, cfg.nx, cfg.ny, cfg.nz) == (3, 96, 32, 4, 32)
+    assert cfg.dx_m == 64.0 and cfg.dt_s == 5.0
+    ref = oracle_mod.load_ref()
+    for dx, dt, nu, pr in ((64.0, 5.0, 1.5e-5, 0.71), (32.0, 2.5, 0.06, 0.9), (8.0, 0.1, 2.0, 0.71)):
+        cfg.dx_m, cfg.dt_s, cfg.molecular_nu_m2_s, cfg.prandtl_air = dx, dt, nu, pr
+        tr = A.MesoscaleTransport()
+        assert lib.aero_lbm_mesoscale_derive_transport(ctypes.byref(cfg), ctypes.byref(tr)) == 1
+        if ref is not None:
+            out = (ctypes.c_float * 5)()
+            ref.ref_derive_transport(cfg.nx, cfg.ny, cfg.nz, dx, dt, nu, pr, cfg.turbulent_prandtl, out)
+            got = (tr.velocity_scale_m_s_per_lattice, tr.nu_molecular_lattice, tr.alpha_molecular_lattice,
+                   tr.tau_shear_molecular, tr.tau_thermal_molecular)
+            assert got == tuple(out)
+    cfg.abi_version = 2
+    assert lib.aero_lbm_mesoscale_derive_transport(ctypes.byref(cfg), ctypes.byref(tr)) == 0
+    key = ctypes.c_longlong(0)
+    assert lib.aero_lbm_mesoscale_create_context(ctypes.byref(cfg), ctypes.byref(key)) == 0
+    assert lib.aero_lbm_mesoscale_set_tau_floor(0.4) == 0
