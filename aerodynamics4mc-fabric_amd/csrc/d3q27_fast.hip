@@ -158,51 +158,11 @@ __device__ __forceinline__ float temperature_fast(const StepParams& P, const flo
     return clampf(adv + fmaf(P.alpha, lap, source), -1.0f, 1.0f);
 }
 
-// ---------------------------------------------------------------- rare paths
-// Rarely taken branches read their parameters through the device copy of the
-// step parameters, so the bulk path keeps no scalar registers for them. (Calls
-// to out-of-line functions would give every wave a scratch segment, which costs
-// more than the branches themselves.)
-__device__ __forceinline__ void copy27(float* d, const float* s) {
+// A-A face store: this step's post-collision values of the cells in the first
+// three layers of faces whose rule reads the previous step (symmetry, convective)
+__device__ __forceinline__ void face_store(const StepParams& P, int x, int y, int z, const float g[27],
+                                           float* __restrict__ store_out) {
 #pragma unroll
-    for (int q = 0; q < 27; ++q) d[q] = s[q];
-}
-
-// links through faces whose rule is not plain bounce-back (jni.cpp:743-785)
-template <class St, int Mode>
-__device__ __forceinline__ void rare_face_links(const StepParams* __restrict__ Pd, const uint8_t* flags,
-                                             const float* ref_p, const typename St::T* fin, const float* store_in,
-                                             int cell, int x, int y, int z, uint32_t links, float* g) {
-    const StepParams& P = *Pd;
-    const PrevPost<St, Mode != 0> prev{fin, store_in, (size_t)P.plane};
-    for (int q = 0; q < 27; ++q)
-        if (((links >> q) & 1u) && link_oob(P, x, y, z, q))
-            g[q] = face_link<St, Mode != 0>(P, flags, ref_p, prev, cell, x, y, z, q, g[q]);
-}
-
-// temperature next to a Dirichlet / Neumann thermal face, or with thermal
-// periodicity that differs from the hydrodynamic one
-__device__ __forceinline__ float rare_temperature(const StepParams* __restrict__ Pd, const float* T, const uint8_t* flags,
-                                               const float* src, int cell, int x, int y, int z, float ux, float uy,
-                                               float uz) {
-    const StepParams& P = *Pd;
-    return temperature_step(P, T, flags, src, cell, x, y, z, make_nbr(P, x, y, z), false, ux, uy, uz);
-}
-
-// reseed of a non-finite cell, then the edge sponge
-template <bool Shifted>
-__device__ __forceinline__ void rare_after_collide(const StepParams* __restrict__ Pd, CellFields C, int cell, int x, int y,
-                                                int z, float* g, CollideResult* r) {
-    const StepParams& P = *Pd;
-    if (!r->ok) reseed<Shifted>(P, C, cell, g, *r);
-    if (P.sponge_on) sponge<Shifted>(P, x, y, z, g);
-}
-
-// A-A face store: post-collision values of the first three layers of faces whose
-// rule reads the previous step (symmetry, convective)
-__device__ __forceinline__ void rare_face_store(const StepParams* __restrict__ Pd, int x, int y, int z, const float* g,
-                                             float* store_out) {
-    const StepParams& P = *Pd;
     for (int face = 0; face < 6; ++face) {
         if (!((P.face_store_faces >> face) & 1)) continue;
         const int axis = face >> 1;
@@ -215,39 +175,118 @@ __device__ __forceinline__ void rare_face_store(const StepParams* __restrict__ P
         const int a = axis == 0 ? y : x;
         const int bb = axis == 2 ? y : z;
         const int cdir = (face & 1) ? 0 : 2;  // incoming through the min face: c = +1 (index 2)
-        const long long off = P.face_store_off[face];
+        float* dst = store_out + P.face_store_off[face] + (size_t)(layer * 9) * A * B + (size_t)a * B + bb;
+#pragma unroll
         for (int j = 0; j < 9; ++j) {
             const int q = axis == 0 ? cdir * 9 + j : axis == 1 ? (j / 3) * 9 + cdir * 3 + (j % 3) : j * 3 + cdir;
-            store_out[off + ((size_t)(layer * 9 + j) * A + a) * B + bb] = g[q];
+            dst[(size_t)j * A * B] = g[q];
         }
     }
 }
 
-// ---------------------------------------------------------------- step kernel
+// ---------------------------------------------------------------- general cell update
+// Every rule a cell can need: per-link addressing, face rules other than
+// bounce-back, the general temperature stencil, reseed of non-finite cells, the
+// edge sponge and the A-A face store. The rare-cell kernel runs it for the cells
+// the link mask marks kMaskRare; keeping
+// it out of the bulk kernel keeps that kernel small (registers, instruction
+// cache).
+template <class St, class Uv, int Mode>
+__device__ __forceinline__ void general_cell(const StepParams& P, const CellFields& C, uint32_t m, int cell, int x,
+                                             int y, int z, const typename St::T* __restrict__ fin, typename St::T* fout,
+                                             const float* __restrict__ t_in, float* __restrict__ t_out,
+                                             typename Uv::T* __restrict__ u_state, const float* __restrict__ store_in,
+                                             float* __restrict__ store_out, int local_first) {
+    using T = typename St::T;
+    const size_t N = (size_t)P.cells, Np = (size_t)P.plane;
+    const uint32_t links = m & kMaskLinks;
+    const Nbr nb = make_nbr(P, x, y, z);
+    const T* const src = Mode == 0 ? fin : fout;
+    float g[27];
+    if (Mode == 1 || local_first) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) g[q] = St::ld(src + q * Np + cell);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const bool sp = (links >> q) & 1u;
+            // special link: the cell's own post-collision value of opp(q) (halfway bounce-back)
+            const int qs = Mode == 0 ? (sp ? lat_opp(q) : q) : (sp ? q : lat_opp(q));
+            const int s = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
+            g[q] = St::ld(src + qs * Np + s);
+        }
+    }
+    if ((m & kMaskFace) && !local_first) {
+        const PrevPost<St, Mode != 0> prev{fin, store_in, Np};
+        for (int q = 0; q < 27; ++q) {
+            if (((links >> q) & 1u) && link_oob(P, x, y, z, q)) {
+                float bnc = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 27; ++k)
+                    if (k == q) bnc = g[k];
+                const float v = face_link<St, Mode != 0>(P, C.flags, C.ref_p, prev, cell, x, y, z, q, bnc);
+#pragma unroll
+                for (int k = 0; k < 27; ++k)
+                    if (k == q) g[k] = v;
+            }
+        }
+    }
+    float temp = 0.0f;
+    if (P.thermal_on) {
+        const float ux0 = Uv::ld(u_state + cell), uy0 = Uv::ld(u_state + N + cell), uz0 = Uv::ld(u_state + 2 * N + cell);
+        if ((m & kMaskTherm) || !P.thermal_wrap_matches)
+            temp = temperature_step(P, t_in, C.flags, C.src, cell, x, y, z, nb, false, ux0, uy0, uz0);
+        else
+            temp = temperature_fast(P, t_in, C.src, cell, x, y, z, nb, links, ux0, uy0, uz0);
+        t_out[cell] = temp;
+    }
+    CollideResult r = collide<St::kShifted>(P, g, temp);
+    if (!r.ok) reseed<St::kShifted>(P, C, cell, g, r);
+    if (P.sponge_on) sponge<St::kShifted>(P, x, y, z, g);
+    if (P.thermal_on) {
+        Uv::st(u_state + cell, r.ux);
+        Uv::st(u_state + N + cell, r.uy);
+        Uv::st(u_state + 2 * N + cell, r.uz);
+    }
+    if (Mode != 0 && P.face_store_faces) face_store(P, x, y, z, g, store_out);
+    if (Mode == 0) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) St::st(fout + q * Np + cell, g[q]);
+    } else if (Mode == 1) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) St::st(fout + lat_opp(q) * Np + cell, g[q]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const bool sp = (links >> lat_opp(q)) & 1u;  // target x + c_q solid or out of the domain
+            const int d = sp ? cell : cell + nb.off(lat_cx(q), lat_cy(q), lat_cz(q));
+            St::st(fout + (sp ? lat_opp(q) : q) * Np + d, g[q]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- bulk step kernel
 // Mode: 0 = A-B pull, 1 = A-A even (local), 2 = A-A odd (pull + push).
 // local_first: the state is the initial one (incoming values in own slots, no
-// stream pending): collide in place and apply no face rules.
+// stream pending): collide in place.
 //
-// Streaming runs in two passes. The bulk pass moves every link as if all
-// neighbours were fluid: stores to a neighbour row outside the domain are
-// skipped (a wave-uniform branch) and a neighbour z outside the domain has a
-// dropped byte offset, while accesses to solid neighbours touch their slots,
-// which no fluid cell reads. The fix-up pass then applies
-// halfway bounce-back to the links the cell's mask flags, for only those link
-// directions that some lane of the wave needs (a wave-uniform branch each).
+// Cells at faces whose rule is not plain bounce-back, next to Dirichlet / Neumann
+// thermal faces, with sponge or face-store duties, and all cells when the thermal
+// periodicity differs from the hydrodynamic one carry kMaskRare and are left to
+// the rare-cell kernel; here every special link is halfway bounce-back.
 template <class St, class Uv, int Mode, int StPol>
 __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
                                                    const typename St::T* __restrict__ fin, typename St::T* fout,
                                                    const float* __restrict__ t_in, float* __restrict__ t_out,
-                                                   typename Uv::T* __restrict__ u_state, const float* __restrict__ store_in,
-                                                   float* __restrict__ store_out, int local_first) {
+                                                   typename Uv::T* __restrict__ u_state, float* __restrict__ store_out,
+                                                   int local_first) {
     const TileCoord tc = tile_cell(P);
     if (!tc.valid) return;
     const int x = tc.x, y = tc.y, z = tc.z;
     const RowNbr R = row_nbr(P, x, y);
     const int cell = R.row + z;
     const uint32_t m = mask[cell];
-    if (m & kMaskSolid) return;  // solid cells hold no populations in the fast paths
+    if (m & (kMaskSolid | kMaskRare)) return;
     const size_t N = (size_t)P.cells;
     const uint32_t links = m & kMaskLinks;
     using T = typename St::T;
@@ -258,79 +297,53 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     const Rsrc rin = make_rsrc(src, 27 * Ni * (int)sizeof(T));
     const Rsrc rout = make_rsrc(fout, 27 * Ni * (int)sizeof(T));
     float g[27];
-    // ---- incoming populations
     if (Mode == 1 || local_first) {
 #pragma unroll
         for (int q = 0; q < 27; ++q) g[q] = St::template bld<kPolDefault>(rin, zb[1], plane_row<T>(Ni, q, R.row));
     } else {
+        // Each link has two candidate sources: the neighbour's slot (streaming) and
+        // the cell's own slot of the opposite direction (halfway bounce-back, when
+        // the mask flags the link). Both lie in one buffer; the scalar offset is the
+        // lower of the two and each lane picks its vector offset, so a bounced link
+        // costs no second load.
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const int dx = -lat_cx(q), dy = -lat_cy(q), dz = -lat_cz(q);
-            const int qs = Mode == 0 ? q : lat_opp(q);
-            // a row outside the domain is read at the cell's own row; the fix-up replaces it
-            const int row = R.ok(dx, dy) ? R.off(dx, dy) : R.row;
-            g[q] = St::template bld<kPolDefault>(rin, zb[dz + 1], plane_row<T>(Ni, qs, row));
-        }
-        // halfway bounce-back: the cell's own post-collision value of opp(q)
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
+            const int qa = Mode == 0 ? q : lat_opp(q);  // streamed: plane at the neighbour
+            const int qb = Mode == 0 ? lat_opp(q) : q;  // bounced: plane at the cell itself
+            const int sa = plane_row<T>(Ni, qa, R.ok(dx, dy) ? R.off(dx, dy) : R.row);
+            const int sb = plane_row<T>(Ni, qb, R.row);
+            const int so = min(sa, sb);
             const bool sp = (links >> q) & 1u;
-            if (__builtin_amdgcn_ballot_w64(sp)) {
-                const int qs = Mode == 0 ? lat_opp(q) : q;
-                if (sp) g[q] = St::template bld<kPolDefault>(rin, zb[1], plane_row<T>(Ni, qs, R.row));
-            }
+            const int voff = sp ? zb[1] + (sb - so) : zb[dz + 1] + (sa - so);
+            g[q] = St::template bld<kPolDefault>(rin, voff, so);
         }
     }
-    // ---- links through faces with a rule other than bounce-back
-    if ((m & kMaskFace) && !local_first) {
-        float gt[27];
-        copy27(gt, g);
-        rare_face_links<St, Mode>(P.self_dev, C.flags, C.ref_p, fin, store_in, cell, x, y, z, links, gt);
-        copy27(g, gt);
-    }
-    // ---- temperature
     float temp = 0.0f;
+    float ux0 = 0.0f, uy0 = 0.0f, uz0 = 0.0f;
     if (P.thermal_on) {
-        const float ux0 = Uv::ld(u_state + cell), uy0 = Uv::ld(u_state + N + cell), uz0 = Uv::ld(u_state + 2 * N + cell);
-        if ((m & kMaskTherm) || !P.thermal_wrap_matches)
-            temp = rare_temperature(P.self_dev, t_in, C.flags, C.src, cell, x, y, z, ux0, uy0, uz0);
-        else
-            temp = temperature_fast(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0);
-        t_out[cell] = temp;
+        ux0 = Uv::ld(u_state + cell);
+        uy0 = Uv::ld(u_state + N + cell);
+        uz0 = Uv::ld(u_state + 2 * N + cell);
+        temp = temperature_fast(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0);
     }
-    // ---- collision
     CollideResult r = collide<St::kShifted>(P, g, temp);
-    if (!r.ok || P.sponge_on) {
-        float gt[27];
-        copy27(gt, g);
-        rare_after_collide<St::kShifted>(P.self_dev, C, cell, x, y, z, gt, &r);
-        copy27(g, gt);
-    }
+    if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
     if (P.thermal_on) {
+        t_out[cell] = temp;
         Uv::st(u_state + cell, r.ux);
         Uv::st(u_state + N + cell, r.uy);
         Uv::st(u_state + 2 * N + cell, r.uz);
     }
-    // ---- face store of this step's post-collision values (A-A only)
-    if (Mode != 0 && P.face_store_faces) {
-        const int fs = P.face_store_faces;
-        const bool near = ((fs & 1) && x < 3) || ((fs & 2) && x >= P.nx - 3) || ((fs & 4) && y < 3) ||
-                          ((fs & 8) && y >= P.ny - 3) || ((fs & 16) && z < 3) || ((fs & 32) && z >= P.nz - 3);
-        if (near) {
-            float gt[27];
-            copy27(gt, g);
-            rare_face_store(P.self_dev, x, y, z, gt, store_out);
-        }
-    }
-    // ---- write back
+    if (Mode != 0 && P.face_store_faces) face_store(*P.self_dev, x, y, z, g, store_out);
     const RowNbr R2 = row_nbr(P, fresh_s(x), fresh_s(y));
     if (Mode != 2) {
 #pragma unroll
         for (int q = 0; q < 27; ++q)
             St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, Mode == 0 ? q : lat_opp(q), R2.row), g[q]);
     } else {
-        // push to x + c_q; a target outside the domain is dropped, a solid target
-        // receives a value no fluid cell reads
+        // push to x + c_q; a target outside the domain is dropped (row: skipped, z:
+        // out-of-range offset), a solid target receives a value no fluid cell reads
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const int dx = lat_cx(q), dy = lat_cy(q), dz = lat_cz(q);
@@ -344,6 +357,30 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
                 if (sp) St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, lat_opp(q), R2.row), g[q]);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------- rare-cell kernels
+// One thread per listed rare cell. The listed cells are independent of the bulk
+// cells within a step (every cell reads the previous state and writes only its
+// own outputs), so this kernel runs beside the bulk kernel on a second stream.
+template <class St, class Uv, int Mode>
+__global__ void __launch_bounds__(64) rare_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
+                                                   const typename St::T* __restrict__ fin, typename St::T* fout,
+                                                   const float* __restrict__ t_in, float* __restrict__ t_out,
+                                                   typename Uv::T* __restrict__ u_state, const float* __restrict__ store_in,
+                                                   float* __restrict__ store_out, int local_first,
+                                                   const int* __restrict__ list, int count) {
+    // the bulk kernel runs beside this one and finishes first without it: give these
+    // long dependency chains the issue slots first
+    __builtin_amdgcn_s_setprio(3);
+    const int yz = P.ny * P.nz;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const int cell = list[i];
+        const int x = cell / yz, rem = cell - x * yz;
+        const int y = rem / P.nz, z = rem - y * P.nz;
+        general_cell<St, Uv, Mode>(P, C, mask[cell], cell, x, y, z, fin, fout, t_in, t_out, u_state, store_in,
+                                   store_out, local_first);
     }
 }
 
@@ -503,7 +540,67 @@ __global__ void __launch_bounds__(256) mask_kernel(StepParams P, const uint8_t* 
             m |= 1u << q;
         }
     }
-    mask[cell] = m;
+    bool rare = !P.thermal_wrap_matches || (m & (kMaskFace | kMaskTherm));
+    const int d = min(min(min(x, y), z), min(min(P.nx - 1 - x, P.ny - 1 - y), P.nz - 1 - z));
+    if (P.sponge_on && d < 4) rare = true;
+    mask[cell] = m | (rare ? kMaskRare : 0u);
+}
+
+// Ordered list of the fluid cells marked kMaskRare (ascending cell index, so a
+// wavefront of the rare-cell kernel reads consecutive cells): per-workgroup
+// counts, one-workgroup exclusive scan, per-workgroup scatter.
+__device__ __forceinline__ bool is_rare(const uint32_t* mask, int cells, int cell) {
+    return cell < cells && (mask[cell] & kMaskRare) && !(mask[cell] & kMaskSolid);
+}
+
+__global__ void __launch_bounds__(256) rare_count_kernel(int cells, const uint32_t* __restrict__ mask,
+                                                         int* __restrict__ counts) {
+    __shared__ int wave_n[4];
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    const uint64_t b = __ballot(is_rare(mask, cells, cell));
+    if ((threadIdx.x & 63) == 0) wave_n[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wave_n[0] + wave_n[1] + wave_n[2] + wave_n[3];
+}
+
+__global__ void __launch_bounds__(1024) rare_scan_kernel(int blocks, int* __restrict__ counts, int* __restrict__ total) {
+    __shared__ int part[1024];
+    const int per = (blocks + 1023) / 1024;
+    const int b0 = threadIdx.x * per, b1 = min(blocks, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += counts[b];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < 1024; ++t) {
+            const int v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    int run = part[threadIdx.x];
+    for (int b = b0; b < b1; ++b) {
+        const int v = counts[b];
+        counts[b] = run;  // exclusive offset
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(256) rare_scatter_kernel(int cells, const uint32_t* __restrict__ mask,
+                                                           const int* __restrict__ offsets, int* __restrict__ list) {
+    __shared__ int wave_n[4];
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    const bool r = is_rare(mask, cells, cell);
+    const uint64_t b = __ballot(r);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_n[wave] = __popcll(b);
+    __syncthreads();
+    int base = offsets[blockIdx.x];
+    for (int w = 0; w < wave; ++w) base += wave_n[w];
+    if (r) list[base + __popcll(b & ((1ull << lane) - 1ull))] = cell;
 }
 
 }  // namespace fast
@@ -561,58 +658,86 @@ static int store_policy(size_t working_set) {
     return working_set > kResidentBytes ? fast::kPolStream : fast::kPolDefault;
 }
 
-template <class St, class Uv, int StPol>
-static void launch_step_p(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
+template <class St, class Uv, int Mode, int StPol>
+static void launch_step_m(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
                           const float* t_in, float* t_out, void* u_state, const float* store_in, float* store_out,
-                          int mode, int local_first, hipStream_t s) {
+                          int local_first, const FastRare& rare, hipStream_t s) {
     using T = typename St::T;
     using U = typename Uv::T;
     const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
+    const int lf = Mode == 2 ? 0 : local_first;
+    const bool side = rare.count > 0;
+    if (side) {
+        // listed rare cells on the side stream, concurrent with the bulk kernel
+        (void)hipEventRecord(rare.fork, s);
+        (void)hipStreamWaitEvent(rare.side, rare.fork, 0);
+        const dim3 rgrid((unsigned)std::min(16384, (rare.count + 63) / 64));
+        hipLaunchKernelGGL((fast::rare_kernel<St, Uv, Mode>), rgrid, dim3(64), 0, rare.side, P, C, mask,
+                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf, rare.list,
+                           rare.count);
+        (void)hipEventRecord(rare.join, rare.side);
+    }
+    hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
+                       (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
+    if (side) (void)hipStreamWaitEvent(s, rare.join, 0);
+}
+
+template <class St, class Uv, int StPol>
+static void launch_step_p(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
+                          const float* t_in, float* t_out, void* u_state, const float* store_in, float* store_out,
+                          int mode, int local_first, const FastRare& rare, hipStream_t s) {
     if (mode == 0)
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 0, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
-                           (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, local_first);
+        launch_step_m<St, Uv, 0, StPol>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, local_first,
+                                        rare, s);
     else if (mode == 1)
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 1, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
-                           (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, local_first);
+        launch_step_m<St, Uv, 1, StPol>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, local_first,
+                                        rare, s);
     else
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, 2, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
-                           (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, 0);
+        launch_step_m<St, Uv, 2, StPol>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, 0, rare, s);
 }
 
 template <class St, class Uv>
 static void launch_step_t(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
                           const float* t_in, float* t_out, void* u_state, const float* store_in, float* store_out,
-                          int mode, int local_first, hipStream_t s) {
+                          int mode, int local_first, const FastRare& rare, hipStream_t s) {
     const size_t N = (size_t)P.cells;
     // populations (two arrays for A-B), temperature pair, carried velocity, link mask, flags
     const size_t ws = (mode == 0 ? 2 : 1) * 27 * sizeof(typename St::T) * (size_t)P.plane +
                       N * (8 + 3 * sizeof(typename Uv::T) + 5);
     if (store_policy(ws) == fast::kPolStream)
         launch_step_p<St, Uv, fast::kPolStream>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out,
-                                                mode, local_first, s);
+                                                mode, local_first, rare, s);
     else
         launch_step_p<St, Uv, fast::kPolDefault>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out,
-                                                 mode, local_first, s);
+                                                 mode, local_first, rare, s);
 }
 
 void launch_fast_step(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* fin,
                       void* fout, const float* t_in, float* t_out, void* u_state, const float* store_in,
-                      float* store_out, int mode, int local_first, hipStream_t s) {
+                      float* store_out, int mode, int local_first, const FastRare& rare, hipStream_t s) {
     using namespace fast;
     switch (st) {
         case FastStorage::F32:
             launch_step_t<StF32, UF32>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, mode,
-                                       local_first, s);
+                                       local_first, rare, s);
             break;
         case FastStorage::F16:
             launch_step_t<StF16, UF16>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, mode,
-                                       local_first, s);
+                                       local_first, rare, s);
             break;
         case FastStorage::BF16:
             launch_step_t<StBF16, UF16>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, mode,
-                                        local_first, s);
+                                        local_first, rare, s);
             break;
     }
+}
+
+void launch_fast_rare_list(const StepParams& P, const uint32_t* mask, int* list, int* scratch, int* counter,
+                           hipStream_t s) {
+    const int blocks = (P.cells + 255) / 256;
+    hipLaunchKernelGGL(fast::rare_count_kernel, dim3(blocks), dim3(256), 0, s, P.cells, mask, scratch);
+    hipLaunchKernelGGL(fast::rare_scan_kernel, dim3(1), dim3(1024), 0, s, blocks, scratch, counter);
+    hipLaunchKernelGGL(fast::rare_scatter_kernel, dim3(blocks), dim3(256), 0, s, P.cells, mask, scratch, list);
 }
 
 void launch_fast_readout(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* f,

@@ -13,6 +13,10 @@ constexpr uint32_t kMaskLinks = (1u << 27) - 1u;
 constexpr uint32_t kMaskSolid = 1u << 27;  // the cell itself is solid
 constexpr uint32_t kMaskFace = 1u << 28;   // a link leaves through a face whose rule is not plain bounce-back
 constexpr uint32_t kMaskTherm = 1u << 29;  // a stencil neighbour is outside the domain at a Dirichlet/Neumann thermal face
+// the cell is updated by the rare-cell kernel: a face rule other than plain
+// bounce-back or a Dirichlet / Neumann thermal face applies, the sponge touches
+// it, or the thermal periodicity differs from the hydrodynamic one
+constexpr uint32_t kMaskRare = 1u << 30;
 
 // largest population array the fast step kernel addresses (one buffer resource,
 // per-lane byte offsets below 2^31)
@@ -24,13 +28,25 @@ constexpr long long kFastMaxBufferBytes = (1ll << 31) - 1;
 int fast_plane_stride(int cells);
 
 void launch_fast_mask(const StepParams& P, const uint8_t* flags, uint32_t* mask, hipStream_t s);
+// ascending list of the fluid cells marked kMaskRare; *counter receives their
+// number; scratch holds (cells + 255) / 256 ints
+void launch_fast_rare_list(const StepParams& P, const uint32_t* mask, int* list, int* scratch, int* counter,
+                           hipStream_t s);
+
+// per-context buffers of the rare-cell pass
+struct FastRare {
+    const int* list;   // fluid cells marked kMaskRare
+    int count;
+    hipStream_t side;  // stream of the listed rare cells (runs beside the bulk kernel)
+    hipEvent_t fork, join;
+};
 void launch_fast_init(const StepParams& P, const CellFields& C, FastStorage st, void* f, float* T, float* T_other, void* u_state,
                       float t_init, hipStream_t s);
 // mode: 0 = A-B pull (local_first: collide the initial state in place),
 //       1 = A-A even (local), 2 = A-A odd (pull + push)
 void launch_fast_step(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* fin,
                       void* fout, const float* t_in, float* t_out, void* u_state, const float* store_in,
-                      float* store_out, int mode, int local_first, hipStream_t s);
+                      float* store_out, int mode, int local_first, const FastRare& rare, hipStream_t s);
 // layout: 0 = A-B post-collision, 1 = A-A after an even step, 2 = A-A after an odd
 // step, 3 = initial state (incoming values in own slots, no face rules)
 void launch_fast_readout(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* f,

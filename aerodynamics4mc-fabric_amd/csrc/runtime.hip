@@ -63,6 +63,10 @@ bool ensure_device() {
     s.device_arch = prop.gcnArchName;
     s.cu_count = prop.multiProcessorCount;
     if (!hip_ok(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+    if (!hip_ok(hipStreamCreateWithFlags(&s.side_stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
+    if (!hip_ok(hipEventCreateWithFlags(&s.ev_fork, hipEventDisableTiming), "hipEventCreate") ||
+        !hip_ok(hipEventCreateWithFlags(&s.ev_join, hipEventDisableTiming), "hipEventCreate"))
+        return false;
     s.device_ready = true;
     return true;
 }
@@ -348,8 +352,14 @@ bool prepare_params(Context& c, StepParams& P) {
                           P.hydro_wrap_hi[1], P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]};
         if (std::memcmp(w, c.class_wrap, sizeof(w)) != 0) {
             launch_classify(P, c.flags, rt().stream);
-            storage_rebuild_masks(c, P);
             std::memcpy(c.class_wrap, w, sizeof(w));
+            c.mask_valid = false;
+        }
+        // the fast kernels' link mask also depends on face rules, sponge and face store
+        int sig[kMaskSigLen];
+        mask_signature(P, sig);
+        if (c.mask && (!c.mask_valid || std::memcmp(sig, c.mask_sig, sizeof(sig)) != 0)) {
+            if (!storage_rebuild_masks(c, P)) return false;
         }
     }
     P.self_dev = c.params_dev;
@@ -491,7 +501,7 @@ static bool ingest_payload(Context& c, const StepParams& P, const float* packet)
         return false;
     launch_ingest(P, c.payload_dev, c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, st);
     launch_classify(P, c.flags, st);
-    storage_rebuild_masks(c, P);
+    if (!storage_rebuild_masks(c, P)) return false;
     const int w[6] = {P.hydro_wrap_lo[0], P.hydro_wrap_hi[0], P.hydro_wrap_lo[1],
                       P.hydro_wrap_hi[1], P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]};
     std::memcpy(c.class_wrap, w, sizeof(w));

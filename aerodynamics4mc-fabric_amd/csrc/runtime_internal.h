@@ -16,6 +16,8 @@ namespace aero {
 
 enum class Storage { F32Exact, F32Fast, F16, BF16, F32AA };
 
+constexpr int kMaskSigLen = 22;
+
 struct Context {
     int nx = 0, ny = 0, nz = 0;
     size_t cells = 0;
@@ -33,6 +35,13 @@ struct Context {
     int aa_layout = 3;             // A-A: 1 after an even step, 2 after an odd step, 3 initial state
     int store_cur = 0;             // A-A: face store written by the last step
     uint32_t* mask = nullptr;      // fast paths: per-cell link mask
+    // fast paths: cells the rare-cell kernel updates
+    int* rare_list = nullptr;
+    int* rare_counter = nullptr;
+    int* rare_scratch = nullptr;
+    int rare_count = 0;
+    int mask_sig[kMaskSigLen] = {};  // parameters the link mask was built with
+    bool mask_valid = false;
     // device buffers
     uint8_t* flags = nullptr;
     float* ref_p = nullptr;
@@ -63,6 +72,9 @@ struct Context {
     ~Context();
 };
 
+// parameters the fast kernels' link mask depends on (besides the cell flags)
+void mask_signature(const StepParams& P, int* out);
+
 struct RuntimeConfig {
     bool initialized = false;
     int grid_size = 0, nx = 0, ny = 0, nz = 0, in_ch = 0, out_ch = 0;
@@ -89,6 +101,8 @@ struct RuntimeState {
     int cu_count = 0;
     bool device_ready = false;
     hipStream_t stream = nullptr;
+    hipStream_t side_stream = nullptr;  // second stream (default priority) for concurrent work
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::unordered_map<long long, std::unique_ptr<Context>> contexts;
     Timing timing;
 };
@@ -113,6 +127,6 @@ bool storage_step(Context& c, const StepParams& P);
 bool storage_readout(Context& c, const StepParams& P, int stride, int out_ch, float scale, float* out_dev);
 bool storage_sample(Context& c, const StepParams& P, int x, int y, int z, float* out_dev4);
 size_t context_device_bytes(const Context& c);
-void storage_rebuild_masks(Context& c, const StepParams& P);
+bool storage_rebuild_masks(Context& c, const StepParams& P);
 
 }  // namespace aero

@@ -11,6 +11,18 @@ static FastStorage fast_kind(Storage s) {
     return s == Storage::F16 ? FastStorage::F16 : (s == Storage::BF16 ? FastStorage::BF16 : FastStorage::F32);
 }
 
+// link mask and rare-cell list of the fast kernels
+static bool alloc_fast_aux(Context& c) {
+    const size_t N = c.cells;
+    if (!(c.alloc((void**)&c.mask, N * 4) && c.alloc((void**)&c.rare_list, N * 4) &&
+          c.alloc((void**)&c.rare_counter, 4) && c.alloc((void**)&c.rare_scratch, ((N + 255) / 256) * 4) &&
+          true))
+        return false;
+    c.rare_count = 0;
+    c.mask_valid = false;
+    return true;
+}
+
 bool storage_alloc(Context& c) {
     const size_t N = c.cells;
     const size_t Np = (size_t)fast_plane_stride((int)c.cells);  // fast kernels' plane stride
@@ -28,12 +40,12 @@ bool storage_alloc(Context& c) {
             return c.alloc((void**)&c.f32[0], 27 * N * 4) && c.alloc((void**)&c.f32[1], 27 * N * 4);
         case Storage::F32Fast:
             return c.alloc((void**)&c.f32[0], 27 * Np * 4) && c.alloc((void**)&c.f32[1], 27 * Np * 4) &&
-                   c.alloc((void**)&c.mask, N * 4);
+                   alloc_fast_aux(c);
         case Storage::F16:
         case Storage::BF16:
-            return c.alloc((void**)&c.h16, 27 * Np * 2) && c.alloc((void**)&c.mask, N * 4);
+            return c.alloc((void**)&c.h16, 27 * Np * 2) && alloc_fast_aux(c);
         case Storage::F32AA:
-            return c.alloc((void**)&c.f32[0], 27 * Np * 4) && c.alloc((void**)&c.mask, N * 4);
+            return c.alloc((void**)&c.f32[0], 27 * Np * 4) && alloc_fast_aux(c);
     }
     return false;
 }
@@ -57,8 +69,35 @@ size_t context_device_bytes(const Context& c) {
     return b;
 }
 
-void storage_rebuild_masks(Context& c, const StepParams& P) {
-    if (c.mask) launch_fast_mask(P, c.flags, c.mask, rt().stream);
+void mask_signature(const StepParams& P, int* out) {
+    int k = 0;
+    for (int a = 0; a < 3; ++a) {
+        out[k++] = P.hydro_wrap_lo[a];
+        out[k++] = P.hydro_wrap_hi[a];
+    }
+    for (int f = 0; f < 6; ++f) {
+        out[k++] = P.face[f].hydro_kind;
+        out[k++] = P.face[f].thermal_kind;
+    }
+    out[k++] = P.convective_disabled;
+    out[k++] = P.sponge_on;
+    out[k++] = P.face_store_faces;
+    out[k++] = P.thermal_wrap_matches;
+}
+
+bool storage_rebuild_masks(Context& c, const StepParams& P) {
+    if (!c.mask) return true;
+    hipStream_t st = rt().stream;
+    launch_fast_mask(P, c.flags, c.mask, st);
+    launch_fast_rare_list(P, c.mask, c.rare_list, c.rare_scratch, c.rare_counter, st);
+    int count = 0;
+    if (!hip_ok(hipMemcpyAsync(&count, c.rare_counter, 4, hipMemcpyDeviceToHost, st), "rare count") ||
+        !hip_ok(hipStreamSynchronize(st), "sync"))
+        return false;
+    c.rare_count = count;
+    mask_signature(P, c.mask_sig);
+    c.mask_valid = true;
+    return true;
 }
 
 static bool ensure_face_store(Context& c, const StepParams& P) {
@@ -106,16 +145,18 @@ bool storage_step(Context& c, const StepParams& P) {
         c.aa_layout = 3;
     }
     if (c.storage == Storage::F32Fast) {
+        const FastRare rare{c.rare_list, c.rare_count, rt().side_stream, rt().ev_fork, rt().ev_join};
         launch_fast_step(P, C, fk, c.mask, c.f32[c.f_cur], c.f32[1 - c.f_cur], c.T[c.t_cur], c.T[1 - c.t_cur],
-                         c.u_state, nullptr, nullptr, 0, c.post_collision ? 0 : 1, st);
+                         c.u_state, nullptr, nullptr, 0, c.post_collision ? 0 : 1, rare, st);
         c.f_cur ^= 1;
         c.post_collision = true;
     } else {
         const bool even = c.aa_layout == 3 || c.aa_layout == 2;
         const int mode = even ? 1 : 2;
+        const FastRare rare{c.rare_list, c.rare_count, rt().side_stream, rt().ev_fork, rt().ev_join};
         launch_fast_step(P, C, fk, c.mask, aa_buffer(c), aa_buffer(c), c.T[c.t_cur], c.T[1 - c.t_cur], c.u_state,
                          c.face_store[c.store_cur], c.face_store[1 - c.store_cur], mode, c.aa_layout == 3 ? 1 : 0,
-                         st);
+                         rare, st);
         c.store_cur ^= 1;
         c.aa_layout = even ? 1 : 2;
         c.post_collision = true;

@@ -83,12 +83,38 @@ def config3_small(steps=300, n=32):
     return out
 
 
-JOBS = {"config1": config1, "config3_small": config3_small}
+def config3_full(steps=1000, n=128):
+    """FP64 oracle only (about 15 min on 8 cores): the reference for the FP16 / BF16 /
+    FP32 accuracy of the 128^3 workload after 1000 steps."""
+    cfg, ov, pk, t, solid = W.config3(n, beta=0.1, cs=0.1)
+    out = {"name": "config3_full", "n": n, "steps": steps,
+           "desc": "D3Q27 cumulant + SGS (C=0.1) + Boussinesq (beta 0.1, gravity -z) at 128^3: ~10% solid "
+                   "boxes (seed 5), Gaussian hot blob dT 0.02 sigma 8 on the z=0 floor (seed 4), inflow 0.05; "
+                   "FP64 oracle"}
+    idx = probes(n ** 3, count=4096, seed=7)
+    t0 = time.time()
+    o = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=64)
+    o.set_temperature(t)
+    o.run(pk, steps)
+    full = summarize(o.flow(), o.temperature(), n, idx)
+    out["fp64"] = {"probe_cells": idx,
+                   "probe_flow": full["probe_flow"],  # output channels (u_x, u_y, u_z, rho - 1)
+                   "probe_temperature": [float("%.9g" % v) for v in full["probe_temperature"]],
+                   "plane_sums": [[float("%.12g" % v) for v in row] for row in full["plane_sums"]],
+                   "flow_sha256": full["flow_sha256"], "seconds": time.time() - t0}
+    print(f"config3_full fp64: {time.time() - t0:.1f} s", flush=True)
+    return out
+
+
+JOBS = {"config1": config1, "config3_small": config3_small, "config3_full": config3_full}
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(JOBS)
+    names = sys.argv[1:] or ["config1", "config3_small"]
     for name in names:
         res = JOBS[name]()
         with open(os.path.join(HERE, f"{name}.json"), "w") as fh:
-            json.dump(res, fh, indent=0)
+            if name == "config3_full":
+                json.dump(res, fh, separators=(",", ":"))
+            else:
+                json.dump(res, fh, indent=0)
         print("wrote", name, flush=True)
