@@ -109,12 +109,21 @@ assert ctypes.sizeof(MesoscaleConfig) == 96
 assert ctypes.sizeof(MesoscaleTransport) == 20
 
 
+class WorldDelta(Structure):
+    _fields_ = [("type", c_int), ("x", c_int), ("y", c_int), ("z", c_int), ("data0", c_int), ("data1", c_int),
+                ("data2", c_int), ("data3", c_int), ("value0", c_float), ("value1", c_float), ("value2", c_float),
+                ("value3", c_float)]
+
+
 class AeroLbmError(RuntimeError):
     pass
 
 
 _F = POINTER(c_float)
 _I = POINTER(c_int)
+_U8 = POINTER(ctypes.c_uint8)
+_U16 = POINTER(ctypes.c_uint16)
+_I16 = POINTER(ctypes.c_int16)
 
 # name -> (restype, argtypes); mirrors include/aero_lbm_capi.h
 PROTOTYPES = {
@@ -165,6 +174,38 @@ PROTOTYPES = {
     "aero_lbm_mesoscale_set_tau_floor": (c_int, [c_float]),
     "aero_lbm_mesoscale_run_steps_timed": (c_int, [c_longlong, POINTER(MesoscaleConfig), _F, c_int, _F]),
     "aero_lbm_mesoscale_get_state": (c_int, [c_longlong, _F]),
+    "aero_lbm_set_d3q27_f16_boundary_shell_rect": (c_int, [c_int, c_int, c_int, c_longlong, _F, c_int, c_int, c_int,
+                                                           c_float]),
+    "aero_lbm_clear_d3q27_f16_boundary_shell_rect": (c_int, [c_longlong]),
+    "aero_lbm_simulation_create_service": (c_int, [POINTER(c_longlong)]),
+    "aero_lbm_simulation_release_service": (None, [c_longlong]),
+    "aero_lbm_simulation_set_focus": (c_int, [c_longlong, c_int, c_int, c_int, c_int]),
+    "aero_lbm_simulation_submit_world_deltas": (c_int, [c_longlong, POINTER(WorldDelta), c_int]),
+    "aero_lbm_simulation_ensure_brick_world_runtime": (c_int, [c_longlong, c_longlong, c_int, c_float, c_float]),
+    "aero_lbm_simulation_set_brick_world_solver_mode": (c_int, [c_longlong, c_longlong, c_int]),
+    "aero_lbm_simulation_set_brick_world_active_hints": (c_int, [c_longlong, c_longlong, c_int, _I, c_int]),
+    "aero_lbm_simulation_set_brick_world_exact_active_hints": (c_int, [c_longlong, c_longlong, c_int, _I, c_int]),
+    "aero_lbm_simulation_get_brick_world_runtime_status": (c_int, [c_longlong, c_longlong, _I, c_int]),
+    "aero_lbm_simulation_step_brick_world_runtime": (c_int, [c_longlong, c_longlong, c_int]),
+    "aero_lbm_simulation_get_brick_world_resident_brick_count": (c_int, [c_longlong, c_longlong]),
+    "aero_lbm_simulation_copy_brick_world_resident_bricks": (c_int, [c_longlong, c_longlong, _I, c_int]),
+    "aero_lbm_simulation_get_brick_world_active_brick_count": (c_int, [c_longlong, c_longlong]),
+    "aero_lbm_simulation_copy_brick_world_active_bricks": (c_int, [c_longlong, c_longlong, _I, c_int]),
+    "aero_lbm_simulation_upload_brick_world_static_brick": (c_int, [c_longlong, c_longlong, c_int, c_int, c_int, c_int,
+                                                                    _U8, _U8, _U16, _F, _U8, _U8]),
+    "aero_lbm_simulation_upload_brick_world_static_brick_with_sources": (
+        c_int, [c_longlong, c_longlong, c_int, c_int, c_int, c_int, _U8, _U8, _U16, _F, _U8, _F, _U8, _U8]),
+    "aero_lbm_simulation_queue_brick_world_static_brick_upload": (c_int, [c_longlong, c_longlong, c_int, c_int, c_int,
+                                                                          c_int, _U8, _U8, _U16, _F, _U8, _U8]),
+    "aero_lbm_simulation_upload_brick_world_dynamic_brick": (c_int, [c_longlong, c_longlong, c_int, c_int, c_int,
+                                                                     c_int, _F, _F, _F]),
+    "aero_lbm_simulation_upload_brick_world_boundary_reference_brick": (c_int, [c_longlong, c_longlong, c_int, c_int,
+                                                                                c_int, c_int, _F, _F, _F]),
+    "aero_lbm_simulation_copy_brick_world_dynamic_brick": (c_int, [c_longlong, c_longlong, c_int, c_int, c_int, c_int,
+                                                                   _F, _F, _F]),
+    "aero_lbm_simulation_copy_brick_world_packed_flow_atlas": (c_int, [c_longlong, c_longlong, c_int, c_int, c_int,
+                                                                       c_int, c_int, _I16, c_int]),
+    "aero_lbm_simulation_last_error": (c_char_p, []),
 }
 
 
@@ -326,3 +367,97 @@ class Mesoscale:
         if self.key:
             self.lib.aero_lbm_mesoscale_release_context(self.key)
             self.key = 0
+
+
+def _ptr(a: np.ndarray, kind):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(kind)
+
+
+class BrickWorld:
+    """One brick world of the native brick runtime (aero_lbm_simulation_*)."""
+
+    def __init__(self, world: int = 1, size: int = 16, dx: float = 1.0, dt: float = 0.05,
+                 mode: int = SOLVER_D3Q27_FP16_INPLACE):
+        self.lib = load()
+        key = c_longlong(0)
+        self._check(self.lib.aero_lbm_simulation_create_service(ctypes.byref(key)), "create service")
+        self.service, self.world, self.size = key.value, world, size
+        self._check(self.lib.aero_lbm_simulation_ensure_brick_world_runtime(self.service, world, size, dx, dt),
+                    "ensure world")
+        self._check(self.lib.aero_lbm_simulation_set_brick_world_solver_mode(self.service, world, mode), "solver mode")
+
+    def _check(self, ok, what):
+        if not ok:
+            msg = self.lib.aero_lbm_simulation_last_error()
+            raise AeroLbmError(f"{what} failed: {msg.decode() if msg else ''}")
+
+    def upload_static(self, coord, obstacle, surface_kind=None, open_face=None, emitter=None, fan_src=None,
+                      emitter_src=None):
+        n = self.size
+        cells = n ** 3
+        ob = np.ascontiguousarray(obstacle, dtype=np.uint8).reshape(-1)
+        kind = np.zeros(cells, np.uint8) if surface_kind is None else np.ascontiguousarray(surface_kind, np.uint8)
+        of = np.zeros(cells, np.uint16) if open_face is None else np.ascontiguousarray(open_face, np.uint16)
+        em = np.zeros(cells, np.float32) if emitter is None else np.ascontiguousarray(emitter, np.float32)
+        sky = np.zeros(cells * 6, np.uint8)
+        fs = None if fan_src is None else np.ascontiguousarray(fan_src, np.uint8)
+        es = None if emitter_src is None else np.ascontiguousarray(emitter_src, np.float32)
+        self._check(self.lib.aero_lbm_simulation_upload_brick_world_static_brick_with_sources(
+            self.service, self.world, n, *coord, _ptr(ob, _U8), _ptr(kind, _U8), _ptr(of, _U16), _ptr(em, _F),
+            None if fs is None else _ptr(fs, _U8), None if es is None else _ptr(es, _F), _ptr(sky, _U8),
+            _ptr(sky, _U8)), "upload static")
+
+    def upload_dynamic(self, coord, flow, air=None, surface=None, reference=False):
+        n = self.size
+        fl = np.ascontiguousarray(flow, np.float32).reshape(-1)
+        a = np.zeros(n ** 3, np.float32) if air is None else np.ascontiguousarray(air, np.float32)
+        s = np.zeros(n ** 3, np.float32) if surface is None else np.ascontiguousarray(surface, np.float32)
+        fn = (self.lib.aero_lbm_simulation_upload_brick_world_boundary_reference_brick if reference
+              else self.lib.aero_lbm_simulation_upload_brick_world_dynamic_brick)
+        self._check(fn(self.service, self.world, n, *coord, _ptr(fl, _F), _ptr(a, _F), _ptr(s, _F)), "upload dynamic")
+
+    def hints(self, coords, exact=True):
+        c = np.ascontiguousarray(np.array(coords, np.int32).reshape(-1))
+        fn = (self.lib.aero_lbm_simulation_set_brick_world_exact_active_hints if exact
+              else self.lib.aero_lbm_simulation_set_brick_world_active_hints)
+        self._check(fn(self.service, self.world, self.size, _ptr(c, _I), len(c) // 3), "hints")
+
+    def step(self, steps: int = 1):
+        self._check(self.lib.aero_lbm_simulation_step_brick_world_runtime(self.service, self.world, steps), "step")
+
+    def status(self):
+        out = np.zeros(8, np.int32)
+        self._check(self.lib.aero_lbm_simulation_get_brick_world_runtime_status(self.service, self.world,
+                                                                                 _ptr(out, _I), 8), "status")
+        return out
+
+    def dynamic(self, coord):
+        n = self.size
+        fl = np.zeros(n ** 3 * 4, np.float32)
+        a = np.zeros(n ** 3, np.float32)
+        s = np.zeros(n ** 3, np.float32)
+        self._check(self.lib.aero_lbm_simulation_copy_brick_world_dynamic_brick(
+            self.service, self.world, n, *coord, _ptr(fl, _F), _ptr(a, _F), _ptr(s, _F)), "copy dynamic")
+        return fl, a, s
+
+    def atlas(self, coord, stride: int = 4):
+        sn = (self.size + stride - 1) // stride
+        out = np.zeros(sn ** 3 * 4, np.int16)
+        self._check(self.lib.aero_lbm_simulation_copy_brick_world_packed_flow_atlas(
+            self.service, self.world, self.size, *coord, stride, _ptr(out, _I16), out.size), "atlas")
+        return out
+
+    def bricks(self, active=True):
+        count_fn = (self.lib.aero_lbm_simulation_get_brick_world_active_brick_count if active
+                    else self.lib.aero_lbm_simulation_get_brick_world_resident_brick_count)
+        copy_fn = (self.lib.aero_lbm_simulation_copy_brick_world_active_bricks if active
+                   else self.lib.aero_lbm_simulation_copy_brick_world_resident_bricks)
+        n = count_fn(self.service, self.world)
+        out = np.zeros(max(n, 1) * 3, np.int32)
+        got = copy_fn(self.service, self.world, _ptr(out, _I), n)
+        assert got == n
+        return sorted(tuple(int(v) for v in out[3 * i:3 * i + 3]) for i in range(n))
+
+    def release(self):
+        self.lib.aero_lbm_simulation_release_service(self.service)

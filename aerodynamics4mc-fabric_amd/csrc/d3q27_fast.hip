@@ -184,87 +184,6 @@ __device__ __forceinline__ void face_store(const StepParams& P, int x, int y, in
     }
 }
 
-// ---------------------------------------------------------------- general cell update
-// Every rule a cell can need: per-link addressing, face rules other than
-// bounce-back, the general temperature stencil, reseed of non-finite cells, the
-// edge sponge and the A-A face store. The rare-cell kernel runs it for the cells
-// the link mask marks kMaskRare; keeping
-// it out of the bulk kernel keeps that kernel small (registers, instruction
-// cache).
-template <class St, class Uv, int Mode>
-__device__ __forceinline__ void general_cell(const StepParams& P, const CellFields& C, uint32_t m, int cell, int x,
-                                             int y, int z, const typename St::T* __restrict__ fin, typename St::T* fout,
-                                             const float* __restrict__ t_in, float* __restrict__ t_out,
-                                             typename Uv::T* __restrict__ u_state, const float* __restrict__ store_in,
-                                             float* __restrict__ store_out, int local_first) {
-    using T = typename St::T;
-    const size_t N = (size_t)P.cells, Np = (size_t)P.plane;
-    const uint32_t links = m & kMaskLinks;
-    const Nbr nb = make_nbr(P, x, y, z);
-    const T* const src = Mode == 0 ? fin : fout;
-    float g[27];
-    if (Mode == 1 || local_first) {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) g[q] = St::ld(src + q * Np + cell);
-    } else {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-            const bool sp = (links >> q) & 1u;
-            // special link: the cell's own post-collision value of opp(q) (halfway bounce-back)
-            const int qs = Mode == 0 ? (sp ? lat_opp(q) : q) : (sp ? q : lat_opp(q));
-            const int s = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
-            g[q] = St::ld(src + qs * Np + s);
-        }
-    }
-    if ((m & kMaskFace) && !local_first) {
-        const PrevPost<St, Mode != 0> prev{fin, store_in, Np};
-        for (int q = 0; q < 27; ++q) {
-            if (((links >> q) & 1u) && link_oob(P, x, y, z, q)) {
-                float bnc = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 27; ++k)
-                    if (k == q) bnc = g[k];
-                const float v = face_link<St, Mode != 0>(P, C.flags, C.ref_p, prev, cell, x, y, z, q, bnc);
-#pragma unroll
-                for (int k = 0; k < 27; ++k)
-                    if (k == q) g[k] = v;
-            }
-        }
-    }
-    float temp = 0.0f;
-    if (P.thermal_on) {
-        const float ux0 = Uv::ld(u_state + cell), uy0 = Uv::ld(u_state + N + cell), uz0 = Uv::ld(u_state + 2 * N + cell);
-        if ((m & kMaskTherm) || !P.thermal_wrap_matches)
-            temp = temperature_step(P, t_in, C.flags, C.src, cell, x, y, z, nb, false, ux0, uy0, uz0);
-        else
-            temp = temperature_fast(P, t_in, C.src, cell, x, y, z, nb, links, ux0, uy0, uz0);
-        t_out[cell] = temp;
-    }
-    CollideResult r = collide<St::kShifted>(P, g, temp);
-    if (!r.ok) reseed<St::kShifted>(P, C, cell, g, r);
-    if (P.sponge_on) sponge<St::kShifted>(P, x, y, z, g);
-    if (P.thermal_on) {
-        Uv::st(u_state + cell, r.ux);
-        Uv::st(u_state + N + cell, r.uy);
-        Uv::st(u_state + 2 * N + cell, r.uz);
-    }
-    if (Mode != 0 && P.face_store_faces) face_store(P, x, y, z, g, store_out);
-    if (Mode == 0) {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) St::st(fout + q * Np + cell, g[q]);
-    } else if (Mode == 1) {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) St::st(fout + lat_opp(q) * Np + cell, g[q]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-            const bool sp = (links >> lat_opp(q)) & 1u;  // target x + c_q solid or out of the domain
-            const int d = sp ? cell : cell + nb.off(lat_cx(q), lat_cy(q), lat_cz(q));
-            St::st(fout + (sp ? lat_opp(q) : q) * Np + d, g[q]);
-        }
-    }
-}
-
 // ---------------------------------------------------------------- bulk step kernel
 // Mode: 0 = A-B pull, 1 = A-A even (local), 2 = A-A odd (pull + push).
 // local_first: the state is the initial one (incoming values in own slots, no
@@ -329,6 +248,10 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     }
     CollideResult r = collide<St::kShifted>(P, g, temp);
     if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
+    if (P.shell_on) {
+        const int layer = C.shell[cell];
+        if (layer) shell_blend<St::kShifted>(P, C.shell_state, cell, layer, g, r);
+    }
     if (P.thermal_on) {
         t_out[cell] = temp;
         Uv::st(u_state + cell, r.ux);
@@ -361,26 +284,110 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
 }
 
 // ---------------------------------------------------------------- rare-cell kernels
-// One thread per listed rare cell. The listed cells are independent of the bulk
-// cells within a step (every cell reads the previous state and writes only its
-// own outputs), so this kernel runs beside the bulk kernel on a second stream.
+// The listed rare cells take two launches. The gather launch has one thread per
+// (cell, link) and one more per cell for the temperature: every face rule and
+// thermal stencil runs as a short independent job, so the launch is wide and its
+// loads overlap. The update launch has one thread per cell: it reads the 27
+// incoming values and the new temperature back and does the rest of the update.
+// A listed cell reads and writes only slots that no other cell touches in the
+// same step, so both launches are independent of the bulk kernel.
 template <class St, class Uv, int Mode>
-__global__ void __launch_bounds__(64) rare_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
-                                                   const typename St::T* __restrict__ fin, typename St::T* fout,
-                                                   const float* __restrict__ t_in, float* __restrict__ t_out,
-                                                   typename Uv::T* __restrict__ u_state, const float* __restrict__ store_in,
-                                                   float* __restrict__ store_out, int local_first,
-                                                   const int* __restrict__ list, int count) {
-    // the bulk kernel runs beside this one and finishes first without it: give these
-    // long dependency chains the issue slots first
-    __builtin_amdgcn_s_setprio(3);
+__global__ void __launch_bounds__(256) rare_gather_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
+                                                          const typename St::T* __restrict__ fin,
+                                                          const typename St::T* __restrict__ fout,
+                                                          const float* __restrict__ t_in,
+                                                          const typename Uv::T* __restrict__ u_state,
+                                                          const float* __restrict__ store_in, int local_first,
+                                                          const int* __restrict__ list, int count, int cap,
+                                                          float* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;  // 0..26: incoming population, 27: temperature
+    if (i >= count) return;
+    const int cell = list[i];
+    const uint32_t m = mask[cell];
     const int yz = P.ny * P.nz;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const int cell = list[i];
-        const int x = cell / yz, rem = cell - x * yz;
-        const int y = rem / P.nz, z = rem - y * P.nz;
-        general_cell<St, Uv, Mode>(P, C, mask[cell], cell, x, y, z, fin, fout, t_in, t_out, u_state, store_in,
-                                   store_out, local_first);
+    const int x = cell / yz, rem = cell - x * yz;
+    const int y = rem / P.nz, z = rem - y * P.nz;
+    const size_t N = (size_t)P.cells, Np = (size_t)P.plane;
+    const uint32_t links = m & kMaskLinks;
+    const Nbr nb = make_nbr(P, x, y, z);
+    float v = 0.0f;
+    if (q == 27) {
+        if (!P.thermal_on) return;
+        const float ux0 = Uv::ld(u_state + cell), uy0 = Uv::ld(u_state + N + cell), uz0 = Uv::ld(u_state + 2 * N + cell);
+        if ((m & kMaskTherm) || !P.thermal_wrap_matches)
+            v = temperature_step(P, t_in, C.flags, C.src, cell, x, y, z, nb, false, ux0, uy0, uz0);
+        else
+            v = temperature_fast(P, t_in, C.src, cell, x, y, z, nb, links, ux0, uy0, uz0);
+    } else {
+        const typename St::T* const src = Mode == 0 ? fin : fout;
+        const bool sp = (links >> q) & 1u;
+        if (Mode == 1 || local_first) {
+            v = St::ld(src + q * Np + cell);
+        } else {
+            // special link: the cell's own post-collision value of opp(q) (halfway bounce-back)
+            const int qs = Mode == 0 ? (sp ? lat_opp(q) : q) : (sp ? q : lat_opp(q));
+            const int s = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
+            v = St::ld(src + qs * Np + s);
+        }
+        if ((m & kMaskFace) && !local_first && sp && link_oob(P, x, y, z, q)) {
+            const PrevPost<St, Mode != 0> prev{fin, store_in, Np};
+            v = face_link<St, Mode != 0>(P, C.flags, C.ref_p[cell], prev, cell, x, y, z, q, v);
+        }
+    }
+    vals[(size_t)q * cap + i] = v;
+}
+
+template <class St, class Uv, int Mode>
+__global__ void __launch_bounds__(128) rare_update_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
+                                                          typename St::T* fout, float* __restrict__ t_out,
+                                                          typename Uv::T* __restrict__ u_state,
+                                                          float* __restrict__ store_out, const int* __restrict__ list,
+                                                          int count, int cap, const float* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int cell = list[i];
+    const uint32_t m = mask[cell];
+    const int yz = P.ny * P.nz;
+    const int x = cell / yz, rem = cell - x * yz;
+    const int y = rem / P.nz, z = rem - y * P.nz;
+    const size_t N = (size_t)P.cells, Np = (size_t)P.plane;
+    const uint32_t links = m & kMaskLinks;
+    float g[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) g[q] = vals[(size_t)q * cap + i];
+    float temp = 0.0f;
+    if (P.thermal_on) {
+        temp = vals[(size_t)27 * cap + i];
+        t_out[cell] = temp;
+    }
+    CollideResult r = collide<St::kShifted>(P, g, temp);
+    if (!r.ok) reseed<St::kShifted>(P, C, cell, g, r);
+    if (P.shell_on) {
+        const int layer = C.shell[cell];
+        if (layer) shell_blend<St::kShifted>(P, C.shell_state, cell, layer, g, r);
+    }
+    if (P.sponge_on) sponge<St::kShifted>(P, x, y, z, g);
+    if (P.thermal_on) {
+        Uv::st(u_state + cell, r.ux);
+        Uv::st(u_state + N + cell, r.uy);
+        Uv::st(u_state + 2 * N + cell, r.uz);
+    }
+    if (Mode != 0 && P.face_store_faces) face_store(P, x, y, z, g, store_out);
+    if (Mode == 0) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) St::st(fout + q * Np + cell, g[q]);
+    } else if (Mode == 1) {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) St::st(fout + lat_opp(q) * Np + cell, g[q]);
+    } else {
+        const Nbr nb = make_nbr(P, x, y, z);
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const bool sp = (links >> lat_opp(q)) & 1u;  // target x + c_q solid or out of the domain
+            const int d = sp ? cell : cell + nb.off(lat_cx(q), lat_cy(q), lat_cz(q));
+            St::st(fout + (sp ? lat_opp(q) : q) * Np + d, g[q]);
+        }
     }
 }
 
@@ -422,8 +429,8 @@ __device__ __forceinline__ void cell_output_fast(const StepParams& P, const Cell
                 for (int k = 0; k < 27; ++k)
                     if (k == q) bnc = g[k];
                 const float v = layout == 0
-                                    ? face_link<St, false>(P, C.flags, C.ref_p, prev_ab, cell, x, y, z, q, bnc)
-                                    : face_link<St, true>(P, C.flags, C.ref_p, prev_aa, cell, x, y, z, q, bnc);
+                                    ? face_link<St, false>(P, C.flags, C.ref_p[cell], prev_ab, cell, x, y, z, q, bnc)
+                                    : face_link<St, true>(P, C.flags, C.ref_p[cell], prev_aa, cell, x, y, z, q, bnc);
 #pragma unroll
                 for (int k = 0; k < 27; ++k)
                     if (k == q) g[k] = v;
@@ -666,20 +673,21 @@ static void launch_step_m(const StepParams& P, const CellFields& C, const uint32
     using U = typename Uv::T;
     const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
     const int lf = Mode == 2 ? 0 : local_first;
-    const bool side = rare.count > 0;
-    if (side) {
-        // listed rare cells on the side stream, concurrent with the bulk kernel
-        (void)hipEventRecord(rare.fork, s);
-        (void)hipStreamWaitEvent(rare.side, rare.fork, 0);
-        const dim3 rgrid((unsigned)std::min(16384, (rare.count + 63) / 64));
-        hipLaunchKernelGGL((fast::rare_kernel<St, Uv, Mode>), rgrid, dim3(64), 0, rare.side, P, C, mask,
-                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf, rare.list,
-                           rare.count);
-        (void)hipEventRecord(rare.join, rare.side);
+    if (rare.count > 0) {
+        const dim3 ggrid((unsigned)((rare.count + 255) / 256), 28), ugrid((unsigned)((rare.count + 127) / 128));
+        // serial on the stream: a second stream beside the bulk kernel costs more in
+        // cross-stream synchronisation than the two short launches take
+        hipLaunchKernelGGL((fast::rare_gather_kernel<St, Uv, Mode>), ggrid, dim3(256), 0, s, P, C, mask,
+                           (const T*)fin, (const T*)fout, t_in, (const U*)u_state, store_in, lf, rare.list,
+                           rare.count, rare.cap, rare.vals);
+        hipLaunchKernelGGL((fast::rare_update_kernel<St, Uv, Mode>), ugrid, dim3(128), 0, s, P, C, mask, (T*)fout,
+                           t_out, (U*)u_state, store_out, rare.list, rare.count, rare.cap, (const float*)rare.vals);
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
+                           (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
+        return;
     }
     hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
                        (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
-    if (side) (void)hipStreamWaitEvent(s, rare.join, 0);
 }
 
 template <class St, class Uv, int StPol>

@@ -180,62 +180,65 @@ struct PrevPost {
         const int B = axis == 2 ? P.ny : P.nz;
         const int a = axis == 0 ? sy : sx;
         const int b = axis == 2 ? sy : sz;
-        return store[P.faces_dev[face].store_off + ((size_t)(layer * 9 + j) * A + a) * B + b];
+        // faces without a store (and layers beyond the third) load nothing: the caller
+        // computes every rule's candidates and selects afterwards
+        const bool ok = ((P.face_store_faces >> face) & 1) && layer >= 0 && layer <= 2;
+        return ok ? store[P.faces_dev[face].store_off + ((size_t)(layer * 9 + j) * A + a) * B + b] : 0.0f;
     }
 };
 
 // Incoming value for an out-of-domain link q at fluid cell (x,y,z); `bounce` is the
 // cell's own post-collision value of opp(q) (the reference's halfway bounce-back).
+// The candidate values of every rule are loaded unconditionally (clamped in-domain
+// positions) and the rule then selects, so the links of a cell do not form a chain
+// of dependent loads.
 template <class St, bool AA>
-__device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* flags, const float* ref_p, const PrevPost<St, AA>& prev,
-                           int cell, int x, int y, int z, int q, float bounce) {
+__device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* flags, float ref_p_cell,
+                                           const PrevPost<St, AA>& prev, int cell, int x, int y, int z, int q,
+                                           float bounce) {
     const int cx = lat_cx(q), cy = lat_cy(q), cz = lat_cz(q);
     const int sx = wrapc(x - cx, P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
     const int sy = wrapc(y - cy, P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);
     const int sz = wrapc(z - cz, P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]);
     const int face = oob_face(P, sx, sy, sz);
-    const FaceParams fc = P.faces_dev[face];
-    int kind = fc.hydro_kind;
-    if (kind == 0 || kind == 1) kind = P.convective_disabled && kind == 0 ? 2 : 6;
-    if (kind == 2) return bounce;
-    if (kind >= 3 && kind <= 5) {
-        // ref: jni.cpp:1518-1561
-        float rho = 1.0f + clampf(ref_p[cell], -0.03f, 0.03f);
-        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-        if (kind != 5) {
-            ux = fc.vel[0];
-            uy = fc.vel[1];
-            uz = fc.vel[2];
-        } else {
-            rho = 1.0f + clampf(fc.pressure, -0.03f, 0.03f);
-        }
-        rho = clampf(rho, 0.97f, 1.03f);
-        const float s2 = ux * ux + uy * uy + uz * uz;
-        if (s2 > P.kmax_speed * P.kmax_speed && s2 > 0.0f) {
-            const float sc = P.kmax_speed / sqrtf(s2);
-            ux *= sc;
-            uy *= sc;
-            uz *= sc;
-        }
-        return feq_s<St::kShifted>(q, rho, ux, uy, uz);
-    }
-    if (kind == 7) {
-        // symmetry: copy the clamped source (ref: jni.cpp:1582-1587)
-        const int ix = icl(sx, 0, P.nx - 1), iy = icl(sy, 0, P.ny - 1), iz = icl(sz, 0, P.nz - 1);
-        if (flags[(ix * P.ny + iy) * P.nz + iz] & kCellSolid) return bounce;
-        return prev.get(P, face, q, ix, iy, iz);
-    }
-    // convective outflow f1 + 0.15 (f1 - f2) (ref: jni.cpp:1494-1516)
+    const FaceParams& fc = P.faces_dev[face];
+    // symmetry source (ref: jni.cpp:1582-1587) and convective layers (jni.cpp:1494-1516)
+    const int ix = icl(sx, 0, P.nx - 1), iy = icl(sy, 0, P.ny - 1), iz = icl(sz, 0, P.nz - 1);
     const int dx = (sx < 0 || sx >= P.nx) ? cx : 0;
     const int dy = (sy < 0 || sy >= P.ny) ? cy : 0;
     const int dz = (sz < 0 || sz >= P.nz) ? cz : 0;
     const int x1 = icl(x + dx, 0, P.nx - 1), y1 = icl(y + dy, 0, P.ny - 1), z1 = icl(z + dz, 0, P.nz - 1);
-    if (flags[(x1 * P.ny + y1) * P.nz + z1] & kCellSolid) return bounce;
-    const float f1 = prev.get(P, face, q, x1, y1, z1);
     const int x2 = icl(x + 2 * dx, 0, P.nx - 1), y2 = icl(y + 2 * dy, 0, P.ny - 1), z2 = icl(z + 2 * dz, 0, P.nz - 1);
-    if (flags[(x2 * P.ny + y2) * P.nz + z2] & kCellSolid) return f1;
-    const float f2 = prev.get(P, face, q, x2, y2, z2);
-    return f1 + 0.15f * (f1 - f2);  // the w shift cancels: coefficients sum to one
+    const bool s0 = flags[(ix * P.ny + iy) * P.nz + iz] & kCellSolid;
+    const bool s1 = flags[(x1 * P.ny + y1) * P.nz + z1] & kCellSolid;
+    const bool s2 = flags[(x2 * P.ny + y2) * P.nz + z2] & kCellSolid;
+    const float p0 = prev.get(P, face, q, ix, iy, iz);
+    const float p1 = prev.get(P, face, q, x1, y1, z1);
+    const float p2 = prev.get(P, face, q, x2, y2, z2);
+    int kind = fc.hydro_kind;
+    if (kind == 0 || kind == 1) kind = P.convective_disabled && kind == 0 ? 2 : 6;
+    // velocity / pressure / inlet equilibrium (ref: jni.cpp:1518-1561)
+    float rho = 1.0f + clampf(ref_p_cell, -0.03f, 0.03f);
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    if (kind != 5) {
+        ux = fc.vel[0];
+        uy = fc.vel[1];
+        uz = fc.vel[2];
+    } else {
+        rho = 1.0f + clampf(fc.pressure, -0.03f, 0.03f);
+    }
+    rho = clampf(rho, 0.97f, 1.03f);
+    const float s2v = ux * ux + uy * uy + uz * uz;
+    if (s2v > P.kmax_speed * P.kmax_speed && s2v > 0.0f) {
+        const float sc = P.kmax_speed / sqrtf(s2v);
+        ux *= sc;
+        uy *= sc;
+        uz *= sc;
+    }
+    const float eq = feq_s<St::kShifted>(q, rho, ux, uy, uz);
+    const float sym = s0 ? bounce : p0;
+    const float conv = s1 ? bounce : (s2 ? p1 : p1 + 0.15f * (p1 - p2));  // w shift cancels: weights sum to one
+    return kind == 2 ? bounce : (kind >= 3 && kind <= 5) ? eq : (kind == 7 ? sym : conv);
 }
 
 __device__ __forceinline__ bool link_oob(const StepParams& P, int x, int y, int z, int q) {
@@ -246,20 +249,26 @@ __device__ __forceinline__ bool link_oob(const StepParams& P, int x, int y, int 
 }
 
 // ---------------------------------------------------------------- temperature
+// temperature at (x, y, z) for the stencil of a cell whose own value is `self`:
+// periodic wrap, Dirichlet / Neumann thermal faces, solid or adiabatic -> self.
+// Every load is issued unconditionally at a clamped in-domain index, so the 14
+// stencil points of a cell do not form a chain of dependent loads. Velocities are
+// clamped below one lattice unit, so a point lies at most one cell outside the
+// domain and one period brings a wrapped point back in.
 __device__ __forceinline__ float t_at(const StepParams& P, const float* T, const uint8_t* flags, int x, int y, int z,
                                       float self) {
-    if ((x < 0 && P.thermal_wrap_lo[0]) || (x >= P.nx && P.thermal_wrap_hi[0])) { x %= P.nx; if (x < 0) x += P.nx; }
-    if ((y < 0 && P.thermal_wrap_lo[1]) || (y >= P.ny && P.thermal_wrap_hi[1])) { y %= P.ny; if (y < 0) y += P.ny; }
-    if ((z < 0 && P.thermal_wrap_lo[2]) || (z >= P.nz && P.thermal_wrap_hi[2])) { z %= P.nz; if (z < 0) z += P.nz; }
-    if (x < 0 || y < 0 || z < 0 || x >= P.nx || y >= P.ny || z >= P.nz) {
-        const FaceParams fc = P.faces_dev[oob_face(P, x, y, z)];
-        if (fc.thermal_kind == 4) return clampf(fc.temperature, -1.0f, 1.0f);
-        if (fc.thermal_kind == 5) return clampf(self + fc.heat_flux, -1.0f, 1.0f);
-        return self;
-    }
-    const int c = (x * P.ny + y) * P.nz + z;
-    if (flags[c] & kCellSolid) return self;
-    return T[c];
+    if ((x < 0 && P.thermal_wrap_lo[0]) || (x >= P.nx && P.thermal_wrap_hi[0])) x = x < 0 ? x + P.nx : x - P.nx;
+    if ((y < 0 && P.thermal_wrap_lo[1]) || (y >= P.ny && P.thermal_wrap_hi[1])) y = y < 0 ? y + P.ny : y - P.ny;
+    if ((z < 0 && P.thermal_wrap_lo[2]) || (z >= P.nz && P.thermal_wrap_hi[2])) z = z < 0 ? z + P.nz : z - P.nz;
+    const bool inside = x >= 0 && y >= 0 && z >= 0 && x < P.nx && y < P.ny && z < P.nz;
+    const int c = (icl(x, 0, P.nx - 1) * P.ny + icl(y, 0, P.ny - 1)) * P.nz + icl(z, 0, P.nz - 1);
+    const bool solid = flags[c] & kCellSolid;
+    const float tv = T[c];
+    const FaceParams& fc = P.faces_dev[oob_face(P, x, y, z)];
+    const int kind = fc.thermal_kind;
+    const float face_t = kind == 4 ? clampf(fc.temperature, -1.0f, 1.0f)
+                                   : (kind == 5 ? clampf(self + fc.heat_flux, -1.0f, 1.0f) : self);
+    return inside ? (solid ? self : tv) : face_t;
 }
 
 // first-order semi-Lagrangian advection + explicit diffusion + source
@@ -419,6 +428,37 @@ struct CollideResult {
     float ux, uy, uz;
     bool ok;
 };
+
+// Boundary-reference shell (ref: native/src/aero_lbm_jni.cpp:4535-4552 for the
+// blend, 6449-6522 for the layer and target rules): a fluid cell `layer` cells deep
+// (1 = outermost) relaxes its density and velocity toward the stored reference with
+// weight clamp(relax * (1 - (layer - 1) / layers)^2, 0, 0.5). The populations move by
+// the difference of the two equilibria, so the blended density and velocity are the
+// cell's new conserved moments and the other moments keep their post-collision values.
+template <bool Shifted>
+__device__ __forceinline__ void shell_blend(const StepParams& P, const uint16_t* __restrict__ state, int cell,
+                                            int layer, float g[27], CollideResult& r) {
+    const uint2 raw = *reinterpret_cast<const uint2*>(state + (size_t)cell * 4);
+    const float tux = (float)__builtin_bit_cast(_Float16, (uint16_t)(raw.x & 0xFFFFu));
+    const float tuy = (float)__builtin_bit_cast(_Float16, (uint16_t)(raw.x >> 16));
+    const float tuz = (float)__builtin_bit_cast(_Float16, (uint16_t)(raw.y & 0xFFFFu));
+    const float tp = (float)__builtin_bit_cast(_Float16, (uint16_t)(raw.y >> 16));
+    const float shell_t = 1.0f - (float)(layer - 1) / fmaxf(1.0f, (float)P.shell_layers);
+    const float alpha = clampf(P.shell_relax * shell_t * shell_t, 0.0f, 0.5f);
+    float rho = Shifted ? 1.0f : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) rho += g[q];
+    const float trho = clampf(1.0f + clampf(tp, -0.03f, 0.03f), 0.97f, 1.03f);
+    const float nrho = rho + alpha * (trho - rho);
+    const float nux = r.ux + alpha * (tux - r.ux);
+    const float nuy = r.uy + alpha * (tuy - r.uy);
+    const float nuz = r.uz + alpha * (tuz - r.uz);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) g[q] += feq_s<false>(q, nrho, nux, nuy, nuz) - feq_s<false>(q, rho, r.ux, r.uy, r.uz);
+    r.ux = nux;
+    r.uy = nuy;
+    r.uz = nuz;
+}
 
 // g: populations in stored form (f - w when Shifted), overwritten with the post-collision values
 template <bool Shifted>

@@ -37,8 +37,8 @@ void launch_fast_rare_list(const StepParams& P, const uint32_t* mask, int* list,
 struct FastRare {
     const int* list;   // fluid cells marked kMaskRare
     int count;
-    hipStream_t side;  // stream of the listed rare cells (runs beside the bulk kernel)
-    hipEvent_t fork, join;
+    int cap;           // capacity of vals per value (>= count)
+    float* vals;       // 28 x cap: incoming populations and new temperature of the listed cells
 };
 void launch_fast_init(const StepParams& P, const CellFields& C, FastStorage st, void* f, float* T, float* T_other, void* u_state,
                       float t_init, hipStream_t s);

@@ -16,6 +16,8 @@ struct CellFields {
     const float* ref_u;    // payload channels 5-7, three planes
     const float* ref_t;    // payload channel 10 (clamped)
     const float* src;      // payload channel 9 (clamped)
+    const uint8_t* shell;         // boundary-reference shell layer per cell (0: outside)
+    const uint16_t* shell_state;  // FP16 (u_x, u_y, u_z, p) targets of the shell, 4 per cell
 };
 
 // ---- payload ingest and cell classification (d3q27_common.hip) ----

@@ -55,6 +55,9 @@ struct StepParams {
     int source_on;            // internal thermal source channel active
     int convective_disabled;  // DISABLE_CONVECTIVE_OUTFLOW
     int thermal_on;           // temperature field can be non-zero
+    int shell_on;             // boundary-reference shell present (fast paths)
+    int shell_layers;         // shell depth in cells
+    float shell_relax;        // shell relaxation at the outermost layer
     int in_ch;
     float tau_shear_base, tau_normal_base, nu_base;
     float alpha, beta, sgs_c2, nut_cap_ratio;

@@ -63,10 +63,6 @@ bool ensure_device() {
     s.device_arch = prop.gcnArchName;
     s.cu_count = prop.multiProcessorCount;
     if (!hip_ok(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
-    if (!hip_ok(hipStreamCreateWithFlags(&s.side_stream, hipStreamNonBlocking), "hipStreamCreate")) return false;
-    if (!hip_ok(hipEventCreateWithFlags(&s.ev_fork, hipEventDisableTiming), "hipEventCreate") ||
-        !hip_ok(hipEventCreateWithFlags(&s.ev_join, hipEventDisableTiming), "hipEventCreate"))
-        return false;
     s.device_ready = true;
     return true;
 }
@@ -345,6 +341,9 @@ bool prepare_params(Context& c, StepParams& P) {
     }
     P.faces_dev = c.faces_dev;
     P.thermal_on = c.thermal_on ? 1 : 0;
+    P.shell_on = c.shell_on ? 1 : 0;
+    P.shell_layers = c.shell_on ? c.shell_layers : 0;
+    P.shell_relax = c.shell_on ? c.shell_relax : 0.0f;
     // bulk cells index their neighbours without bounds checks, so the classes must
     // have been built with the periodic faces in force now
     if (c.has_payload) {
@@ -452,7 +451,9 @@ Context* get_context(long long key, int nx, int ny, int nz, bool create) {
     return c;
 }
 
-CellFields cell_fields(const Context& c) { return CellFields{c.flags, c.ref_p, c.ref_u, c.ref_t, c.src}; }
+CellFields cell_fields(const Context& c) {
+    return CellFields{c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, c.shell, c.shell_state};
+}
 
 static bool host_any_nonzero(const float* p, size_t n, size_t stride, size_t off) {
     for (size_t i = 0; i < n; ++i)

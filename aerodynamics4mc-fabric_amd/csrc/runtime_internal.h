@@ -40,6 +40,8 @@ struct Context {
     int* rare_counter = nullptr;
     int* rare_scratch = nullptr;
     int rare_count = 0;
+    int rare_cap = 0;              // capacity of rare_vals per value
+    float* rare_vals = nullptr;    // 28 x rare_cap values of the listed cells
     int mask_sig[kMaskSigLen] = {};  // parameters the link mask was built with
     bool mask_valid = false;
     // device buffers
@@ -55,6 +57,12 @@ struct Context {
     float* face_store[2] = {nullptr, nullptr};  // A-A: near-face post-collision values per parity
     size_t face_store_floats = 0;
     float* payload_dev = nullptr;
+    // boundary-reference shell of the fast paths
+    uint8_t* shell = nullptr;
+    uint16_t* shell_state = nullptr;
+    bool shell_on = false;
+    int shell_layers = 0;
+    float shell_relax = 0.0f;
     float* out_dev = nullptr;
     float* sample_dev = nullptr;
     FaceParams* faces_dev = nullptr;   // device copy of the face table
@@ -101,8 +109,6 @@ struct RuntimeState {
     int cu_count = 0;
     bool device_ready = false;
     hipStream_t stream = nullptr;
-    hipStream_t side_stream = nullptr;  // second stream (default priority) for concurrent work
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::unordered_map<long long, std::unique_ptr<Context>> contexts;
     Timing timing;
 };

@@ -95,6 +95,19 @@ bool storage_rebuild_masks(Context& c, const StepParams& P) {
         !hip_ok(hipStreamSynchronize(st), "sync"))
         return false;
     c.rare_count = count;
+    if (count > c.rare_cap) {
+        // per-step values of the listed cells: 27 incoming populations and the temperature
+        if (c.rare_vals) {
+            (void)hipFree(c.rare_vals);
+            for (auto& p : c.allocations)
+                if (p == c.rare_vals) p = nullptr;
+            c.rare_vals = nullptr;
+            c.rare_cap = 0;
+        }
+        const int cap = (count + 255) & ~255;
+        if (!c.alloc((void**)&c.rare_vals, (size_t)28 * cap * sizeof(float))) return false;
+        c.rare_cap = cap;
+    }
     mask_signature(P, c.mask_sig);
     c.mask_valid = true;
     return true;
@@ -145,7 +158,7 @@ bool storage_step(Context& c, const StepParams& P) {
         c.aa_layout = 3;
     }
     if (c.storage == Storage::F32Fast) {
-        const FastRare rare{c.rare_list, c.rare_count, rt().side_stream, rt().ev_fork, rt().ev_join};
+        const FastRare rare{c.rare_list, c.rare_count, c.rare_cap, c.rare_vals};
         launch_fast_step(P, C, fk, c.mask, c.f32[c.f_cur], c.f32[1 - c.f_cur], c.T[c.t_cur], c.T[1 - c.t_cur],
                          c.u_state, nullptr, nullptr, 0, c.post_collision ? 0 : 1, rare, st);
         c.f_cur ^= 1;
@@ -153,7 +166,7 @@ bool storage_step(Context& c, const StepParams& P) {
     } else {
         const bool even = c.aa_layout == 3 || c.aa_layout == 2;
         const int mode = even ? 1 : 2;
-        const FastRare rare{c.rare_list, c.rare_count, rt().side_stream, rt().ev_fork, rt().ev_join};
+        const FastRare rare{c.rare_list, c.rare_count, c.rare_cap, c.rare_vals};
         launch_fast_step(P, C, fk, c.mask, aa_buffer(c), aa_buffer(c), c.T[c.t_cur], c.T[1 - c.t_cur], c.u_state,
                          c.face_store[c.store_cur], c.face_store[1 - c.store_cur], mode, c.aa_layout == 3 ? 1 : 0,
                          rare, st);

@@ -260,6 +260,82 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_get_timing_snapshot(AeroLbmTimingSnapshot* out
  * stored (lattice units), summed in FP64 on the device in a fixed order. ---- */
 AERO_LBM_CAPI_EXPORT int aero_lbm_get_conservation_rect(int nx, int ny, int nz, long long context_key, double* out4);
 
+/* ---- boundary-reference shell of the in-place paths
+ * (ref: native/include/aero_lbm_capi.h:183-194, native/src/aero_lbm_jni.cpp:9962-10030).
+ * flow_state: cells x flow_channels floats (u_x, u_y, u_z, p first); face_mask bits
+ * x-, x+, y-, y+, z-, z+; fluid cells within shell_layers of a flagged face relax
+ * their density and velocity toward the (FP16-rounded) reference each step. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_set_d3q27_f16_boundary_shell_rect(
+    int nx, int ny, int nz, long long context_key, const float* flow_state, int flow_channels,
+    int face_mask, int shell_layers, float relaxation);
+AERO_LBM_CAPI_EXPORT int aero_lbm_clear_d3q27_f16_boundary_shell_rect(long long context_key);
+
+/* ---- native brick runtime (ref: native/include/aero_lbm_capi.h:303-696,
+ * native/src/aero_lbm_simulation_bridge.cpp:4562-5535). One service (key 1), worlds
+ * of cubic bricks keyed by the caller, bricks keyed by integer brick coordinates.
+ * Flow state: 4 floats per cell (lattice u_x, u_y, u_z, p); packed atlas: 4 int16
+ * per sampled cell (velocity in m/s over +-5.6, pressure over +-0.03). Counts and
+ * copies return -1 on error; the rest return 1/0 with the reason in
+ * aero_lbm_simulation_last_error(). World delta type 10 (client source patches) is
+ * accepted and ignored. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_create_service(long long* out_service_key);
+AERO_LBM_CAPI_EXPORT void aero_lbm_simulation_release_service(long long service_key);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_focus(long long service_key, int block_x, int block_y, int block_z,
+                                                       int radius_blocks);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_submit_world_deltas(long long service_key,
+                                                                 const AeroLbmWorldDelta* deltas, int delta_count);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_ensure_brick_world_runtime(long long service_key, long long world_key,
+                                                                        int brick_size, float dx_meters,
+                                                                        float dt_seconds);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_brick_world_solver_mode(long long service_key, long long world_key,
+                                                                         int solver_mode);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_brick_world_active_hints(long long service_key, long long world_key,
+                                                                          int brick_size, const int* brick_coords,
+                                                                          int brick_count);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_brick_world_exact_active_hints(long long service_key,
+                                                                                long long world_key, int brick_size,
+                                                                                const int* brick_coords,
+                                                                                int brick_count);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_get_brick_world_runtime_status(long long service_key, long long world_key,
+                                                                            int* out_status, int status_count);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_step_brick_world_runtime(long long service_key, long long world_key,
+                                                                      int step_count);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_get_brick_world_resident_brick_count(long long service_key,
+                                                                                  long long world_key);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_resident_bricks(long long service_key,
+                                                                              long long world_key, int* out_coords,
+                                                                              int brick_capacity);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_get_brick_world_active_brick_count(long long service_key,
+                                                                                long long world_key);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_active_bricks(long long service_key, long long world_key,
+                                                                            int* out_coords, int brick_capacity);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_upload_brick_world_static_brick(
+    long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
+    const uint8_t* obstacle, const uint8_t* surface_kind, const uint16_t* open_face_mask,
+    const float* emitter_power_watts, const uint8_t* face_sky_exposure, const uint8_t* face_direct_exposure);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_upload_brick_world_static_brick_with_sources(
+    long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
+    const uint8_t* obstacle, const uint8_t* surface_kind, const uint16_t* open_face_mask,
+    const float* emitter_power_watts, const uint8_t* source_fan_dir, const float* source_emitter_power_watts,
+    const uint8_t* face_sky_exposure, const uint8_t* face_direct_exposure);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_queue_brick_world_static_brick_upload(
+    long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
+    const uint8_t* obstacle, const uint8_t* surface_kind, const uint16_t* open_face_mask,
+    const float* emitter_power_watts, const uint8_t* face_sky_exposure, const uint8_t* face_direct_exposure);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_upload_brick_world_dynamic_brick(
+    long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
+    const float* flow_state, const float* air_temperature, const float* surface_temperature);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_upload_brick_world_boundary_reference_brick(
+    long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
+    const float* flow_state, const float* air_temperature, const float* surface_temperature);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_dynamic_brick(
+    long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
+    float* out_flow_state, float* out_air_temperature, float* out_surface_temperature);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_packed_flow_atlas(
+    long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
+    int sample_stride, int16_t* out_packed_flow, int packed_value_count);
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_last_error(void);
+
 /* ---- mesoscale D3Q19 step (ref: native/include/aero_lbm_capi.h:286-301) ----
  * forcing: cells x 25 floats, state: cells x 6 floats (ambient, deep ground,
  * surface, wind x, y, z), cell = (x*ny + y)*nz + z, y vertical. */

@@ -46,6 +46,12 @@ typedef struct SFX(Ctx) {
     R *T, *T_next;
     R w[27];
     R kmax_speed;
+    /* boundary-reference shell (jni.cpp:4535-4552, 6449-6522): per-cell layer, FP16-
+     * rounded targets (u_x, u_y, u_z, p) decoded to R, depth and outer relaxation */
+    unsigned char* shell_layer;
+    R* shell_target;
+    int shell_layers;
+    R shell_relax;
 } SFX(Ctx);
 
 /* hydro_core.h:130-205 (tables): q = (ix*3+iy)*3+iz, c = ix-1, ..., w = product of
@@ -435,6 +441,21 @@ static void SFX(collide)(SFX(Ctx) * c) {
             const R s = SFX(guo)(c, q, ux, uy, uz, F[0], F[1], F[2], om_off);
             c->f_post[(size_t)q * N + cell] = fc[q] + s;
         }
+        if (c->shell_layer && c->shell_layer[cell]) {
+            /* the density and velocity of the post-collision state relax toward the
+             * reference by alpha = clamp(relax * (1 - (layer-1)/layers)^2, 0, 0.5);
+             * the populations move by the difference of the two equilibria */
+            const R* tg = c->shell_target + 4 * cell;
+            const R st = K(1.0) - (R)(c->shell_layer[cell] - 1) / SFX(rmax)(K(1.0), (R)c->shell_layers);
+            const R al = SFX(clampr)(c->shell_relax * st * st, K(0.0), K(0.5));
+            R rp = K(0.0);
+            for (int q = 0; q < 27; ++q) rp += c->f_post[(size_t)q * N + cell];
+            const R trho = SFX(clampr)(K(1.0) + SFX(clampr)(tg[3], K(-0.03), K(0.03)), K(0.97), K(1.03));
+            const R nr = rp + al * (trho - rp);
+            const R nx = ux + al * (tg[0] - ux), ny = uy + al * (tg[1] - uy), nz = uz + al * (tg[2] - uz);
+            for (int q = 0; q < 27; ++q)
+                c->f_post[(size_t)q * N + cell] += SFX(feq)(c, q, nr, nx, ny, nz) - SFX(feq)(c, q, rp, ux, uy, uz);
+        }
         {
             const int yz = c->ny * c->nz;
             const int x = (int)(cell / (size_t)yz);
@@ -721,7 +742,7 @@ ORACLE_API void SFX(oracle_destroy)(void* h) {
     if (!c) return;
     free(c->f); free(c->f_post); free(c->rho); free(c->ux); free(c->uy); free(c->uz);
     free(c->ref_ux); free(c->ref_uy); free(c->ref_uz); free(c->ref_p); free(c->ref_T); free(c->src);
-    free(c->obstacle); free(c->T); free(c->T_next); free(c);
+    free(c->obstacle); free(c->T); free(c->T_next); free(c->shell_layer); free(c->shell_target); free(c);
 }
 
 /* jni.cpp:9582-9601 assign_temperature_state */
@@ -838,3 +859,21 @@ ORACLE_API R SFX(oracle_feq)(int q, R rho, R ux, R uy, R uz) {
     return SFX(feq)(&c, q, rho, ux, uy, uz);
 }
 ORACLE_API void SFX(oracle_weights)(R* w) { SFX(init_weights)(w); }
+
+/* boundary-reference shell: layer per cell (0 = none) and 4 decoded targets per cell;
+ * layers <= 0 removes the shell */
+ORACLE_API void SFX(oracle_set_shell)(void* h, const unsigned char* layer, const float* targets, int layers,
+                                      float relax) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    free(c->shell_layer);
+    free(c->shell_target);
+    c->shell_layer = NULL;
+    c->shell_target = NULL;
+    if (layers <= 0) return;
+    c->shell_layer = (unsigned char*)malloc(c->cells);
+    c->shell_target = (R*)malloc(sizeof(R) * 4 * c->cells);
+    memcpy(c->shell_layer, layer, c->cells);
+    for (size_t i = 0; i < 4 * c->cells; ++i) c->shell_target[i] = (R)targets[i];
+    c->shell_layers = layers;
+    c->shell_relax = (R)relax;
+}

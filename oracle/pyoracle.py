@@ -53,6 +53,7 @@ def load():
         getattr(lib, p + "oracle_feq").argtypes = [c_int, real, real, real, real]
         getattr(lib, p + "oracle_feq").restype = real
         getattr(lib, p + "oracle_weights").argtypes = [R]
+        getattr(lib, p + "oracle_set_shell").argtypes = [c_void_p, c_void_p, POINTER(c_float), c_int, c_float]
     lib.oracle_sanitize_config.argtypes = [c_void_p]
     lib.oracle_set_threads.argtypes = [c_int]
     lib.oracle_set_threads(min(16, os.cpu_count() or 1))
@@ -151,6 +152,14 @@ class Oracle:
         out = np.zeros(6, dtype=self.real)
         self._f("oracle_get_params")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
         return dict(zip(("tau_shear", "tau_normal", "nu", "alpha", "beta", "sgs_c2"), out.tolist()))
+
+    def set_shell(self, layer, targets, layers, relax):
+        """Boundary-reference shell: layer per cell (uint8, 0 = none), decoded targets
+        (cells x 4: u_x, u_y, u_z, p), depth and outer relaxation; layers <= 0 removes it."""
+        self._layer = np.ascontiguousarray(layer, dtype=np.uint8).reshape(-1)
+        self._targets = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        self._f("oracle_set_shell")(self.h, self._layer.ctypes.data_as(c_void_p), _fp(self._targets), int(layers),
+                                    float(relax))
 
     def collide_cell(self, f, temperature=0.0):
         """Post-collision populations of one fluid cell (context must be 1x1x1)."""

@@ -103,6 +103,15 @@ def config3_full(steps=1000, n=128):
                    "plane_sums": [[float("%.12g" % v) for v in row] for row in full["plane_sums"]],
                    "flow_sha256": full["flow_sha256"], "seconds": time.time() - t0}
     print(f"config3_full fp64: {time.time() - t0:.1f} s", flush=True)
+    # the FP32 oracle's own distance from FP64 at the probes (another ~15 min)
+    o32 = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=32)
+    o32.set_temperature(t)
+    o32.run(pk, steps)
+    ref = np.array(full["probe_flow"])[:, :3]
+    got = o32.flow().reshape(-1, 4)[idx, :3].astype(np.float64)
+    fluid = solid.reshape(-1)[idx] == 0
+    out["fp32_oracle_err"] = float(np.max(np.abs(got[fluid] - ref[fluid]))) / 0.05
+    out["fp32_oracle_note"] = "max |u_fp32_oracle - u_fp64_oracle| / U_in over the fluid probes after 1000 steps"
     return out
 
 
