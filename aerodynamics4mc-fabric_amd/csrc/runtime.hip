@@ -692,6 +692,16 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_context_realtime_cached_initialized(long long 
     return (c && c->has_payload && c->initialized) ? 1 : 0;
 }
 
+// A stepped context is read with the parameters of its last step (the face rules
+// its stored state was streamed with), whatever the global configuration is now.
+static bool readout_params(Context& c, StepParams& P) {
+    if (c.params_valid) {
+        P = c.params_host;
+        return true;
+    }
+    return prepare_params(c, P);
+}
+
 static int readout_impl(int nx, int ny, int nz, long long key, int stride, float* out, int value_count) {
     std::lock_guard<std::mutex> lk(rt().mu);
     clear_error();
@@ -707,7 +717,7 @@ static int readout_impl(int nx, int ny, int nz, long long key, int stride, float
         return 0;
     }
     StepParams P;
-    if (!prepare_params(*c, P)) return 0;
+    if (!readout_params(*c, P)) return 0;
     if (!storage_readout(*c, P, stride, 4, 1.0f, c->out_dev)) return 0;
     const bool ok = hip_ok(hipMemcpyAsync(out, c->out_dev, (size_t)value_count * 4, hipMemcpyDeviceToHost,
                                           rt().stream), "readback") &&
@@ -735,7 +745,7 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_sample_flow_point_rect(int nx, int ny, int nz,
     Context* c = get_context(key, nx, ny, nz, false);
     if (!c || !c->initialized) return 0;
     StepParams P;
-    if (!prepare_params(*c, P)) return 0;
+    if (!readout_params(*c, P)) return 0;
     if (!storage_sample(*c, P, x, y, z, c->sample_dev)) return 0;
     return (hip_ok(hipMemcpyAsync(out, c->sample_dev, 16, hipMemcpyDeviceToHost, rt().stream), "readback") &&
             hip_ok(hipStreamSynchronize(rt().stream), "sync"))

@@ -1,0 +1,234 @@
+"""GPU brick runtime through the C ABI.
+
+* A brick world steps each brick through this library's own context API: its
+  first step builds the 11-channel packet, configures the compact boundary from
+  the packet's mean velocity and steps with the packet; later steps are cached.
+  The same sequence driven by hand from bricks_oracle's packet, mean velocity and
+  seed rules must give bit-identical flow fields.
+* The boundary-reference shell blend against the FP32 oracle with the same shell
+  (FP32 in-place storage; the two collision codes differ by rounding only).
+* The packed atlas of a live brick equals the oracle's quantisation of the
+  context's own atlas extraction.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import aero_lbm as A
+import bricks_oracle as B
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compact_config(v, n):
+    """Velocity inlet x-, convective outflow x+, symmetry elsewhere (what the runtime sets)."""
+    cfg = A.default_config()
+    cfg.enabled = 1
+    cfg.preset = 0
+    cfg.reference_length = float(n)
+    for k in range(3):
+        cfg.initial_velocity[k] = v[k]
+    for f in (cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.z_min, cfg.z_max):
+        ctypes.memset(ctypes.addressof(f), 0, ctypes.sizeof(f))
+        f.hydrodynamic_kind = A.HYDRO_SYMMETRY
+        f.thermal_kind = 0
+    cfg.x_min.hydrodynamic_kind = A.HYDRO_VELOCITY
+    for k in range(3):
+        cfg.x_min.velocity[k] = v[k]
+    cfg.x_max.hydrodynamic_kind = A.HYDRO_CONVECTIVE
+    return cfg
+
+
+class Manual:
+    """A standalone context driven the way the runtime drives a brick."""
+
+    def __init__(self, lib, n, key, mode, packet):
+        self.lib, self.n = lib, n
+        self.v, ok = B.packet_mean_velocity(packet)
+        assert ok
+        lib.aero_lbm_set_realtime_solver_mode(mode)
+        self.ctx = A.Context(n, n, n, key=key)
+        self._cfg()
+        self.ctx.step(packet)
+        lib.aero_lbm_benchmark_reset_config()
+        lib.aero_lbm_set_realtime_solver_mode(0)
+
+    def _cfg(self):
+        A.check(self.lib.aero_lbm_benchmark_set_config(ctypes.byref(compact_config(self.v, self.n))), "config")
+
+    def step(self, k=1):
+        for _ in range(k):
+            self._cfg()
+            self.ctx.step(None)
+            self.lib.aero_lbm_benchmark_reset_config()
+
+
+@pytest.fixture(scope="module")
+def lib():
+    lib = A.load()
+    A.check(lib.aero_lbm_set_solver_overrides(ctypes.byref(W.reference_overrides(A.KERNEL_FAST))), "overrides")
+    return lib
+
+
+def brick_static(n, seed):
+    ob = W.solid_boxes(n, n, n, 0.08, seed=seed, edge=(2, 4), keep_clear_x=2).reshape(-1).astype(np.uint8)
+    return ob
+
+
+def constrained(ob, flow, air):
+    f, a, s = flow.copy(), air.copy(), np.zeros_like(air)
+    B.apply_constraints(ob, np.zeros(ob.size, np.uint16), np.zeros(ob.size, np.uint8), np.zeros(ob.size, np.float32),
+                        f, a, s)
+    return f, a
+
+
+@pytest.mark.parametrize("mode", [A.SOLVER_D3Q27_FP16_INPLACE, A.SOLVER_D3Q27_BF16_INPLACE])
+def test_single_brick_matches_manual_driving(lib, mode):
+    n = 16
+    ob = brick_static(n, 3)
+    rng = np.random.default_rng(1)
+    flow = np.tile(np.array([0.04, 0.0, 0.0, 0.0], np.float32), n ** 3)
+    flow[0::4] += rng.uniform(-1e-3, 1e-3, n ** 3).astype(np.float32)
+    air = np.zeros(n ** 3, np.float32)
+    w = A.BrickWorld(world=11, size=n, mode=mode)
+    try:
+        w.hints([(0, 0, 0)], exact=True)
+        w.upload_static((0, 0, 0), ob)
+        w.upload_dynamic((0, 0, 0), flow, air)
+        log(f"world step 1 mode={mode}")
+        w.step(1)
+        f, a = constrained(ob, flow, air)
+        pk = B.build_packet(n, ob, np.zeros(n ** 3, np.uint8), np.zeros(n ** 3, np.float32), f, a, [True] * 6)
+        m = Manual(lib, n, 4001, mode, pk)
+        got, _, _ = w.dynamic((0, 0, 0))
+        assert np.array_equal(got, m.ctx.flow())
+        w.step(6)
+        m.step(6)
+        got, air_got, _ = w.dynamic((0, 0, 0))
+        assert np.array_equal(got, m.ctx.flow())
+        assert np.array_equal(air_got, m.ctx.temperature())
+        st = w.status()
+        assert st[3] == 1 and st[4] == 0 and st[6] == 0 and st[7] >= 7
+        # packed atlas of the live brick: quantised context atlas
+        for stride in (1, 4):
+            sn = (n + stride - 1) // stride
+            raw = np.zeros(sn ** 3 * 4, np.float32)
+            A.check(lib.aero_lbm_extract_flow_atlas_rect(n, n, n, m.ctx.key, stride, A.fptr(raw), raw.size), "atlas")
+            q = np.empty((sn ** 3, 4), np.int16)
+            r4 = raw.reshape(-1, 4)
+            q[:, :3] = B.quantize((r4[:, :3] * np.float32(1.0 / 0.05)).astype(np.float32), 5.6)
+            q[:, 3] = B.quantize(r4[:, 3], 0.03)
+            assert np.array_equal(w.atlas((0, 0, 0), stride), q.reshape(-1)), stride
+        m.ctx.release()
+    finally:
+        w.release()
+
+
+def test_boundary_reference_shell_and_seeding(lib):
+    """Brick A carries flow and a boundary reference; brick B (x+ neighbour) has no
+    state and is seeded from A's mean with A's adjacent layer as its x- ghost."""
+    n = 16
+    mode = A.SOLVER_D3Q27_FP16_INPLACE
+    ob_a, ob_b = brick_static(n, 5), brick_static(n, 6)
+    flow = np.tile(np.array([0.03, 0.005, 0.0, 0.001], np.float32), n ** 3)
+    air = np.zeros(n ** 3, np.float32)
+    bref = np.tile(np.array([0.05, 0.0, -0.01, 0.002], np.float32), n ** 3)
+    w = A.BrickWorld(world=12, size=n, mode=mode)
+    try:
+        w.hints([(0, 0, 0), (1, 0, 0)], exact=True)
+        w.upload_static((0, 0, 0), ob_a)
+        w.upload_static((1, 0, 0), ob_b)
+        w.upload_dynamic((0, 0, 0), flow, air)
+        w.upload_dynamic((0, 0, 0), bref, air, reference=True)
+        log("world with shell: step 1")
+        w.step(1)
+        # A: its x+ face touches B, the other five are exposed and read the reference
+        fa, aa = constrained(ob_a, flow, air)
+        exposed = [True, False, True, True, True, True]
+        pk_a = B.build_packet(n, ob_a, np.zeros(n ** 3, np.uint8), np.zeros(n ** 3, np.float32), fa, aa, exposed,
+                              bref, air)
+        ma = Manual(lib, n, 4101, mode, pk_a)
+        A.check(lib.aero_lbm_set_d3q27_f16_boundary_shell_rect(n, n, n, ma.ctx.key, A.fptr(bref), 4, 0b111101, 8,
+                                                               0.1), "shell")
+        # B: seeded with A's (constrained) mean, then A's x = n-1 layer as its x = 0 ghost
+        seed = B.seed_from_neighbours([(fa, aa, np.zeros(n ** 3, np.float32))])
+        fb = np.tile(seed[0], n ** 3)
+        ab = np.full(n ** 3, seed[1], np.float32)
+        sb = np.full(n ** 3, seed[2], np.float32)
+        B.copy_ghost(n, fa, aa, np.zeros(n ** 3, np.float32), 0, False, fb, ab, sb)
+        fb, ab = constrained(ob_b, fb, ab)
+        pk_b = B.build_packet(n, ob_b, np.zeros(n ** 3, np.uint8), np.zeros(n ** 3, np.float32), fb, ab,
+                              [False] * 6)
+        mb = Manual(lib, n, 4102, mode, pk_b)
+        got_a, _, _ = w.dynamic((0, 0, 0))
+        got_b, _, _ = w.dynamic((1, 0, 0))
+        assert np.array_equal(got_a, ma.ctx.flow())
+        assert np.array_equal(got_b, mb.ctx.flow())
+        w.step(5)
+        ma.step(5)
+        mb.step(5)
+        got_a, _, _ = w.dynamic((0, 0, 0))
+        got_b, _, _ = w.dynamic((1, 0, 0))
+        assert np.array_equal(got_a, ma.ctx.flow())
+        assert np.array_equal(got_b, mb.ctx.flow())
+        ma.ctx.release()
+        mb.ctx.release()
+    finally:
+        w.release()
+
+
+def shell_targets(bref, n):
+    """FP16-rounded (ties away) shell targets after the speed and pressure clamps."""
+    f = bref.reshape(-1, 4).astype(np.float32).copy()
+    kmax = np.float32(0.57735026919 * 0.6)
+    s2 = (f[:, :3] ** 2).sum(1)
+    over = s2 > kmax * kmax
+    f[over, :3] *= (kmax / np.sqrt(s2[over]))[:, None]
+    f[:, 3] = np.clip(f[:, 3], -0.03, 0.03)
+    bits = np.array([[B.half_bits(v) for v in row] for row in f], np.uint16)
+    return bits.view(np.float16).astype(np.float32)
+
+
+def test_shell_blend_vs_oracle(lib, oracle_mod):
+    n, steps = 12, 30
+    solid = W.solid_boxes(n, n, n, 0.08, seed=9, edge=(2, 3), keep_clear_x=2)
+    pk = W.make_packet(solid, u0=(0.02, 0.0, 0.0), perturb=2e-3, seed_u=4)
+    bref = np.tile(np.array([0.06, -0.02, 0.01, 0.004], np.float32), n ** 3)
+    bref[0::4] += np.linspace(0, 0.01, n ** 3, dtype=np.float32)
+    v, _ = B.packet_mean_velocity(pk.reshape(-1, 11))
+    cfg = compact_config(v, n)
+    ov = W.reference_overrides(A.KERNEL_FAST)
+    A.check(lib.aero_lbm_init_rect(n, n, n, 11, 4), "init")
+    A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "config")
+    lib.aero_lbm_set_realtime_solver_mode(A.SOLVER_D3Q27_FP32_INPLACE)
+    ctx = A.Context(n, n, n, key=4201)
+    ctx.step(pk)
+    mask, layers, relax = 0b110101, 5, 0.3
+    A.check(lib.aero_lbm_set_d3q27_f16_boundary_shell_rect(n, n, n, ctx.key, A.fptr(bref), 4, mask, layers, relax),
+            "shell")
+    o = oracle_mod.Oracle(n, n, n, cfg, overrides=ov, precision=32)
+    o.step(pk)
+    o.set_shell(B.shell_layers(n, mask, layers, solid.reshape(-1)), shell_targets(bref, n), layers, relax)
+    for _ in range(steps - 1):
+        ctx.step(None)
+        o.step(None)
+    got = ctx.flow().reshape(-1, 4)
+    ref = o.flow().reshape(-1, 4)
+    fluid = solid.reshape(-1) == 0
+    err = float(np.max(np.abs(got[fluid, :3] - ref[fluid, :3])))
+    log(f"shell blend FP32 in place vs FP32 oracle: max velocity difference {err:.3e}")
+    assert err < 1e-5
+    # the shell acts: without it the run ends elsewhere
+    o2 = oracle_mod.Oracle(n, n, n, cfg, overrides=ov, precision=32)
+    o2.run(pk, steps)
+    assert float(np.max(np.abs(o2.flow().reshape(-1, 4)[fluid, :3] - ref[fluid, :3]))) > 1e-3
+    ctx.release()
+    lib.aero_lbm_set_realtime_solver_mode(0)
