@@ -206,6 +206,12 @@ PROTOTYPES = {
     "aero_lbm_simulation_copy_brick_world_packed_flow_atlas": (c_int, [c_longlong, c_longlong, c_int, c_int, c_int,
                                                                        c_int, c_int, _I16, c_int]),
     "aero_lbm_simulation_last_error": (c_char_p, []),
+    "aero_lbm_batch_create": (c_int, [POINTER(c_longlong), c_int, c_int, c_int, _F, c_int, POINTER(c_longlong)]),
+    "aero_lbm_batch_run": (c_int, [c_longlong, c_int, c_int, _F]),
+    "aero_lbm_batch_run_per_brick": (c_int, [c_longlong, c_int, _F]),
+    "aero_lbm_batch_copy_atlas": (c_int, [c_longlong, c_int, _I16, c_int]),
+    "aero_lbm_batch_info": (c_char_p, [c_longlong]),
+    "aero_lbm_batch_release": (None, [c_longlong]),
 }
 
 
@@ -461,3 +467,40 @@ class BrickWorld:
 
     def release(self):
         self.lib.aero_lbm_simulation_release_service(self.service)
+
+
+class Batch:
+    """Contexts stepped together (aero_lbm_batch_*)."""
+
+    def __init__(self, keys, steps_per_publish=4, stride=4, scales=None, autotune=False):
+        self.lib = load()
+        self.keys = list(keys)
+        k = (c_longlong * len(self.keys))(*self.keys)
+        sc = None if scales is None else np.ascontiguousarray(scales, np.float32)
+        out = c_longlong(0)
+        check(self.lib.aero_lbm_batch_create(k, len(self.keys), steps_per_publish, stride,
+                                             None if sc is None else fptr(sc), 1 if autotune else 0,
+                                             ctypes.byref(out)), "batch_create")
+        self.id, self.steps, self.stride = out.value, steps_per_publish, stride
+
+    def run(self, cycles=1, graph=True):
+        ms = c_float(0)
+        check(self.lib.aero_lbm_batch_run(self.id, cycles, 1 if graph else 0, ctypes.byref(ms)), "batch_run")
+        return ms.value
+
+    def run_per_brick(self, cycles=1):
+        ms = c_float(0)
+        check(self.lib.aero_lbm_batch_run_per_brick(self.id, cycles, ctypes.byref(ms)), "batch_run_per_brick")
+        return ms.value
+
+    def atlas(self, index, n):
+        sn = (n + self.stride - 1) // self.stride
+        out = np.zeros(sn ** 3 * 4, np.int16)
+        check(self.lib.aero_lbm_batch_copy_atlas(self.id, index, _ptr(out, _I16), out.size), "batch_copy_atlas")
+        return out
+
+    def info(self):
+        return self.lib.aero_lbm_batch_info(self.id).decode()
+
+    def release(self):
+        self.lib.aero_lbm_batch_release(self.id)

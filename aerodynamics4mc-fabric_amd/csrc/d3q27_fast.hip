@@ -43,12 +43,11 @@ struct TileCoord {
 // XCD-aware decode of the cell handled by this thread. Workgroups are dispatched
 // round-robin over the 8 XCDs; logical tile (L & 7) * per + (L >> 3) gives each
 // XCD one contiguous eighth of the brick.
-__device__ __forceinline__ TileCoord tile_cell(const StepParams& P) {
+__device__ __forceinline__ TileCoord tile_cell(const StepParams& P, int L) {
     const int nbz = (P.nz + kTileZ - 1) / kTileZ;
     const int nby = (P.ny + kTileY - 1) / kTileY;
     const int nb = nbz * nby * P.nx;
     const int per = (nb + 7) >> 3;
-    const int L = blockIdx.x;
     const int logical = (L & 7) * per + (L >> 3);
     const int bx = logical / (nbz * nby);
     const int r = logical - bx * nbz * nby;
@@ -193,13 +192,16 @@ __device__ __forceinline__ void face_store(const StepParams& P, int x, int y, in
 // thermal faces, with sponge or face-store duties, and all cells when the thermal
 // periodicity differs from the hydrodynamic one carry kMaskRare and are left to
 // the rare-cell kernel; here every special link is halfway bounce-back.
-template <class St, class Uv, int Mode, int StPol>
-__global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
-                                                   const typename St::T* __restrict__ fin, typename St::T* fout,
-                                                   const float* __restrict__ t_in, float* __restrict__ t_out,
-                                                   typename Uv::T* __restrict__ u_state, float* __restrict__ store_out,
-                                                   int local_first) {
-    const TileCoord tc = tile_cell(P);
+// L: the workgroup's tile index within the brick. Shell: the context has a
+// boundary-reference shell (a separate instantiation, so contexts without one keep
+// the shorter kernel and its register budget).
+template <class St, class Uv, int Mode, int StPol, bool Shell>
+__device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields& C, const uint32_t* __restrict__ mask,
+                                          const typename St::T* __restrict__ fin, typename St::T* fout,
+                                          const float* __restrict__ t_in, float* __restrict__ t_out,
+                                          typename Uv::T* __restrict__ u_state, float* __restrict__ store_out,
+                                          int local_first, int L) {
+    const TileCoord tc = tile_cell(P, L);
     if (!tc.valid) return;
     const int x = tc.x, y = tc.y, z = tc.z;
     const RowNbr R = row_nbr(P, x, y);
@@ -248,7 +250,7 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     }
     CollideResult r = collide<St::kShifted>(P, g, temp);
     if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
-    if (P.shell_on) {
+    if (Shell && P.shell_on) {
         const int layer = C.shell[cell];
         if (layer) shell_blend<St::kShifted>(P, C.shell_state, cell, layer, g, r);
     }
@@ -283,6 +285,16 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     }
 }
 
+template <class St, class Uv, int Mode, int StPol, bool Shell>
+__global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
+                                                   const typename St::T* __restrict__ fin, typename St::T* fout,
+                                                   const float* __restrict__ t_in, float* __restrict__ t_out,
+                                                   typename Uv::T* __restrict__ u_state, float* __restrict__ store_out,
+                                                   int local_first) {
+    bulk_body<St, Uv, Mode, StPol, Shell>(P, C, mask, fin, fout, t_in, t_out, u_state, store_out, local_first,
+                                          blockIdx.x);
+}
+
 // ---------------------------------------------------------------- rare-cell kernels
 // The listed rare cells take two launches. The gather launch has one thread per
 // (cell, link) and one more per cell for the temperature: every face rule and
@@ -291,17 +303,15 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
 // incoming values and the new temperature back and does the rest of the update.
 // A listed cell reads and writes only slots that no other cell touches in the
 // same step, so both launches are independent of the bulk kernel.
+// i: index in the rare list, q: 0..26 incoming population, 27 temperature
 template <class St, class Uv, int Mode>
-__global__ void __launch_bounds__(256) rare_gather_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
-                                                          const typename St::T* __restrict__ fin,
-                                                          const typename St::T* __restrict__ fout,
-                                                          const float* __restrict__ t_in,
-                                                          const typename Uv::T* __restrict__ u_state,
-                                                          const float* __restrict__ store_in, int local_first,
-                                                          const int* __restrict__ list, int count, int cap,
-                                                          float* __restrict__ vals) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = blockIdx.y;  // 0..26: incoming population, 27: temperature
+__device__ __forceinline__ void gather_body(const StepParams& P, const CellFields& C, const uint32_t* __restrict__ mask,
+                                            const typename St::T* __restrict__ fin,
+                                            const typename St::T* __restrict__ fout, const float* __restrict__ t_in,
+                                            const typename Uv::T* __restrict__ u_state,
+                                            const float* __restrict__ store_in, int local_first,
+                                            const int* __restrict__ list, int count, int cap,
+                                            float* __restrict__ vals, int i, int q) {
     if (i >= count) return;
     const int cell = list[i];
     const uint32_t m = mask[cell];
@@ -339,12 +349,24 @@ __global__ void __launch_bounds__(256) rare_gather_kernel(StepParams P, CellFiel
 }
 
 template <class St, class Uv, int Mode>
-__global__ void __launch_bounds__(128) rare_update_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
-                                                          typename St::T* fout, float* __restrict__ t_out,
-                                                          typename Uv::T* __restrict__ u_state,
-                                                          float* __restrict__ store_out, const int* __restrict__ list,
-                                                          int count, int cap, const float* __restrict__ vals) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) rare_gather_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
+                                                          const typename St::T* __restrict__ fin,
+                                                          const typename St::T* __restrict__ fout,
+                                                          const float* __restrict__ t_in,
+                                                          const typename Uv::T* __restrict__ u_state,
+                                                          const float* __restrict__ store_in, int local_first,
+                                                          const int* __restrict__ list, int count, int cap,
+                                                          float* __restrict__ vals) {
+    gather_body<St, Uv, Mode>(P, C, mask, fin, fout, t_in, u_state, store_in, local_first, list, count, cap, vals,
+                              blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+template <class St, class Uv, int Mode>
+__device__ __forceinline__ void update_body(const StepParams& P, const CellFields& C, const uint32_t* __restrict__ mask,
+                                            typename St::T* fout, float* __restrict__ t_out,
+                                            typename Uv::T* __restrict__ u_state, float* __restrict__ store_out,
+                                            const int* __restrict__ list, int count, int cap,
+                                            const float* __restrict__ vals, int i) {
     if (i >= count) return;
     const int cell = list[i];
     const uint32_t m = mask[cell];
@@ -389,6 +411,16 @@ __global__ void __launch_bounds__(128) rare_update_kernel(StepParams P, CellFiel
             St::st(fout + (sp ? lat_opp(q) : q) * Np + d, g[q]);
         }
     }
+}
+
+template <class St, class Uv, int Mode>
+__global__ void __launch_bounds__(128) rare_update_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
+                                                          typename St::T* fout, float* __restrict__ t_out,
+                                                          typename Uv::T* __restrict__ u_state,
+                                                          float* __restrict__ store_out, const int* __restrict__ list,
+                                                          int count, int cap, const float* __restrict__ vals) {
+    update_body<St, Uv, Mode>(P, C, mask, fout, t_out, u_state, store_out, list, count, cap, vals,
+                              blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- readout
@@ -482,6 +514,76 @@ __global__ void __launch_bounds__(256) readout_kernel(StepParams P, CellFields C
     d[2] = o[2];
     d[3] = o[3];
     for (int k = 4; k < out_ch; ++k) d[k] = 0.0f;
+}
+
+// ---------------------------------------------------------------- batched bricks
+// Every workgroup reads its brick's entry of the table (a wave-uniform index, so
+// the parameters come through scalar loads) and runs the single-brick code on it.
+template <class St, class Uv, int Mode, int StPol, bool Shell>
+__global__ void __launch_bounds__(256) step_kernel_batch(const BatchBrick* __restrict__ table,
+                                                         const uint16_t* __restrict__ owner) {
+    using T = typename St::T;
+    const int b = __builtin_amdgcn_readfirstlane((int)owner[blockIdx.x]);
+    const BatchBrick& B = table[b];
+    bulk_body<St, Uv, Mode, StPol, Shell>(B.P, B.C, B.mask, (const T*)B.f, (T*)B.f, B.t_in, B.t_out,
+                                          (typename Uv::T*)B.u_state, B.store_out, 0,
+                                          (int)blockIdx.x - B.bulk_begin);
+}
+
+template <class St, class Uv, int Mode>
+__global__ void __launch_bounds__(256) rare_gather_batch(const BatchBrick* __restrict__ table,
+                                                         const uint16_t* __restrict__ owner) {
+    using T = typename St::T;
+    const int b = __builtin_amdgcn_readfirstlane((int)owner[blockIdx.x]);
+    const BatchBrick& B = table[b];
+    gather_body<St, Uv, Mode>(B.P, B.C, B.mask, (const T*)B.f, (const T*)B.f, B.t_in,
+                              (const typename Uv::T*)B.u_state, B.store_in, 0, B.rare_list, B.rare_count,
+                              B.rare_cap, B.rare_vals, ((int)blockIdx.x - B.gather_begin) * 256 + (int)threadIdx.x,
+                              blockIdx.y);
+}
+
+template <class St, class Uv, int Mode>
+__global__ void __launch_bounds__(128) rare_update_batch(const BatchBrick* __restrict__ table,
+                                                         const uint16_t* __restrict__ owner) {
+    using T = typename St::T;
+    const int b = __builtin_amdgcn_readfirstlane((int)owner[blockIdx.x]);
+    const BatchBrick& B = table[b];
+    update_body<St, Uv, Mode>(B.P, B.C, B.mask, (T*)B.f, B.t_out, (typename Uv::T*)B.u_state, B.store_out,
+                              B.rare_list, B.rare_count, B.rare_cap, B.rare_vals,
+                              ((int)blockIdx.x - B.update_begin) * 128 + (int)threadIdx.x);
+}
+
+// lround of a float (halves away from zero) through double, exact for every float
+__device__ __forceinline__ int16_t quantize_i16(float v, float range) {
+    if (!isfinite(v)) return 0;
+    const float t = fminf(fmaxf(v / range, -1.0f), 1.0f) * 32767.0f;
+    const double d = (double)t;
+    return (int16_t)(d >= 0.0 ? floor(d + 0.5) : ceil(d - 0.5));
+}
+
+// one thread per sampled cell (ref: native/src/aero_lbm_simulation_bridge.cpp:5363-5380,
+// quantisation 3122-3128)
+template <class St>
+__global__ void __launch_bounds__(256) publish_batch(const BatchBrick* __restrict__ table,
+                                                     const uint16_t* __restrict__ owner) {
+    const int b = __builtin_amdgcn_readfirstlane((int)owner[blockIdx.x]);
+    const BatchBrick& B = table[b];
+    const StepParams& P = B.P;
+    const int stride = B.publish_stride;
+    const int sx = (P.nx + stride - 1) / stride, sy = (P.ny + stride - 1) / stride, sz = (P.nz + stride - 1) / stride;
+    const int i = ((int)blockIdx.x - B.publish_begin) * 256 + (int)threadIdx.x;
+    if (i >= sx * sy * sz) return;
+    const int ax = i / (sy * sz), rem = i - ax * sy * sz;
+    const int ay = rem / sz, az = rem - ay * sz;
+    float o[4];
+    cell_output_fast<St>(P, B.C, B.mask, (const typename St::T*)B.f, B.store_out, B.publish_layout,
+                         min(P.nx - 1, ax * stride), min(P.ny - 1, ay * stride), min(P.nz - 1, az * stride), 1.0f, o);
+    short4 q;
+    q.x = quantize_i16(o[0] * B.publish_scale, 5.6f);
+    q.y = quantize_i16(o[1] * B.publish_scale, 5.6f);
+    q.z = quantize_i16(o[2] * B.publish_scale, 5.6f);
+    q.w = quantize_i16(o[3], 0.03f);
+    reinterpret_cast<short4*>(B.publish_out)[i] = q;
 }
 
 // ---------------------------------------------------------------- init and link masks
@@ -666,28 +768,37 @@ static int store_policy(size_t working_set) {
 }
 
 template <class St, class Uv, int Mode, int StPol>
+static void launch_bulk(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
+                        const float* t_in, float* t_out, void* u_state, float* store_out, int lf, hipStream_t s) {
+    using T = typename St::T;
+    using U = typename Uv::T;
+    const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
+    if (P.shell_on)
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, true>), grid, block, 0, s, P, C, mask,
+                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
+    else
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, false>), grid, block, 0, s, P, C, mask,
+                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
+}
+
+template <class St, class Uv, int Mode, int StPol>
 static void launch_step_m(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
                           const float* t_in, float* t_out, void* u_state, const float* store_in, float* store_out,
                           int local_first, const FastRare& rare, hipStream_t s) {
     using T = typename St::T;
     using U = typename Uv::T;
-    const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
     const int lf = Mode == 2 ? 0 : local_first;
     if (rare.count > 0) {
-        const dim3 ggrid((unsigned)((rare.count + 255) / 256), 28), ugrid((unsigned)((rare.count + 127) / 128));
         // serial on the stream: a second stream beside the bulk kernel costs more in
         // cross-stream synchronisation than the two short launches take
+        const dim3 ggrid((unsigned)((rare.count + 255) / 256), 28), ugrid((unsigned)((rare.count + 127) / 128));
         hipLaunchKernelGGL((fast::rare_gather_kernel<St, Uv, Mode>), ggrid, dim3(256), 0, s, P, C, mask,
                            (const T*)fin, (const T*)fout, t_in, (const U*)u_state, store_in, lf, rare.list,
                            rare.count, rare.cap, rare.vals);
         hipLaunchKernelGGL((fast::rare_update_kernel<St, Uv, Mode>), ugrid, dim3(128), 0, s, P, C, mask, (T*)fout,
                            t_out, (U*)u_state, store_out, rare.list, rare.count, rare.cap, (const float*)rare.vals);
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
-                           (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
-        return;
     }
-    hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol>), grid, block, 0, s, P, C, mask, (const T*)fin,
-                       (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
+    launch_bulk<St, Uv, Mode, StPol>(P, C, mask, fin, fout, t_in, t_out, u_state, store_out, lf, s);
 }
 
 template <class St, class Uv, int StPol>
@@ -746,6 +857,56 @@ void launch_fast_rare_list(const StepParams& P, const uint32_t* mask, int* list,
     hipLaunchKernelGGL(fast::rare_count_kernel, dim3(blocks), dim3(256), 0, s, P.cells, mask, scratch);
     hipLaunchKernelGGL(fast::rare_scan_kernel, dim3(1), dim3(1024), 0, s, blocks, scratch, counter);
     hipLaunchKernelGGL(fast::rare_scatter_kernel, dim3(blocks), dim3(256), 0, s, P.cells, mask, scratch, list);
+}
+
+int fast_tile_blocks(const StepParams& P) { return fast::tile_grid(P); }
+int fast_store_policy(size_t working_set_bytes) { return store_policy(working_set_bytes); }
+
+template <class St, class Uv, int Mode, int StPol>
+static void batch_step_t(bool shell, const BatchBrick* table, const BatchGrids& g, hipStream_t s) {
+    if (g.gather_blocks > 0) {
+        hipLaunchKernelGGL((fast::rare_gather_batch<St, Uv, Mode>), dim3((unsigned)g.gather_blocks, 28), dim3(256), 0,
+                           s, table, g.gather_owner);
+        hipLaunchKernelGGL((fast::rare_update_batch<St, Uv, Mode>), dim3((unsigned)g.update_blocks), dim3(128), 0, s,
+                           table, g.update_owner);
+    }
+    const dim3 block(fast::kTileZ, fast::kTileY);
+    if (shell)
+        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, true>), dim3((unsigned)g.bulk_blocks), block,
+                           0, s, table, g.bulk_owner);
+    else
+        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, false>), dim3((unsigned)g.bulk_blocks), block,
+                           0, s, table, g.bulk_owner);
+}
+
+template <class St, class Uv>
+static void batch_step_s(int mode, int pol, bool shell, const BatchBrick* table, const BatchGrids& g, hipStream_t s) {
+    if (pol == fast::kPolStream) {
+        if (mode == 1) batch_step_t<St, Uv, 1, fast::kPolStream>(shell, table, g, s);
+        else batch_step_t<St, Uv, 2, fast::kPolStream>(shell, table, g, s);
+    } else {
+        if (mode == 1) batch_step_t<St, Uv, 1, fast::kPolDefault>(shell, table, g, s);
+        else batch_step_t<St, Uv, 2, fast::kPolDefault>(shell, table, g, s);
+    }
+}
+
+void launch_batch_step(FastStorage st, int mode, int store_policy_, bool shell, const BatchBrick* table,
+                       const BatchGrids& g, hipStream_t s) {
+    switch (st) {
+        case FastStorage::F32: batch_step_s<fast::StF32, fast::UF32>(mode, store_policy_, shell, table, g, s); break;
+        case FastStorage::F16: batch_step_s<fast::StF16, fast::UF16>(mode, store_policy_, shell, table, g, s); break;
+        case FastStorage::BF16: batch_step_s<fast::StBF16, fast::UF16>(mode, store_policy_, shell, table, g, s); break;
+    }
+}
+
+void launch_batch_publish(FastStorage st, const BatchBrick* table, const BatchGrids& g, hipStream_t s) {
+    if (g.publish_blocks <= 0) return;
+    const dim3 grid((unsigned)g.publish_blocks);
+    switch (st) {
+        case FastStorage::F32: hipLaunchKernelGGL(fast::publish_batch<fast::StF32>, grid, dim3(256), 0, s, table, g.publish_owner); break;
+        case FastStorage::F16: hipLaunchKernelGGL(fast::publish_batch<fast::StF16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
+        case FastStorage::BF16: hipLaunchKernelGGL(fast::publish_batch<fast::StBF16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
+    }
 }
 
 void launch_fast_readout(const StepParams& P, const CellFields& C, FastStorage st, const uint32_t* mask, const void* f,

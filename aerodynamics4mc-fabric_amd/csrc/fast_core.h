@@ -453,8 +453,22 @@ __device__ __forceinline__ void shell_blend(const StepParams& P, const uint16_t*
     const float nux = r.ux + alpha * (tux - r.ux);
     const float nuy = r.uy + alpha * (tuy - r.uy);
     const float nuz = r.uz + alpha * (tuz - r.uz);
+    // feq(nrho, nu) - feq(rho, u) = w_q (a + c.b + c.D.c): the equilibrium's moments
+    // of order 0, 1 and 2 change by a, b and D
+    const float a = (nrho - rho) - 1.5f * (nrho * (nux * nux + nuy * nuy + nuz * nuz) -
+                                          rho * (r.ux * r.ux + r.uy * r.uy + r.uz * r.uz));
+    const float bx = 3.0f * (nrho * nux - rho * r.ux), by = 3.0f * (nrho * nuy - rho * r.uy),
+                bz = 3.0f * (nrho * nuz - rho * r.uz);
+    const float dxx = 4.5f * (nrho * nux * nux - rho * r.ux * r.ux), dyy = 4.5f * (nrho * nuy * nuy - rho * r.uy * r.uy),
+                dzz = 4.5f * (nrho * nuz * nuz - rho * r.uz * r.uz);
+    const float dxy = 9.0f * (nrho * nux * nuy - rho * r.ux * r.uy), dxz = 9.0f * (nrho * nux * nuz - rho * r.ux * r.uz),
+                dyz = 9.0f * (nrho * nuy * nuz - rho * r.uy * r.uz);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) g[q] += feq_s<false>(q, nrho, nux, nuy, nuz) - feq_s<false>(q, rho, r.ux, r.uy, r.uz);
+    for (int q = 0; q < 27; ++q) {
+        const float cx = (float)lat_cx(q), cy = (float)lat_cy(q), cz = (float)lat_cz(q);
+        const float quad = cx * cx * dxx + cy * cy * dyy + cz * cz * dzz + cx * cy * dxy + cx * cz * dxz + cy * cz * dyz;
+        g[q] += wq(q) * (a + cx * bx + cy * by + cz * bz + quad);
+    }
     r.ux = nux;
     r.uy = nuy;
     r.uz = nuz;

@@ -53,4 +53,44 @@ void launch_fast_readout(const StepParams& P, const CellFields& C, FastStorage s
                          const float* store, int layout, int stride, int out_ch, float scale, float* out,
                          hipStream_t s);
 
+// ---- many bricks per launch ----
+// One brick of a batched step: its step parameters and buffers for that step.
+// Blocks of a batched grid find their brick through a per-block owner table; a
+// brick's blocks start at a multiple of 8 so its XCD-aware tile order is kept.
+struct BatchBrick {
+    StepParams P;
+    CellFields C;
+    const uint32_t* mask;
+    void* f;                   // A-A population buffer
+    const float* t_in;
+    float* t_out;
+    void* u_state;
+    const float* store_in;
+    float* store_out;
+    const int* rare_list;
+    int rare_count, rare_cap;
+    float* rare_vals;
+    int bulk_begin, gather_begin, update_begin, publish_begin;  // first block of each grid
+    int publish_layout;        // A-A layout the publish reads (after this step)
+    int publish_stride;
+    float publish_scale;       // lattice -> m/s factor of the published velocity
+    int16_t* publish_out;      // 4 int16 per sampled cell
+};
+
+// one batched step: every brick of `table` takes one A-A step of parity `mode`
+// (1 even, 2 odd); the grids are sized by the owner tables
+struct BatchGrids {
+    const uint16_t* bulk_owner;
+    const uint16_t* gather_owner;
+    const uint16_t* update_owner;
+    const uint16_t* publish_owner;
+    int bulk_blocks, gather_blocks, update_blocks, publish_blocks;
+};
+void launch_batch_step(FastStorage st, int mode, int store_policy, bool shell, const BatchBrick* table,
+                       const BatchGrids& g, hipStream_t s);
+// int16 flow atlas of every brick (velocity over +-5.6 m/s, pressure over +-0.03)
+void launch_batch_publish(FastStorage st, const BatchBrick* table, const BatchGrids& g, hipStream_t s);
+int fast_tile_blocks(const StepParams& P);
+int fast_store_policy(size_t working_set_bytes);
+
 }  // namespace aero

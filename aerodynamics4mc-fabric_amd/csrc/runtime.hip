@@ -361,6 +361,11 @@ bool prepare_params(Context& c, StepParams& P) {
             if (!storage_rebuild_masks(c, P)) return false;
         }
     }
+    return sync_params(c, P);
+}
+
+// keep the context's host and device copies of its step parameters equal to P
+bool sync_params(Context& c, StepParams& P) {
     P.self_dev = c.params_dev;
     if (!c.params_valid || std::memcmp(&c.params_host, &P, sizeof(P)) != 0) {
         if (!hip_ok(hipStreamSynchronize(rt().stream), "sync")) return false;
@@ -523,6 +528,7 @@ bool step_context(Context& c, const float* packet, float* out, int out_ch, float
     }
     update_thermal_activity(c, P);
     P.thermal_on = c.thermal_on ? 1 : 0;
+    if (!sync_params(c, P)) return false;  // the step runs with (and records) the thermal switch just decided
     if (!storage_step(c, P)) return false;
     c.steps += 1;
     if (out) {

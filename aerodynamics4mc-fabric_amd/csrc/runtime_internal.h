@@ -121,6 +121,7 @@ bool ensure_device();
 bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P);
 // resolve_params + upload of the face table into the context's device copy
 bool prepare_params(Context& c, StepParams& P);
+bool sync_params(Context& c, StepParams& P);
 void sanitize_benchmark_config(AeroLbmBenchmarkConfig& c);
 Context* find_context(long long key);
 Context* get_context(long long key, int nx, int ny, int nz, bool create);

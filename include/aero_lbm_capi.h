@@ -336,6 +336,26 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_packed_flow_atlas(
     int sample_stride, int16_t* out_packed_flow, int packed_value_count);
 AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_last_error(void);
 
+/* ---- batched stepping (extension: no counterpart in the reference, whose brick
+ * runtime steps and publishes each brick with its own launches and readback,
+ * native/src/aero_lbm_simulation_bridge.cpp:1004-1171, 5301-5410). The contexts
+ * (in-place storages, same storage and step parity, each stepped at least once)
+ * advance together: per step one rare-cell gather, one rare-cell update and one bulk
+ * launch for all bricks, each brick keeping the parameters of its last individual
+ * step; every steps_per_publish (even) steps the int16 atlas of every brick
+ * (publish_stride, velocity x publish_scales[i] over +-5.6, pressure over +-0.03)
+ * is written and copied to host memory. The cycle is captured in a HIP graph.
+ * autotune_launch: time brick orders and store cache policies, keep the fastest.
+ * Stepping a member outside the batch invalidates the batch. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_batch_create(const long long* context_keys, int count, int steps_per_publish,
+                                               int publish_stride, const float* publish_scales, int autotune_launch,
+                                               long long* out_batch);
+AERO_LBM_CAPI_EXPORT int aero_lbm_batch_run(long long batch, int cycles, int use_graph, float* out_ms);
+AERO_LBM_CAPI_EXPORT int aero_lbm_batch_run_per_brick(long long batch, int cycles, float* out_ms);
+AERO_LBM_CAPI_EXPORT int aero_lbm_batch_copy_atlas(long long batch, int index, int16_t* out, int value_count);
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_batch_info(long long batch);
+AERO_LBM_CAPI_EXPORT void aero_lbm_batch_release(long long batch);
+
 /* ---- mesoscale D3Q19 step (ref: native/include/aero_lbm_capi.h:286-301) ----
  * forcing: cells x 25 floats, state: cells x 6 floats (ambient, deep ground,
  * surface, wind x, y, z), cell = (x*ny + y)*nz + z, y vertical. */
