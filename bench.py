@@ -170,6 +170,78 @@ def run_meso(steps, warmup, no_cpu):
     return out
 
 
+BATCH_BRICKS = ((32, 40), (64, 12), (128, 4))  # (edge, count)
+
+
+def run_batched(steps, warmup, per_brick_steps=40):
+    """40 x 32^3 + 12 x 64^3 + 4 x 128^3 FP16 bricks, each with seeded solid boxes
+    (5-20 % fill), its own inflow (speed 0.02-0.08, direction uniform in the x-y
+    plane), Boussinesq buoyancy and a seeded hot blob; stepped together with the
+    int16 atlas of every brick published every 4 steps (stride 4), the cycle replayed
+    from a HIP graph. The same bricks stepped one launch sequence per brick (with the
+    per-brick atlas read back and quantised on the host) give the per-brick rate."""
+    import math
+    import aero_lbm as A
+    import workloads as W
+    lib = A.load()
+    A.check(lib.aero_lbm_init_rect(32, 32, 32, 11, 4), "init_rect")
+    ov = W.reference_overrides(A.KERNEL_FAST)
+    A.check(lib.aero_lbm_set_solver_overrides(ctypes.byref(ov)), "set_solver_overrides")
+    rng = np.random.default_rng(2024)
+    keys, cells = [], 0
+    key = 20000
+    for edge, count in BATCH_BRICKS:
+        for _ in range(count):
+            seed = int(rng.integers(1, 1 << 30))
+            fill = float(rng.uniform(0.05, 0.20))
+            speed = float(rng.uniform(0.02, 0.08))
+            ang = float(rng.uniform(0.0, 2.0 * math.pi))
+            v = (speed * math.cos(ang), speed * math.sin(ang), 0.0)
+            cfg = W.channel_config(speed, 50.0, sgs=True, buoyancy=True)
+            cfg.initial_velocity[0], cfg.initial_velocity[1] = v[0], v[1]
+            cfg.x_min.velocity[0], cfg.x_min.velocity[1] = v[0], v[1]
+            solid = W.solid_boxes(edge, edge, edge, fill, seed=seed, edge=(2, max(3, edge // 8)))
+            t = W.hot_blob(edge, edge, edge, 0.02, edge / 16.0, seed=seed + 1, axis=2)
+            t[solid.astype(bool)] = 0.0
+            pk = W.make_packet(solid, u0=v, perturb=1e-3, seed_u=seed + 2, temperature=t)
+            A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "config")
+            lib.aero_lbm_set_realtime_solver_mode(A.SOLVER_D3Q27_FP16_INPLACE)
+            ctx = A.Context(edge, edge, edge, key=key)
+            ctx.set_temperature(t)
+            ctx.step(pk)  # upload, initialise, first step (untimed)
+            keys.append(key)
+            cells += edge ** 3
+            key += 1
+    lib.aero_lbm_set_realtime_solver_mode(0)
+    K = 4
+    batch = A.Batch(keys, steps_per_publish=K, stride=4, scales=[20.0] * len(keys), autotune=True)
+    info = batch.info()
+    batch.run(max(1, warmup // K), graph=True)
+    cycles = max(1, steps // K)
+    t0 = time.perf_counter()
+    device_ms = batch.run(cycles, graph=True)
+    wall = time.perf_counter() - t0
+    pb_cycles = max(1, per_brick_steps // K)
+    pb_ms = batch.run_per_brick(pb_cycles)
+    batch.release()
+    for k in keys:
+        lib.aero_lbm_release_context(k)
+    nsteps = cycles * K
+    ms = wall * 1e3 / nsteps
+    b_lup = B_LUP["fp16"]  # FP16 populations + temperature + flag byte
+    out = {"workload": "batched_bricks_40x32^3_12x64^3_4x128^3_fp16_graph_step_publish4", "bricks": len(keys),
+           "cells": cells, "steps": nsteps, "glups": cells * nsteps / wall / 1e9, "ms_per_step": ms,
+           "device_ms_per_step": device_ms / nsteps,
+           "per_brick_glups": cells * pb_cycles * K / (pb_ms / 1e3) / 1e9, "per_brick_ms_per_step": pb_ms / (pb_cycles * K),
+           "launch": info,
+           "roofline": {"bound": "hbm", "achieved": cells * b_lup / (device_ms / nsteps * 1e-3) / 1e9,
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": cells * b_lup / (device_ms / nsteps * 1e-3) / HBM_PEAK, "traffic": None,
+                        "bytes_per_lup": b_lup}}
+    out["speedup_vs_per_brick"] = out["glups"] / out["per_brick_glups"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +252,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-meso", action="store_true", help="skip the D3Q19 mesoscale line")
     ap.add_argument("--meso-steps", type=int, default=500)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-bricks line")
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) five times, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -251,6 +325,12 @@ def main():
         print(f"bench: mesoscale d3q19 {MESO_DIMS} steps={args.meso_steps}", file=sys.stderr, flush=True)
         line["mesoscale"] = run_meso(args.meso_steps, min(args.warmup, 20), args.no_cpu)
         print(f"bench: mesoscale {line['mesoscale']['glups']:.2f} GLUPS", file=sys.stderr, flush=True)
+    if not args.no_batch:
+        print(f"bench: batched bricks steps={args.batch_steps}", file=sys.stderr, flush=True)
+        line["batched"] = run_batched(args.batch_steps, min(args.warmup, 20))
+        print(f"bench: batched {line['batched']['glups']:.2f} GLUPS vs per-brick "
+              f"{line['batched']['per_brick_glups']:.2f} ({line['batched']['speedup_vs_per_brick']:.2f}x); "
+              f"{line['batched']['launch']}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 
