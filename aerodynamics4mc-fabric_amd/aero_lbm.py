@@ -115,6 +115,11 @@ class WorldDelta(Structure):
                 ("value3", c_float)]
 
 
+class AdjointConfig(Structure):
+    _fields_ = [("n", c_int), ("heat_grid", c_int), ("steps", c_int), ("checkpoint_every", c_int), ("nu", c_float),
+                ("beta", c_float), ("alpha", c_float), ("u0", c_float * 3), ("probe_count", c_int)]
+
+
 class AeroLbmError(RuntimeError):
     pass
 
@@ -212,6 +217,11 @@ PROTOTYPES = {
     "aero_lbm_batch_copy_atlas": (c_int, [c_longlong, c_int, _I16, c_int]),
     "aero_lbm_batch_info": (c_char_p, [c_longlong]),
     "aero_lbm_batch_release": (None, [c_longlong]),
+    "aero_lbm_adjoint_create": (c_int, [POINTER(AdjointConfig), _U8, _I, _F, POINTER(c_longlong)]),
+    "aero_lbm_adjoint_forward": (c_int, [c_longlong, _F, _F, POINTER(c_double), _F]),
+    "aero_lbm_adjoint_backward": (c_int, [c_longlong, _F, _F]),
+    "aero_lbm_adjoint_forward_fp64": (c_int, [c_longlong, POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
+    "aero_lbm_adjoint_release": (None, [c_longlong]),
 }
 
 
@@ -504,3 +514,45 @@ class Batch:
 
     def release(self):
         self.lib.aero_lbm_batch_release(self.id)
+
+
+class Adjoint:
+    """Forward / backward pair of the brick-step adjoint (aero_lbm_adjoint_*)."""
+
+    def __init__(self, n, heat_grid, steps, every, nu, beta, alpha, u0, solid, probes, target):
+        self.lib = load()
+        cfg = AdjointConfig(n, heat_grid, steps, every, nu, beta, alpha, (c_float * 3)(*u0), len(probes))
+        sol = np.ascontiguousarray(np.asarray(solid, np.uint8).reshape(-1))
+        pr = np.ascontiguousarray(np.asarray(probes, np.int32))
+        tg = np.ascontiguousarray(np.asarray(target, np.float32).reshape(-1))
+        h = c_longlong(0)
+        check(self.lib.aero_lbm_adjoint_create(ctypes.byref(cfg), _ptr(sol, _U8), _ptr(pr, _I), fptr(tg),
+                                               ctypes.byref(h)), "adjoint_create")
+        self.h, self.n, self.m, self.nprobe = h.value, n, heat_grid, len(probes)
+
+    def forward(self, inflow, heat):
+        fi = np.ascontiguousarray(inflow, np.float32).reshape(-1)
+        hh = np.ascontiguousarray(heat, np.float32).reshape(-1)
+        loss = c_double(0)
+        u = np.zeros(3 * self.nprobe, np.float32)
+        check(self.lib.aero_lbm_adjoint_forward(self.h, fptr(fi), fptr(hh), ctypes.byref(loss), fptr(u)),
+              "adjoint_forward")
+        return loss.value, u.reshape(-1, 3)
+
+    def backward(self):
+        gi = np.zeros(3 * self.n * self.n, np.float32)
+        gh = np.zeros(self.m ** 3, np.float32)
+        check(self.lib.aero_lbm_adjoint_backward(self.h, fptr(gi), fptr(gh)), "adjoint_backward")
+        return gi, gh
+
+    def forward_fp64(self, inflow, heat):
+        fi = np.ascontiguousarray(inflow, np.float64).reshape(-1)
+        hh = np.ascontiguousarray(heat, np.float64).reshape(-1)
+        loss = c_double(0)
+        D = POINTER(c_double)
+        check(self.lib.aero_lbm_adjoint_forward_fp64(self.h, fi.ctypes.data_as(D), hh.ctypes.data_as(D),
+                                                     ctypes.byref(loss)), "adjoint_forward_fp64")
+        return loss.value
+
+    def release(self):
+        self.lib.aero_lbm_adjoint_release(self.h)

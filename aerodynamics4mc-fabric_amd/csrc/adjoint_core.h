@@ -1,0 +1,295 @@
+// Differentiable D3Q27 brick step for the adjoint: the cumulant collision with
+// Boussinesq buoyancy and Guo forcing, A-B pull streaming with halfway
+// bounce-back, a per-cell velocity inlet on the x- face, convective outflow on
+// the x+ face, periodic y and z, and the benchmark-mode temperature update. The
+// same code runs in FP32 / FP64 and on forward-mode dual numbers, on the device
+// and on the host, so the adjoint kernels differentiate exactly the forward they
+// replay (ref: native/src/aero_lbm_jni.cpp:2209-2413 collide, 1475-1602 face
+// rules, 2145-2188 temperature; the Smagorinsky term is off in this model).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace aero {
+namespace adj {
+
+#define AD_HD __host__ __device__ __forceinline__
+
+// forward-mode dual number: value and one tangent
+template <class T>
+struct Dual {
+    T v, d;
+    AD_HD Dual() : v(0), d(0) {}
+    AD_HD Dual(T a) : v(a), d(0) {}
+    AD_HD Dual(T a, T b) : v(a), d(b) {}
+};
+template <class T> AD_HD Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
+template <class T> AD_HD Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
+template <class T> AD_HD Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
+template <class T> AD_HD Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+template <class T> AD_HD Dual<T> operator/(Dual<T> a, Dual<T> b) {
+    const T q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+template <class T> AD_HD Dual<T> operator+(Dual<T> a, T b) { return {a.v + b, a.d}; }
+template <class T> AD_HD Dual<T> operator+(T a, Dual<T> b) { return {a + b.v, b.d}; }
+template <class T> AD_HD Dual<T> operator-(Dual<T> a, T b) { return {a.v - b, a.d}; }
+template <class T> AD_HD Dual<T> operator-(T a, Dual<T> b) { return {a - b.v, -b.d}; }
+template <class T> AD_HD Dual<T> operator*(Dual<T> a, T b) { return {a.v * b, a.d * b}; }
+template <class T> AD_HD Dual<T> operator*(T a, Dual<T> b) { return {a * b.v, a * b.d}; }
+template <class T> AD_HD Dual<T> operator/(Dual<T> a, T b) { return {a.v / b, a.d / b}; }
+template <class T> AD_HD Dual<T>& operator+=(Dual<T>& a, Dual<T> b) { return a = a + b; }
+template <class T> AD_HD Dual<T>& operator-=(Dual<T>& a, Dual<T> b) { return a = a - b; }
+template <class T> AD_HD Dual<T>& operator*=(Dual<T>& a, Dual<T> b) { return a = a * b; }
+
+template <class T> AD_HD T val(T x) { return x; }
+template <class T> AD_HD T val(Dual<T> x) { return x.v; }
+template <class T> AD_HD T tan_(T) { return T(0); }
+template <class T> AD_HD T tan_(Dual<T> x) { return x.d; }
+AD_HD float sqrt_(float x) { return sqrtf(x); }
+AD_HD double sqrt_(double x) { return sqrt(x); }
+template <class T> AD_HD Dual<T> sqrt_(Dual<T> x) {
+    const T s = sqrt_(x.v);
+    return {s, x.d / (T(2) * s)};
+}
+// min / max / clamp choose by value (the derivative of the chosen branch)
+template <class R> AD_HD R rmin(R a, R b) { return val(b) < val(a) ? b : a; }
+template <class R> AD_HD R rmax(R a, R b) { return val(a) < val(b) ? b : a; }
+template <class R, class S> AD_HD R rclamp(R v, S lo, S hi) { return rmin(R(hi), rmax(R(lo), v)); }
+
+struct Params {
+    int nx, ny, nz;
+    double tau_shear, tau_normal;  // relaxation times of the off-diagonal and diagonal second moments
+    float beta;                   // Boussinesq coefficient
+    int g_axis, g_sign;           // gravity axis and sign (force = -g_sign * beta T along g_axis)
+    float alpha;                  // thermal diffusivity (lattice)
+    float kmax;                   // velocity cap
+    float rho_in;                 // inlet density
+    float u0[3];                  // initial velocity
+};
+
+AD_HD int cxq(int q) { return q / 9 - 1; }
+AD_HD int cyq(int q) { return (q / 3) % 3 - 1; }
+AD_HD int czq(int q) { return q % 3 - 1; }
+AD_HD int oppq(int q) { return 26 - q; }
+template <class R> AD_HD R wq(int q) {
+    const int n = (cxq(q) != 0) + (cyq(q) != 0) + (czq(q) != 0);
+    return n == 0 ? R(8.0 / 27.0) : (n == 1 ? R(2.0 / 27.0) : (n == 2 ? R(1.0 / 54.0) : R(1.0 / 216.0)));
+}
+template <class R> AD_HD R feq(int q, R rho, R ux, R uy, R uz) {
+    const R cu = R(3) * (R(float(cxq(q))) * ux + R(float(cyq(q))) * uy + R(float(czq(q))) * uz);
+    const R uu = ux * ux + uy * uy + uz * uz;
+    return wq<R>(q) * rho * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * uu);
+}
+
+#define AD_IDX(a, b, c) (((a) * 3 + (b)) * 3 + (c))
+
+// populations (c = -1, 0, +1) <-> raw moments (order 0, 1, 2) along one line
+template <class R> AD_HD void fwd_line(R& a, R& b, R& c) {
+    const R s = a + c, t = c - a;
+    a = s + b;
+    b = t;
+    c = s;
+}
+template <class R> AD_HD void inv_line(R& m0, R& m1, R& m2) {
+    const R h2 = R(0.5) * m2, h1 = R(0.5) * m1;
+    const R c0 = m0 - m2;
+    m0 = h2 - h1;  // c = -1
+    m1 = c0;       // c = 0
+    m2 = h2 + h1;  // c = +1
+}
+template <class R> AD_HD void shift_line(R& m0, R& m1, R& m2, R u) {
+    const R k1 = m1 - u * m0;
+    m2 = m2 - R(2) * u * m1 + u * u * m0;
+    m1 = k1;
+}
+template <class R> AD_HD void unshift_line(R& k0, R& k1, R& k2, R u) {
+    k2 = k2 + R(2) * u * k1 + u * u * k0;
+    k1 = k1 + u * k0;
+}
+// apply a line transform along axis a (0 = x, 1 = y, 2 = z) to all 9 lines
+template <class R, class Fn> AD_HD void lines(R m[27], int axis, Fn fn) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int a0 = axis == 0 ? AD_IDX(0, i, j) : axis == 1 ? AD_IDX(i, 0, j) : AD_IDX(i, j, 0);
+            const int st = axis == 0 ? 9 : axis == 1 ? 3 : 1;
+            fn(m[a0], m[a0 + st], m[a0 + 2 * st]);
+        }
+}
+
+// One fluid cell: f (27 incoming populations) -> post-collision populations; u_out
+// receives the collision velocity. T: the cell's new temperature.
+template <class R> AD_HD void collide(const Params& P, R f[27], R T, R u_out[3]) {
+    lines(f, 2, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
+    lines(f, 1, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
+    lines(f, 0, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
+    const R rho = f[AD_IDX(0, 0, 0)];
+    const R rho_s = rmax(R(1e-6f), rho);
+    R u[3] = {f[AD_IDX(1, 0, 0)] / rho_s, f[AD_IDX(0, 1, 0)] / rho_s, f[AD_IDX(0, 0, 1)] / rho_s};
+    // Boussinesq: F / rho = clamp(beta T, +-0.02) against gravity
+    R b = rclamp(R(P.beta) * T, -0.02f, 0.02f);
+    if (P.g_sign > 0) b = -b;
+    u[P.g_axis] = u[P.g_axis] + R(0.5) * b;
+    {
+        const R s2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        if (val(s2) > P.kmax * P.kmax) {
+            const R sc = R(P.kmax) / sqrt_(s2);
+            u[0] = u[0] * sc;
+            u[1] = u[1] * sc;
+            u[2] = u[2] * sc;
+        }
+    }
+    for (int k = 0; k < 3; ++k) u_out[k] = u[k];
+    const R ux = u[0], uy = u[1], uz = u[2];
+    lines(f, 0, [ux](R& a, R& b, R& c) { shift_line(a, b, c, ux); });
+    lines(f, 1, [uy](R& a, R& b, R& c) { shift_line(a, b, c, uy); });
+    lines(f, 2, [uz](R& a, R& b, R& c) { shift_line(a, b, c, uz); });
+    const R cs2 = R(1.0 / 3.0);
+    // relaxation rates clamped to [0, 1.95] in the closure (ref: jni.cpp:1384-1430);
+    // the Guo prefactor takes the unclamped shear rate
+    const double om_off = 1.0 / P.tau_shear;
+    const R sd = R(fmin(fmax(1.0 / P.tau_normal, 0.0), 1.95)), so = R(fmin(fmax(om_off, 0.0), 1.95));
+    const R eq2 = rho_s * cs2;
+    const R d200 = f[AD_IDX(2, 0, 0)] + sd * (eq2 - f[AD_IDX(2, 0, 0)]);
+    const R d020 = f[AD_IDX(0, 2, 0)] + sd * (eq2 - f[AD_IDX(0, 2, 0)]);
+    const R d002 = f[AD_IDX(0, 0, 2)] + sd * (eq2 - f[AD_IDX(0, 0, 2)]);
+    const R om = R(1) - so;
+    const R d110 = om * f[AD_IDX(1, 1, 0)], d101 = om * f[AD_IDX(1, 0, 1)], d011 = om * f[AD_IDX(0, 1, 1)];
+    const R ir = R(1) / rho_s;
+    // Gaussian closure of orders 4 and 6, zero odd central moments (ref: jni.cpp:1384-1430)
+    const R d220 = (d200 * d020 + R(2) * d110 * d110) * ir;
+    const R d202 = (d200 * d002 + R(2) * d101 * d101) * ir;
+    const R d022 = (d020 * d002 + R(2) * d011 * d011) * ir;
+    const R d211 = (d200 * d011 + R(2) * d110 * d101) * ir;
+    const R d121 = (d020 * d101 + R(2) * d110 * d011) * ir;
+    const R d112 = (d002 * d110 + R(2) * d101 * d011) * ir;
+    const R d222 = (d200 * d020 * d002 + R(2) * d110 * d110 * d002 + R(2) * d101 * d101 * d020 +
+                    R(2) * d011 * d011 * d200 + R(8) * d110 * d101 * d011) * ir * ir;
+    for (int k = 0; k < 27; ++k) f[k] = R(0);
+    f[AD_IDX(0, 0, 0)] = rho_s;
+    f[AD_IDX(2, 0, 0)] = d200;
+    f[AD_IDX(0, 2, 0)] = d020;
+    f[AD_IDX(0, 0, 2)] = d002;
+    f[AD_IDX(1, 1, 0)] = d110;
+    f[AD_IDX(1, 0, 1)] = d101;
+    f[AD_IDX(0, 1, 1)] = d011;
+    f[AD_IDX(2, 2, 0)] = d220;
+    f[AD_IDX(2, 0, 2)] = d202;
+    f[AD_IDX(0, 2, 2)] = d022;
+    f[AD_IDX(2, 1, 1)] = d211;
+    f[AD_IDX(1, 2, 1)] = d121;
+    f[AD_IDX(1, 1, 2)] = d112;
+    f[AD_IDX(2, 2, 2)] = d222;
+    lines(f, 2, [uz](R& a, R& b, R& c) { unshift_line(a, b, c, uz); });
+    lines(f, 1, [uy](R& a, R& b, R& c) { unshift_line(a, b, c, uy); });
+    lines(f, 0, [ux](R& a, R& b, R& c) { unshift_line(a, b, c, ux); });
+    lines(f, 0, [](R& a, R& b, R& c) { inv_line(a, b, c); });
+    lines(f, 1, [](R& a, R& b, R& c) { inv_line(a, b, c); });
+    lines(f, 2, [](R& a, R& b, R& c) { inv_line(a, b, c); });
+    // Guo source S_q = P w_q [3 (c - u).F + 9 (c.u)(c.F)], F = rho b along the axis.
+    // The branch depends on beta, not on b: at T = 0 the term is zero but its
+    // derivative with respect to T is not.
+    if (P.beta != 0.0f) {
+        const R pref = rmax(R(0), R(1) - R(0.5) * R(fmin(fmax(om_off, 0.0), 1.999)));
+        const R fa = pref * rho_s * b;
+        for (int q = 0; q < 27; ++q) {
+            const R c[3] = {R(float(cxq(q))), R(float(cyq(q))), R(float(czq(q)))};
+            const R cu = c[0] * ux + c[1] * uy + c[2] * uz;
+            f[q] = f[q] + wq<R>(q) * fa * (R(3) * (c[P.g_axis] - u[P.g_axis]) + R(9) * cu * c[P.g_axis]);
+        }
+    }
+}
+
+AD_HD int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+AD_HD int cidx(const Params& P, int x, int y, int z) { return (x * P.ny + y) * P.nz + z; }
+
+// incoming population q of fluid cell (x, y, z) from the previous post-collision
+// field F (27 planes of `cells`): pull, halfway bounce-back at solid sources,
+// velocity inlet (equilibrium at the cell's inflow parameter) on x-, convective
+// outflow f1 + 0.15 (f1 - f2) on x+ (ref: jni.cpp:1494-1561)
+template <class R>
+AD_HD R incoming(const Params& P, const R* F, const uint8_t* solid, const R* uin, int x, int y, int z, int q) {
+    const int N = P.nx * P.ny * P.nz;
+    const int cx = cxq(q);
+    const int sx = x - cx, sy = wrap(y - cyq(q), P.ny), sz = wrap(z - czq(q), P.nz);
+    const int c = cidx(P, x, y, z);
+    if (sx < 0) {
+        const R* u = uin + 3 * (y * P.nz + z);
+        return feq<R>(q, R(P.rho_in), u[0], u[1], u[2]);
+    }
+    if (sx >= P.nx) {
+        const int x1 = x + cx, x2 = x + 2 * cx;
+        if (solid[cidx(P, x1, y, z)]) return F[(size_t)oppq(q) * N + c];
+        const R f1 = F[(size_t)q * N + cidx(P, x1, y, z)];
+        if (solid[cidx(P, x2, y, z)]) return f1;
+        const R f2 = F[(size_t)q * N + cidx(P, x2, y, z)];
+        return f1 + R(0.15) * (f1 - f2);
+    }
+    const int s = cidx(P, sx, sy, sz);
+    if (solid[s]) return F[(size_t)oppq(q) * N + c];
+    return F[(size_t)q * N + s];
+}
+
+// temperature at (x, y, z) for the stencil of a cell whose own value is self:
+// periodic in y and z; outside in x or solid -> self
+template <class R>
+AD_HD R t_at(const Params& P, const R* T, const uint8_t* solid, int x, int y, int z, R self, int* idx) {
+    y = wrap(y, P.ny);
+    z = wrap(z, P.nz);
+    if (x < 0 || x >= P.nx) {
+        *idx = -1;
+        return self;
+    }
+    const int c = cidx(P, x, y, z);
+    if (solid[c]) {
+        *idx = -1;
+        return self;
+    }
+    *idx = c;
+    return T[c];
+}
+
+// first-order semi-Lagrangian advection + FTCS diffusion + source, clamped to
+// +-1 (ref: jni.cpp:2145-2188). The 15 stencil indices (8 corners, 6 neighbours,
+// the cell) and weights go to idx/w (-1: the read fell back to the cell itself).
+template <class R>
+AD_HD R temperature(const Params& P, const R* T, const uint8_t* solid, R src, int x, int y, int z, R ux, R uy, R uz,
+                    int idx[14], R w_corner[8], R* pre_clamp) {
+    const int c = cidx(P, x, y, z);
+    const R tc = T[c];
+    const R px = R(float(x)) - ux, py = R(float(y)) - uy, pz = R(float(z)) - uz;
+    const int x0 = (int)floor(val(px)), y0 = (int)floor(val(py)), z0 = (int)floor(val(pz));
+    const R fx = px - R(float(x0)), fy = py - R(float(y0)), fz = pz - R(float(z0));
+    R cv[8];
+    for (int k = 0; k < 8; ++k)
+        cv[k] = t_at(P, T, solid, x0 + (k >> 2), y0 + ((k >> 1) & 1), z0 + (k & 1), tc, &idx[k]);
+    // weights of corner k = (dx, dy, dz)
+    for (int k = 0; k < 8; ++k) {
+        const R wx = (k >> 2) ? fx : R(1) - fx;
+        const R wy = ((k >> 1) & 1) ? fy : R(1) - fy;
+        const R wz = (k & 1) ? fz : R(1) - fz;
+        w_corner[k] = wx * wy * wz;
+    }
+    R adv = R(0);
+    for (int k = 0; k < 8; ++k) adv = adv + w_corner[k] * cv[k];
+    const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    R sum = R(0);
+    for (int k = 0; k < 6; ++k) sum = sum + t_at(P, T, solid, x + nb[k][0], y + nb[k][1], z + nb[k][2], tc, &idx[8 + k]);
+    const R pre = adv + (R(P.alpha) * (sum - R(6) * tc) + src);
+    *pre_clamp = pre;
+    return rclamp(pre, -1.0f, 1.0f);
+}
+
+// heat source of cell (x, y, z): trilinear sample of the m^3 parameter field at
+// the cell centre, clamped to +-0.006; lo[3] and t[3] receive the coarse base
+// node and fractions
+AD_HD void source_weights(int n, int m, int x, float* pos) {
+    const float p = ((float)x + 0.5f) * (float)m / (float)n - 0.5f;
+    *pos = fminf(fmaxf(p, 0.0f), (float)(m - 1));
+}
+
+}  // namespace adj
+}  // namespace aero

@@ -265,3 +265,33 @@ def meso_config(nx: int, ny: int, nz: int, nu_lattice: float = 5e-4, dx: float =
     cfg.ambient_air_temperature_k = 288.15
     cfg.deep_ground_temperature_k = 289.65
     return cfg
+
+
+def adjoint_case(n: int = 64, heat_grid: int = 16, probes: int = 512, fill: float = 0.05, u_in: float = 0.04,
+                 seed: int = 21) -> dict:
+    """Inflow/heat fit of one n^3 brick: solid boxes, seeded probe cells, a hidden
+    ground-truth parameter set and the starting guess.
+
+    inflow_* are [n*n*3] float32 (y-major, z fastest, then u, v, w) on the x = 0
+    face; heat_* are [heat_grid^3] float32 lattice source rates. The truth is a
+    log-law shear profile in z with a lateral modulation and small cross
+    components; the start is the uniform inflow u_in and no heat."""
+    solid = solid_boxes(n, n, n, fill, seed, edge=(2, max(3, n // 8)), keep_clear_x=3)
+    r = splitmix_array(seed + 1, probes + heat_grid ** 3)
+    fluid = np.flatnonzero(solid.reshape(-1) == 0)
+    pick = np.unique((r[:probes] * len(fluid)).astype(np.int64))
+    extra = np.setdiff1d(fluid, fluid[pick])[: probes - len(pick)]
+    probe_cells = np.sort(np.concatenate([fluid[pick], extra])).astype(np.int32)
+    y, z = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    prof = u_in * (0.6 + 0.4 * np.log1p(z) / np.log1p(n - 1)) * (1.0 + 0.1 * np.sin(2.0 * np.pi * y / n))
+    truth = np.stack([prof, 0.004 * np.cos(2.0 * np.pi * z / n), 0.003 * np.sin(2.0 * np.pi * y / n)], axis=-1)
+    heat_truth = 0.004 * (2.0 * r[probes:] - 1.0)
+    start = np.zeros((n, n, 3))
+    start[..., 0] = u_in
+    return {
+        "n": n, "heat_grid": heat_grid, "solid": solid, "probes": probe_cells,
+        "inflow_truth": truth.reshape(-1).astype(np.float32),
+        "heat_truth": heat_truth.astype(np.float32),
+        "inflow_start": start.reshape(-1).astype(np.float32),
+        "heat_start": np.zeros(heat_grid ** 3, np.float32),
+    }

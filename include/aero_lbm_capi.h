@@ -356,6 +356,38 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_batch_copy_atlas(long long batch, int index, i
 AERO_LBM_CAPI_EXPORT const char* aero_lbm_batch_info(long long batch);
 AERO_LBM_CAPI_EXPORT void aero_lbm_batch_release(long long batch);
 
+/* ---- adjoint of the brick step (extension: no counterpart in the reference).
+ * Model: one n^3 D3Q27 cumulant brick, FP32, Boussinesq buoyancy (gravity -z),
+ * per-(y, z) velocity inlet on x- (the inflow profile, 3 floats per (y, z),
+ * index (y * n + z) * 3), convective outflow on x+, periodic y and z, halfway
+ * bounce-back at solid cells, benchmark-mode temperature update with a heat
+ * source trilinearly upsampled from a heat_grid^3 field (index (i * m + j) * m + k),
+ * no SGS term. Loss: sum over probe cells of |u - target|^2, u the collision
+ * velocity after `steps` steps. Forward stores checkpoints every
+ * checkpoint_every steps; backward replays each segment and returns dL/dinflow
+ * and dL/dheat. forward_fp64 runs the same model in FP64 (finite-difference
+ * checks). */
+typedef struct AeroLbmAdjointConfig {
+    int n;                 /* brick edge */
+    int heat_grid;         /* edge m of the heat-source parameter field */
+    int steps;
+    int checkpoint_every;
+    float nu;              /* lattice viscosity (tau = 0.5 + 3 nu) */
+    float beta;            /* Boussinesq coefficient */
+    float alpha;           /* thermal diffusivity */
+    float u0[3];           /* initial velocity */
+    int probe_count;
+} AeroLbmAdjointConfig;
+AERO_LBM_CAPI_EXPORT int aero_lbm_adjoint_create(const AeroLbmAdjointConfig* config, const uint8_t* solid,
+                                                 const int* probe_cells, const float* target_uvw,
+                                                 long long* out_handle);
+AERO_LBM_CAPI_EXPORT int aero_lbm_adjoint_forward(long long handle, const float* inflow, const float* heat,
+                                                  double* out_loss, float* out_probe_u);
+AERO_LBM_CAPI_EXPORT int aero_lbm_adjoint_backward(long long handle, float* grad_inflow, float* grad_heat);
+AERO_LBM_CAPI_EXPORT int aero_lbm_adjoint_forward_fp64(long long handle, const double* inflow, const double* heat,
+                                                       double* out_loss);
+AERO_LBM_CAPI_EXPORT void aero_lbm_adjoint_release(long long handle);
+
 /* ---- mesoscale D3Q19 step (ref: native/include/aero_lbm_capi.h:286-301) ----
  * forcing: cells x 25 floats, state: cells x 6 floats (ambient, deep ground,
  * surface, wind x, y, z), cell = (x*ny + y)*nz + z, y vertical. */

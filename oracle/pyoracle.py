@@ -250,3 +250,34 @@ class MesoOracle:
         out = np.zeros(3, dtype=self.real)
         getattr(self.lib, self.p + "meso_transport")(self.h, out.ctypes.data_as(POINTER(ct)))
         return tuple(out.tolist())
+
+
+_dual = None
+
+
+def dual_directional(n, m, steps, nu, beta, alpha, u0, solid, probes, target, inflow, heat, d_inflow, d_heat):
+    """FP64 dual-number loss and directional derivative dL/dp . d of the adjoint
+    model (oracle/libadjoint_dual.so)."""
+    global _dual
+    if _dual is None:
+        path = os.path.join(HERE, "libadjoint_dual.so")
+        if not os.path.exists(path):
+            build()
+        _dual = ctypes.CDLL(path)
+        D = POINTER(c_double)
+        _dual.ad_dual_directional.argtypes = [c_int, c_int, c_int, c_double, c_float, c_float, POINTER(c_float),
+                                              c_void_p, c_int, POINTER(c_int), D, D, D, D, D, D, D]
+        _dual.ad_dual_directional.restype = c_int
+    f64 = lambda a: np.ascontiguousarray(a, np.float64).reshape(-1)
+    D = POINTER(c_double)
+    sol = np.ascontiguousarray(np.asarray(solid, np.uint8).reshape(-1))
+    pr = np.ascontiguousarray(np.asarray(probes, np.int32))
+    u0a = np.ascontiguousarray(u0, np.float32)
+    arrs = [f64(target), f64(inflow), f64(heat), f64(d_inflow), f64(d_heat)]
+    loss, dl = c_double(0), c_double(0)
+    ok = _dual.ad_dual_directional(n, m, steps, nu, beta, alpha, u0a.ctypes.data_as(POINTER(c_float)),
+                                   sol.ctypes.data_as(c_void_p), len(pr), pr.ctypes.data_as(POINTER(c_int)),
+                                   *[a.ctypes.data_as(D) for a in arrs], ctypes.byref(loss), ctypes.byref(dl))
+    if not ok:
+        raise RuntimeError("dual directional derivative failed")
+    return loss.value, dl.value
