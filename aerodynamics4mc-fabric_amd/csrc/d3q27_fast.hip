@@ -40,27 +40,64 @@ struct TileCoord {
     bool valid;
 };
 
-// XCD-aware decode of the cell handled by this thread. Workgroups are dispatched
-// round-robin over the 8 XCDs; logical tile (L & 7) * per + (L >> 3) gives each
-// XCD one contiguous eighth of the brick.
+// Tile layout: the x planes flagged xface_bulk (whose rows run face rules, longer
+// than a bulk row) are dealt out first, an equal share to each XCD; the other
+// planes follow, each XCD taking one contiguous range of them. Workgroups are
+// dispatched round-robin over the 8 XCDs, so workgroup L runs on XCD L & 7 as its
+// (L >> 3)-th tile.
+struct TileLayout {
+    int nbp;             // tiles per x plane
+    int faces;           // flagged x planes (0, 1 or 2)
+    int fper, bper;      // face / bulk tiles per XCD
+};
+
+__host__ __device__ __forceinline__ TileLayout tile_layout(const StepParams& P) {
+    TileLayout T;
+    T.nbp = ((P.nz + kTileZ - 1) / kTileZ) * ((P.ny + kTileY - 1) / kTileY);
+    const int lo = P.xface_bulk[0] ? 1 : 0;
+    const int hi = (P.xface_bulk[1] && !(P.nx == 1 && lo)) ? 1 : 0;  // one plane counts once
+    T.faces = lo + hi;
+    T.fper = (T.faces * T.nbp + 7) >> 3;
+    T.bper = ((P.nx - T.faces) * T.nbp + 7) >> 3;
+    return T;
+}
+
 __device__ __forceinline__ TileCoord tile_cell(const StepParams& P, int L) {
     const int nbz = (P.nz + kTileZ - 1) / kTileZ;
-    const int nby = (P.ny + kTileY - 1) / kTileY;
-    const int nb = nbz * nby * P.nx;
-    const int per = (nb + 7) >> 3;
-    const int logical = (L & 7) * per + (L >> 3);
-    const int bx = logical / (nbz * nby);
-    const int r = logical - bx * nbz * nby;
-    const int by = r / nbz;
-    const int bz = r - by * nbz;
+    const TileLayout T = tile_layout(P);
+    const int xcd = L & 7, i = L >> 3;
+    int plane, tile;
+    bool ok;
+    if (i < T.fper) {
+        const int f = xcd * T.fper + i;
+        ok = f < T.faces * T.nbp;
+        plane = f / T.nbp;
+        tile = f - plane * T.nbp;
+        plane = (plane == 0 && P.xface_bulk[0]) ? 0 : P.nx - 1;
+    } else {
+        const int b = xcd * T.bper + (i - T.fper);
+        ok = b < (P.nx - T.faces) * T.nbp;
+        plane = b / T.nbp;
+        tile = b - plane * T.nbp;
+        plane += P.xface_bulk[0] ? 1 : 0;
+    }
+    const int by = tile / nbz;
+    const int bz = tile - by * nbz;
     TileCoord t;
-    t.x = bx;
+    t.x = plane;
     // a wavefront holds threads 64w..64w+63 of the 64 x 4 workgroup: one threadIdx.y
     t.y = by * kTileY + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     t.z = bz * kTileZ + (int)threadIdx.x;
-    t.row_valid = logical < nb && t.y < P.ny;
+    t.row_valid = ok && t.y < P.ny;
     t.valid = t.row_valid && t.z < P.nz;
     return t;
+}
+
+// The tile L lies in an x face plane flagged xface_bulk: its workgroup takes the
+// instantiation of the step body with the face rules, every other workgroup the
+// one without them (separate register allocation for the two paths).
+__device__ __forceinline__ bool xface_tile(const StepParams& P, int L) {
+    return (L >> 3) < tile_layout(P).fper;
 }
 
 // Wave-uniform neighbour rows of the wave's (x, y) row: element offset of row
@@ -117,22 +154,37 @@ __device__ __forceinline__ int plane_row(int N, int q, int row_off) {
 }
 
 __host__ inline int tile_grid(const StepParams& P) {
-    const int nbz = (P.nz + kTileZ - 1) / kTileZ;
-    const int nby = (P.ny + kTileY - 1) / kTileY;
-    const int nb = nbz * nby * P.nx;
-    return 8 * ((nb + 7) >> 3);
+    const TileLayout T = tile_layout(P);
+    return 8 * (T.fper + T.bper);
 }
 
 // link whose pull source is the neighbour at x + d
 __device__ constexpr int link_from(int dx, int dy, int dz) { return ((1 - dx) * 3 + (1 - dy)) * 3 + (1 - dz); }
 
+// face temperature of a stencil point outside the domain (ref: jni.cpp:2145-2188
+// with the face rules of t_at): Dirichlet value, self + flux, or self
+__device__ __forceinline__ float face_temperature(const FaceParams& fc, float self) {
+    return fc.thermal_kind == 4 ? clampf(fc.temperature, -1.0f, 1.0f)
+                                : (fc.thermal_kind == 5 ? clampf(self + fc.heat_flux, -1.0f, 1.0f) : self);
+}
+
 // first-order semi-Lagrangian advection + explicit diffusion + source
 // (ref: native/src/aero_lbm_jni.cpp:2145-2188). Solid or out-of-domain stencil
-// points take the cell's own value; the link mask says which they are.
+// points take the cell's own value; the link mask says which they are. XFace: the
+// cell lies in an x face plane flagged xface_bulk (xlo / xhi, wave-uniform), whose
+// points outside the domain take that face's temperature rule.
+template <bool XFace>
 __device__ __forceinline__ float temperature_fast(const StepParams& P, const float* __restrict__ T,
                                                   const float* __restrict__ src, int cell, int x, int y, int z,
-                                                  const Nbr& nb, uint32_t links, float ux, float uy, float uz) {
+                                                  const Nbr& nb, uint32_t links, float ux, float uy, float uz,
+                                                  bool xlo = false, bool xhi = false) {
     const float tc = T[cell];
+    float tlo = tc, thi = tc;
+    if (XFace) {
+        const FaceParams* fp = P.self_dev->faces_dev;
+        if (xlo) tlo = face_temperature(fp[0], tc);
+        if (xhi) thi = face_temperature(fp[1], tc);
+    }
     const float px = (float)x - ux, py = (float)y - uy, pz = (float)z - uz;
     const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
     const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
@@ -141,7 +193,9 @@ __device__ __forceinline__ float temperature_fast(const StepParams& P, const flo
     const int az = fz0 < (float)z ? -1 : 0;
     auto tv = [&](int dx, int dy, int dz) -> float {
         const bool sp = (links >> link_from(dx, dy, dz)) & 1u;
-        return sp ? tc : T[cell + nb.off(dx, dy, dz)];
+        const float v = sp ? tc : T[cell + nb.off(dx, dy, dz)];
+        if (XFace) return (xlo && dx < 0) ? tlo : ((xhi && dx > 0) ? thi : v);
+        return v;
     };
     const float c000 = tv(ax, ay, az), c100 = tv(ax + 1, ay, az);
     const float c010 = tv(ax, ay + 1, az), c110 = tv(ax + 1, ay + 1, az);
@@ -188,19 +242,20 @@ __device__ __forceinline__ void face_store(const StepParams& P, int x, int y, in
 // local_first: the state is the initial one (incoming values in own slots, no
 // stream pending): collide in place.
 //
-// Cells at faces whose rule is not plain bounce-back, next to Dirichlet / Neumann
-// thermal faces, with sponge or face-store duties, and all cells when the thermal
-// periodicity differs from the hydrodynamic one carry kMaskRare and are left to
-// the rare-cell kernel; here every special link is halfway bounce-back.
+// The x face planes flagged xface_bulk run their face rules here. Other cells at
+// faces whose rule is not plain bounce-back or next to Dirichlet / Neumann thermal
+// faces, cells the sponge touches, and all cells when the thermal periodicity
+// differs from the hydrodynamic one carry kMaskRare and are left to the rare-cell
+// kernels; for the rest every special link is halfway bounce-back.
 // L: the workgroup's tile index within the brick. Shell: the context has a
 // boundary-reference shell (a separate instantiation, so contexts without one keep
 // the shorter kernel and its register budget).
-template <class St, class Uv, int Mode, int StPol, bool Shell>
+template <class St, class Uv, int Mode, int StPol, bool Shell, bool XF>
 __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields& C, const uint32_t* __restrict__ mask,
                                           const typename St::T* __restrict__ fin, typename St::T* fout,
                                           const float* __restrict__ t_in, float* __restrict__ t_out,
-                                          typename Uv::T* __restrict__ u_state, float* __restrict__ store_out,
-                                          int local_first, int L) {
+                                          typename Uv::T* __restrict__ u_state, const float* __restrict__ store_in,
+                                          float* __restrict__ store_out, int local_first, int L) {
     const TileCoord tc = tile_cell(P, L);
     if (!tc.valid) return;
     const int x = tc.x, y = tc.y, z = tc.z;
@@ -240,13 +295,30 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
             g[q] = St::template bld<kPolDefault>(rin, voff, so);
         }
     }
+    // x face planes flagged xface_bulk (inlet, outflow, symmetry, Dirichlet / Neumann
+    // temperature): the 9 links pulled from outside the domain take the face rule,
+    // with the loaded value (the cell's own opposite slot) as the bounce-back
+    // candidate. x is wave-uniform, so only the waves of these planes branch here.
+    const bool xlo = XF && x == 0 && P.xface_bulk[0], xhi = XF && x == P.nx - 1 && P.xface_bulk[1];
+    if (XF && !local_first && (xlo || xhi)) {
+        const StepParams& Q = *P.self_dev;
+        const PrevPost<St, Mode != 0> prev{fin, store_in, (size_t)Q.plane};
+        const float rp = C.ref_p[cell];
+        if (xlo) xface_links<0, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
+        if (xhi) xface_links<1, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
+    }
     float temp = 0.0f;
     float ux0 = 0.0f, uy0 = 0.0f, uz0 = 0.0f;
     if (P.thermal_on) {
         ux0 = Uv::ld(u_state + cell);
         uy0 = Uv::ld(u_state + N + cell);
         uz0 = Uv::ld(u_state + 2 * N + cell);
-        temp = temperature_fast(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0);
+        if (XF && (xlo || xhi))
+            temp = temperature_fast<XF>(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0,
+                                          xlo, xhi);
+        else
+            temp = temperature_fast<false>(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0,
+                                           uz0);
     }
     CollideResult r = collide<St::kShifted>(P, g, temp);
     if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
@@ -289,10 +361,15 @@ template <class St, class Uv, int Mode, int StPol, bool Shell>
 __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
                                                    const typename St::T* __restrict__ fin, typename St::T* fout,
                                                    const float* __restrict__ t_in, float* __restrict__ t_out,
-                                                   typename Uv::T* __restrict__ u_state, float* __restrict__ store_out,
+                                                   typename Uv::T* __restrict__ u_state,
+                                                   const float* __restrict__ store_in, float* __restrict__ store_out,
                                                    int local_first) {
-    bulk_body<St, Uv, Mode, StPol, Shell>(P, C, mask, fin, fout, t_in, t_out, u_state, store_out, local_first,
-                                          blockIdx.x);
+    if (xface_tile(P, blockIdx.x))
+        bulk_body<St, Uv, Mode, StPol, Shell, true>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
+                                                    store_out, local_first, blockIdx.x);
+    else
+        bulk_body<St, Uv, Mode, StPol, Shell, false>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
+                                                     store_out, local_first, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- rare-cell kernels
@@ -328,7 +405,7 @@ __device__ __forceinline__ void gather_body(const StepParams& P, const CellField
         if ((m & kMaskTherm) || !P.thermal_wrap_matches)
             v = temperature_step(P, t_in, C.flags, C.src, cell, x, y, z, nb, false, ux0, uy0, uz0);
         else
-            v = temperature_fast(P, t_in, C.src, cell, x, y, z, nb, links, ux0, uy0, uz0);
+            v = temperature_fast<false>(P, t_in, C.src, cell, x, y, z, nb, links, ux0, uy0, uz0);
     } else {
         const typename St::T* const src = Mode == 0 ? fin : fout;
         const bool sp = (links >> q) & 1u;
@@ -525,9 +602,13 @@ __global__ void __launch_bounds__(256) step_kernel_batch(const BatchBrick* __res
     using T = typename St::T;
     const int b = __builtin_amdgcn_readfirstlane((int)owner[blockIdx.x]);
     const BatchBrick& B = table[b];
-    bulk_body<St, Uv, Mode, StPol, Shell>(B.P, B.C, B.mask, (const T*)B.f, (T*)B.f, B.t_in, B.t_out,
-                                          (typename Uv::T*)B.u_state, B.store_out, 0,
-                                          (int)blockIdx.x - B.bulk_begin);
+    const int L = (int)blockIdx.x - B.bulk_begin;
+    if (xface_tile(B.P, L))
+        bulk_body<St, Uv, Mode, StPol, Shell, true>(B.P, B.C, B.mask, (const T*)B.f, (T*)B.f, B.t_in, B.t_out,
+                                                    (typename Uv::T*)B.u_state, B.store_in, B.store_out, 0, L);
+    else
+        bulk_body<St, Uv, Mode, StPol, Shell, false>(B.P, B.C, B.mask, (const T*)B.f, (T*)B.f, B.t_in, B.t_out,
+                                                     (typename Uv::T*)B.u_state, B.store_in, B.store_out, 0, L);
 }
 
 template <class St, class Uv, int Mode>
@@ -631,6 +712,7 @@ __global__ void __launch_bounds__(256) mask_kernel(StepParams P, const uint8_t* 
     const int y = rem / P.nz;
     const int z = rem - y * P.nz;
     uint32_t m = 0;
+    bool special_elsewhere = false;  // a face rule the bulk kernel does not run
     for (int q = 0; q < 27; ++q) {
         const int sx = wrapc(x - lat_cx(q), P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
         const int sy = wrapc(y - lat_cy(q), P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);
@@ -638,18 +720,22 @@ __global__ void __launch_bounds__(256) mask_kernel(StepParams P, const uint8_t* 
         const bool oob = sx < 0 || sy < 0 || sz < 0 || sx >= P.nx || sy >= P.ny || sz >= P.nz;
         if (oob) {
             m |= 1u << q;
-            const FaceParams fc = P.faces_dev[oob_face(P, sx, sy, sz)];
+            const int face = oob_face(P, sx, sy, sz);
+            const FaceParams fc = P.faces_dev[face];
             const int k = fc.hydro_kind;
             const bool plain_bounce = k == 2 || (k == 0 && P.convective_disabled);
             if (!plain_bounce) m |= kMaskFace;
             // temperature stencil point x - c_q out of the domain (thermal wrap equals hydro wrap
             // whenever this mask drives the fast stencil)
-            if (fc.thermal_kind == 4 || fc.thermal_kind == 5) m |= kMaskTherm;
+            const bool therm = fc.thermal_kind == 4 || fc.thermal_kind == 5;
+            if (therm) m |= kMaskTherm;
+            // links leaving through an x face flagged xface_bulk are the bulk kernel's
+            if ((!plain_bounce || therm) && !(face < 2 && P.xface_bulk[face])) special_elsewhere = true;
         } else if (flags[(sx * P.ny + sy) * P.nz + sz] & kCellSolid) {
             m |= 1u << q;
         }
     }
-    bool rare = !P.thermal_wrap_matches || (m & (kMaskFace | kMaskTherm));
+    bool rare = !P.thermal_wrap_matches || special_elsewhere;
     const int d = min(min(min(x, y), z), min(min(P.nx - 1 - x, P.ny - 1 - y), P.nz - 1 - z));
     if (P.sponge_on && d < 4) rare = true;
     mask[cell] = m | (rare ? kMaskRare : 0u);
@@ -769,16 +855,17 @@ static int store_policy(size_t working_set) {
 
 template <class St, class Uv, int Mode, int StPol>
 static void launch_bulk(const StepParams& P, const CellFields& C, const uint32_t* mask, const void* fin, void* fout,
-                        const float* t_in, float* t_out, void* u_state, float* store_out, int lf, hipStream_t s) {
+                        const float* t_in, float* t_out, void* u_state, const float* store_in, float* store_out,
+                        int lf, hipStream_t s) {
     using T = typename St::T;
     using U = typename Uv::T;
     const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
     if (P.shell_on)
         hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, true>), grid, block, 0, s, P, C, mask,
-                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
+                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
     else
         hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, false>), grid, block, 0, s, P, C, mask,
-                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_out, lf);
+                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
 }
 
 template <class St, class Uv, int Mode, int StPol>
@@ -798,7 +885,7 @@ static void launch_step_m(const StepParams& P, const CellFields& C, const uint32
         hipLaunchKernelGGL((fast::rare_update_kernel<St, Uv, Mode>), ugrid, dim3(128), 0, s, P, C, mask, (T*)fout,
                            t_out, (U*)u_state, store_out, rare.list, rare.count, rare.cap, (const float*)rare.vals);
     }
-    launch_bulk<St, Uv, Mode, StPol>(P, C, mask, fin, fout, t_in, t_out, u_state, store_out, lf, s);
+    launch_bulk<St, Uv, Mode, StPol>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, lf, s);
 }
 
 template <class St, class Uv, int StPol>

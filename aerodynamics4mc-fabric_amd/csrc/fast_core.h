@@ -241,6 +241,69 @@ __device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* f
     return kind == 2 ? bounce : (kind >= 3 && kind <= 5) ? eq : (kind == 7 ? sym : conv);
 }
 
+// The 9 links of an x-face cell pulled from outside the domain (Side 0: x_min,
+// links with c_x = +1; Side 1: x_max, c_x = -1), for the bulk kernel. The same
+// rules and arithmetic as face_link, with the face's rule (wave-uniform) chosen
+// first so only that rule's values are loaded: equilibrium faces load nothing,
+// convective and symmetry faces load their two (one) previous-step values per
+// link. g[q] holds the bounce-back candidate on entry.
+template <int Side, class St, bool AA>
+__device__ __forceinline__ void xface_links(const StepParams& P, const uint8_t* __restrict__ flags, float ref_p_cell,
+                                            const PrevPost<St, AA>& prev, int x, int y, int z, float g[27]) {
+    constexpr int q0 = Side == 0 ? 18 : 0;
+    constexpr int cx = Side == 0 ? 1 : -1;
+    const FaceParams& fc = P.faces_dev[Side];
+    int kind = fc.hydro_kind;
+    if (kind == 0 || kind == 1) kind = P.convective_disabled && kind == 0 ? 2 : 6;
+    if (kind == 2) return;
+    if (kind >= 3 && kind <= 5) {
+        float rho = 1.0f + clampf(ref_p_cell, -0.03f, 0.03f);
+        float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+        if (kind != 5) {
+            ux = fc.vel[0];
+            uy = fc.vel[1];
+            uz = fc.vel[2];
+        } else {
+            rho = 1.0f + clampf(fc.pressure, -0.03f, 0.03f);
+        }
+        rho = clampf(rho, 0.97f, 1.03f);
+        const float s2v = ux * ux + uy * uy + uz * uz;
+        if (s2v > P.kmax_speed * P.kmax_speed && s2v > 0.0f) {
+            const float sc = P.kmax_speed / sqrtf(s2v);
+            ux *= sc;
+            uy *= sc;
+            uz *= sc;
+        }
+#pragma unroll
+        for (int j = 0; j < 9; ++j) g[q0 + j] = feq_s<St::kShifted>(q0 + j, rho, ux, uy, uz);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int q = q0 + j;
+        const int cy = lat_cy(q), cz = lat_cz(q);
+        const int sy = wrapc(y - cy, P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);
+        const int sz = wrapc(z - cz, P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]);
+        if (kind == 7) {
+            const int iy = icl(sy, 0, P.ny - 1), iz = icl(sz, 0, P.nz - 1);
+            const bool s0 = flags[(x * P.ny + iy) * P.nz + iz] & kCellSolid;
+            const float p0 = prev.get(P, Side, q, x, iy, iz);
+            g[q] = s0 ? g[q] : p0;
+        } else {
+            const int dy = (sy < 0 || sy >= P.ny) ? cy : 0;
+            const int dz = (sz < 0 || sz >= P.nz) ? cz : 0;
+            const int x1 = icl(x + cx, 0, P.nx - 1), y1 = icl(y + dy, 0, P.ny - 1), z1 = icl(z + dz, 0, P.nz - 1);
+            const int x2 = icl(x + 2 * cx, 0, P.nx - 1), y2 = icl(y + 2 * dy, 0, P.ny - 1),
+                      z2 = icl(z + 2 * dz, 0, P.nz - 1);
+            const bool s1 = flags[(x1 * P.ny + y1) * P.nz + z1] & kCellSolid;
+            const bool s2 = flags[(x2 * P.ny + y2) * P.nz + z2] & kCellSolid;
+            const float p1 = prev.get(P, Side, q, x1, y1, z1);
+            const float p2 = prev.get(P, Side, q, x2, y2, z2);
+            g[q] = s1 ? g[q] : (s2 ? p1 : p1 + 0.15f * (p1 - p2));
+        }
+    }
+}
+
 __device__ __forceinline__ bool link_oob(const StepParams& P, int x, int y, int z, int q) {
     const int sx = wrapc(x - lat_cx(q), P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
     const int sy = wrapc(y - lat_cy(q), P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);

@@ -14,8 +14,9 @@ constexpr uint32_t kMaskSolid = 1u << 27;  // the cell itself is solid
 constexpr uint32_t kMaskFace = 1u << 28;   // a link leaves through a face whose rule is not plain bounce-back
 constexpr uint32_t kMaskTherm = 1u << 29;  // a stencil neighbour is outside the domain at a Dirichlet/Neumann thermal face
 // the cell is updated by the rare-cell kernel: a face rule other than plain
-// bounce-back or a Dirichlet / Neumann thermal face applies, the sponge touches
-// it, or the thermal periodicity differs from the hydrodynamic one
+// bounce-back or a Dirichlet / Neumann thermal face applies (except through the
+// x faces flagged StepParams::xface_bulk, whose planes the bulk kernel updates),
+// the sponge touches it, or the thermal periodicity differs from the hydrodynamic one
 constexpr uint32_t kMaskRare = 1u << 30;
 
 // largest population array the fast step kernel addresses (one buffer resource,

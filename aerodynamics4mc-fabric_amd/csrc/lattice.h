@@ -74,6 +74,11 @@ struct StepParams {
     // thermal periodic flags equal the hydrodynamic ones, so a cell whose 26
     // neighbours are fluid and in the domain can read its temperature stencil directly
     int thermal_wrap_matches;
+    // x_min / x_max plane: its fluid cells take the face's rules in the bulk step
+    // kernel (the face is not periodic and its hydrodynamic rule is not plain
+    // bounce-back, or its thermal rule is Dirichlet / Neumann). A plane of x is a
+    // set of whole wavefront rows, so the rules run as a wave-uniform branch.
+    int xface_bulk[2];
     // device copy of this whole struct: rarely taken kernel paths read their
     // parameters from it, so the bulk path keeps no scalar registers for them
     const StepParams* self_dev;

@@ -325,6 +325,14 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
     for (int a = 0; a < 3; ++a)
         if (P.thermal_wrap_lo[a] != P.hydro_wrap_lo[a] || P.thermal_wrap_hi[a] != P.hydro_wrap_hi[a])
             P.thermal_wrap_matches = 0;
+    for (int side = 0; side < 2; ++side) {
+        const bool wraps = side ? P.hydro_wrap_hi[0] : P.hydro_wrap_lo[0];
+        const int k = P.face[side].hydro_kind, tk = P.face[side].thermal_kind;
+        const bool plain_bounce = k == AERO_LBM_HYDRO_BOUNDARY_BOUNCE_BACK ||
+                                  (k == AERO_LBM_HYDRO_BOUNDARY_INHERIT_GAME && P.convective_disabled);
+        const bool therm = tk == AERO_LBM_THERMAL_BOUNDARY_TEMPERATURE_DIRICHLET || tk == AERO_LBM_THERMAL_BOUNDARY_HEAT_FLUX_NEUMANN;
+        P.xface_bulk[side] = !wraps && (!plain_bounce || therm) ? 1 : 0;
+    }
     return true;
 }
 

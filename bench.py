@@ -242,6 +242,56 @@ def run_batched(steps, warmup, per_brick_steps=40):
     return out
 
 
+def run_adjoint(reps=3):
+    """Inflow/heat fit of one 64^3 brick (FP32): 200-step forward with checkpoints
+    every 20 steps, backward over the 64x64x3 inflow profile and the 16^3 heat
+    source, loss over 512 probes against velocities of a hidden parameter set.
+    Timed: forward and backward at the starting guess (best of `reps`). Checked:
+    the gradient against FP64 central differences on 32 random unit directions."""
+    import aero_lbm as A
+    import workloads as W
+    case = W.adjoint_case(64, 16, 512)
+    steps, every = 200, 20
+    nu, beta, alpha = 5e-3, 0.1, 0.035
+    mk = lambda tgt: A.Adjoint(64, 16, steps, every, nu, beta, alpha, (0.04, 0.0, 0.0), case["solid"],
+                               case["probes"], tgt)
+    m0 = mk(np.zeros(3 * 512, np.float32))
+    _, u = m0.forward(case["inflow_truth"], case["heat_truth"])
+    m0.release()
+    m = mk(u.reshape(-1))
+    inflow, heat = case["inflow_start"], case["heat_start"]
+    fwd, bwd = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        loss, _ = m.forward(inflow, heat)
+        t1 = time.perf_counter()
+        gi, gh = m.backward()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    g = np.concatenate([gi, gh]).astype(np.float64)
+    p = np.concatenate([inflow, heat]).astype(np.float64)
+    ni = inflow.size
+    rng = np.random.default_rng(11)
+    eps, rel = 1e-5, []
+    for _ in range(32):
+        d = rng.standard_normal(p.size)
+        d /= np.linalg.norm(d)
+        fd = (m.forward_fp64(p[:ni] + eps * d[:ni], p[ni:] + eps * d[ni:]) -
+              m.forward_fp64(p[:ni] - eps * d[:ni], p[ni:] - eps * d[ni:])) / (2 * eps)
+        rel.append(abs(float(np.dot(g, d)) - fd) / abs(fd))
+    m.release()
+    cells = 64 ** 3
+    return {"workload": "adjoint_fit_64^3_200steps_ckpt20_inflow64x64x3_heat16^3_probes512_fp32",
+            "parameters": int(p.size), "loss_at_start": loss,
+            "forward_ms": min(fwd) * 1e3, "backward_ms": min(bwd) * 1e3,
+            "gradient_ms": (min(fwd) + min(bwd)) * 1e3,
+            "forward_glups": cells * steps / min(fwd) / 1e9,
+            "backward_glups": cells * steps / min(bwd) / 1e9,
+            "fd_check": {"directions": 32, "eps": eps, "max_rel": max(rel), "median_rel": float(np.median(rel)),
+                         "reference": "fp64 forward, central differences"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +304,7 @@ def main():
     ap.add_argument("--meso-steps", type=int, default=500)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-bricks line")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-fit line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) five times, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -331,6 +382,12 @@ def main():
         print(f"bench: batched {line['batched']['glups']:.2f} GLUPS vs per-brick "
               f"{line['batched']['per_brick_glups']:.2f} ({line['batched']['speedup_vs_per_brick']:.2f}x); "
               f"{line['batched']['launch']}", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        print("bench: adjoint fit 64^3 x 200 steps", file=sys.stderr, flush=True)
+        line["adjoint"] = run_adjoint()
+        print(f"bench: adjoint forward {line['adjoint']['forward_ms']:.1f} ms, backward "
+              f"{line['adjoint']['backward_ms']:.1f} ms, fd max rel {line['adjoint']['fd_check']['max_rel']:.2e}",
+              file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 
