@@ -188,24 +188,46 @@ __device__ __forceinline__ float temperature_fast(const StepParams& P, const flo
     const float px = (float)x - ux, py = (float)y - uy, pz = (float)z - uz;
     const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
     const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
-    const int ax = fx0 < (float)x ? -1 : 0;
-    const int ay = fy0 < (float)y ? -1 : 0;
-    const int az = fz0 < (float)z ? -1 : 0;
-    auto tv = [&](int dx, int dy, int dz) -> float {
-        const bool sp = (links >> link_from(dx, dy, dz)) & 1u;
-        const float v = sp ? tc : T[cell + nb.off(dx, dy, dz)];
-        if (XFace) return (xlo && dx < 0) ? tlo : ((xhi && dx > 0) ? thi : v);
+    const bool bx = fx0 < (float)x, by = fy0 < (float)y, bz = fz0 < (float)z;  // box starts at -1 on that axis
+    // The 2x2x2 box of the departure point starts at (ax, ay, az) in {-1, 0}^3 relative
+    // to the cell. Offsets of its two layers along each axis (periodic wrap from nb;
+    // a neighbour outside the domain maps to the cell, so every index stays in bounds).
+    const int X0 = bx ? nb.xm : 0, X1 = bx ? 0 : nb.xp;
+    const int Y0 = by ? nb.ym : 0, Y1 = by ? 0 : nb.yp;
+    const int Z0 = bz ? nb.zm : 0, Z1 = bz ? 0 : nb.zp;
+    // corner (i, j, k) is link link_from(ax + i, ay + j, az + k) = base - (9 i + 3 j + k),
+    // base in [13, 26]: after the shift its bit sits at 13 - (9 i + 3 j + k)
+    const int base = 26 - (bx ? 0 : 9) - (by ? 0 : 3) - (bz ? 0 : 1);
+    const uint32_t sh = links >> (base - 13);
+    // a solid or outside point reads the cell itself (the index is selected, not the
+    // loaded value, so every load is unconditional)
+    auto corner = [&](int i, int j, int k, int off) -> float {
+        const bool sp = (sh >> (13 - 9 * i - 3 * j - k)) & 1u;
+        const float v = T[sp ? cell : cell + off];
+        if (XFace) {
+            const bool below = bx && i == 0, above = !bx && i == 1;  // x - 1 / x + 1 layer
+            return (xlo && below) ? tlo : ((xhi && above) ? thi : v);
+        }
         return v;
     };
-    const float c000 = tv(ax, ay, az), c100 = tv(ax + 1, ay, az);
-    const float c010 = tv(ax, ay + 1, az), c110 = tv(ax + 1, ay + 1, az);
-    const float c001 = tv(ax, ay, az + 1), c101 = tv(ax + 1, ay, az + 1);
-    const float c011 = tv(ax, ay + 1, az + 1), c111 = tv(ax + 1, ay + 1, az + 1);
+    const int o00 = X0 + Y0, o10 = X1 + Y0, o01 = X0 + Y1, o11 = X1 + Y1;
+    const float c000 = corner(0, 0, 0, o00 + Z0), c100 = corner(1, 0, 0, o10 + Z0);
+    const float c010 = corner(0, 1, 0, o01 + Z0), c110 = corner(1, 1, 0, o11 + Z0);
+    const float c001 = corner(0, 0, 1, o00 + Z1), c101 = corner(1, 0, 1, o10 + Z1);
+    const float c011 = corner(0, 1, 1, o01 + Z1), c111 = corner(1, 1, 1, o11 + Z1);
     const float c00 = fmaf(c100 - c000, fx, c000), c10 = fmaf(c110 - c010, fx, c010);
     const float c01 = fmaf(c101 - c001, fx, c001), c11 = fmaf(c111 - c011, fx, c011);
     const float c0 = fmaf(c10 - c00, fy, c00), c1 = fmaf(c11 - c01, fy, c01);
     const float adv = fmaf(c1 - c0, fz, c0);
-    const float tsum = (tv(1, 0, 0) + tv(-1, 0, 0)) + (tv(0, 1, 0) + tv(0, -1, 0)) + (tv(0, 0, 1) + tv(0, 0, -1));
+    // the six face neighbours (links 4, 22, 10, 16, 12, 14 pull from +x, -x, +y, -y, +z, -z)
+    auto face = [&](int bit, int off, int dx) -> float {
+        const bool sp = (links >> bit) & 1u;
+        const float v = T[sp ? cell : cell + off];
+        if (XFace) return (xlo && dx < 0) ? tlo : ((xhi && dx > 0) ? thi : v);
+        return v;
+    };
+    const float tsum = (face(4, nb.xp, 1) + face(22, nb.xm, -1)) + (face(10, nb.yp, 0) + face(16, nb.ym, 0)) +
+                       (face(12, nb.zp, 0) + face(14, nb.zm, 0));
     const float lap = fmaf(-6.0f, tc, tsum);
     const float source = (P.source_on && P.in_ch > 9) ? clampf(src[cell], -0.006f, 0.006f) : 0.0f;
     return clampf(adv + fmaf(P.alpha, lap, source), -1.0f, 1.0f);
@@ -272,6 +294,22 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     const int Ni = P.plane;
     const Rsrc rin = make_rsrc(src, 27 * Ni * (int)sizeof(T));
     const Rsrc rout = make_rsrc(fout, 27 * Ni * (int)sizeof(T));
+    const bool xlo = XF && x == 0 && P.xface_bulk[0], xhi = XF && x == P.nx - 1 && P.xface_bulk[1];
+    // temperature first: its stencil loads and arithmetic finish before the 27
+    // population loads, which keeps the two sets of values from being live together
+    float temp = 0.0f;
+    float ux0 = 0.0f, uy0 = 0.0f, uz0 = 0.0f;
+    if (P.thermal_on) {
+        ux0 = Uv::ld(u_state + cell);
+        uy0 = Uv::ld(u_state + N + cell);
+        uz0 = Uv::ld(u_state + 2 * N + cell);
+        if (XF && (xlo || xhi))
+            temp = temperature_fast<XF>(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0,
+                                          xlo, xhi);
+        else
+            temp = temperature_fast<false>(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0,
+                                           uz0);
+    }
     float g[27];
     if (Mode == 1 || local_first) {
 #pragma unroll
@@ -299,26 +337,12 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     // temperature): the 9 links pulled from outside the domain take the face rule,
     // with the loaded value (the cell's own opposite slot) as the bounce-back
     // candidate. x is wave-uniform, so only the waves of these planes branch here.
-    const bool xlo = XF && x == 0 && P.xface_bulk[0], xhi = XF && x == P.nx - 1 && P.xface_bulk[1];
     if (XF && !local_first && (xlo || xhi)) {
         const StepParams& Q = *P.self_dev;
         const PrevPost<St, Mode != 0> prev{fin, store_in, (size_t)Q.plane};
         const float rp = C.ref_p[cell];
         if (xlo) xface_links<0, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
         if (xhi) xface_links<1, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
-    }
-    float temp = 0.0f;
-    float ux0 = 0.0f, uy0 = 0.0f, uz0 = 0.0f;
-    if (P.thermal_on) {
-        ux0 = Uv::ld(u_state + cell);
-        uy0 = Uv::ld(u_state + N + cell);
-        uz0 = Uv::ld(u_state + 2 * N + cell);
-        if (XF && (xlo || xhi))
-            temp = temperature_fast<XF>(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0,
-                                          xlo, xhi);
-        else
-            temp = temperature_fast<false>(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0,
-                                           uz0);
     }
     CollideResult r = collide<St::kShifted>(P, g, temp);
     if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
@@ -634,10 +658,12 @@ __global__ void __launch_bounds__(128) rare_update_batch(const BatchBrick* __res
                               ((int)blockIdx.x - B.update_begin) * 128 + (int)threadIdx.x);
 }
 
-// lround of a float (halves away from zero) through double, exact for every float
+// lround of a float (halves away from zero) through double, exact for every float.
+// The division runs in double and rounds once to float, which equals the correctly
+// rounded float quotient (this file's float divisions are approximate).
 __device__ __forceinline__ int16_t quantize_i16(float v, float range) {
     if (!isfinite(v)) return 0;
-    const float t = fminf(fmaxf(v / range, -1.0f), 1.0f) * 32767.0f;
+    const float t = fminf(fmaxf((float)((double)v / (double)range), -1.0f), 1.0f) * 32767.0f;
     const double d = (double)t;
     return (int16_t)(d >= 0.0 ? floor(d + 0.5) : ceil(d - 0.5));
 }

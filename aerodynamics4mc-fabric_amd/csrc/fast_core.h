@@ -387,104 +387,50 @@ __device__ __forceinline__ float temperature_step(const StepParams& P, const flo
 }
 
 // ---------------------------------------------------------------- collision
-__device__ __forceinline__ void fwd_line(float& a, float& b, float& c) {
-    // populations (c=-1, c=0, c=+1) -> raw moments (order 0, 1, 2)
-    const float s = a + c, t = c - a;
-    a = s + b;
-    b = t;
-    c = s;
-}
-__device__ __forceinline__ void shift_line(float& m0, float& m1, float& m2, float u, float u2) {
-    // raw -> central about u
-    const float k1 = fmaf(-u, m0, m1);
-    m2 = fmaf(-2.0f * u, m1, fmaf(u2, m0, m2));
-    m1 = k1;
-}
-__device__ __forceinline__ void unshift_line(float& k0, float& k1, float& k2, float u, float u2) {
-    // central about u -> raw
-    k2 = fmaf(2.0f * u, k1, fmaf(u2, k0, k2));
-    k1 = fmaf(u, k0, k1);
-}
-__device__ __forceinline__ void inv_line(float& m0, float& m1, float& m2) {
-    // raw moments -> populations (c=-1, c=0, c=+1)
-    const float h2 = 0.5f * m2, h1 = 0.5f * m1;
-    const float fm = h2 - h1, fp = h2 + h1;
-    m0 = m0 - m2;
-    m1 = fm;  // temporarily c=-1 in slot 1
-    m2 = fp;
-}
-
 #define IDX(a, b, c) (((a) * 3 + (b)) * 3 + (c))
-
-__device__ __forceinline__ void forward_raw(float f[27]) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) fwd_line(f[i * 3 + 0], f[i * 3 + 1], f[i * 3 + 2]);  // z
-#pragma unroll
-    for (int ix = 0; ix < 3; ++ix)
-#pragma unroll
-        for (int kz = 0; kz < 3; ++kz) fwd_line(f[IDX(ix, 0, kz)], f[IDX(ix, 1, kz)], f[IDX(ix, 2, kz)]);  // y
-#pragma unroll
-    for (int ky = 0; ky < 3; ++ky)
-#pragma unroll
-        for (int kz = 0; kz < 3; ++kz) fwd_line(f[IDX(0, ky, kz)], f[IDX(1, ky, kz)], f[IDX(2, ky, kz)]);  // x
-}
-
-__device__ __forceinline__ void inverse_raw(float m[27]) {
-#pragma unroll
-    for (int ky = 0; ky < 3; ++ky)
-#pragma unroll
-        for (int kz = 0; kz < 3; ++kz) {
-            inv_line(m[IDX(0, ky, kz)], m[IDX(1, ky, kz)], m[IDX(2, ky, kz)]);
-            const float t = m[IDX(0, ky, kz)];  // c=0 value
-            m[IDX(0, ky, kz)] = m[IDX(1, ky, kz)];
-            m[IDX(1, ky, kz)] = t;
-        }
-#pragma unroll
-    for (int ix = 0; ix < 3; ++ix)
-#pragma unroll
-        for (int kz = 0; kz < 3; ++kz) {
-            inv_line(m[IDX(ix, 0, kz)], m[IDX(ix, 1, kz)], m[IDX(ix, 2, kz)]);
-            const float t = m[IDX(ix, 0, kz)];
-            m[IDX(ix, 0, kz)] = m[IDX(ix, 1, kz)];
-            m[IDX(ix, 1, kz)] = t;
-        }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        inv_line(m[i * 3 + 0], m[i * 3 + 1], m[i * 3 + 2]);
-        const float t = m[i * 3 + 0];
-        m[i * 3 + 0] = m[i * 3 + 1];
-        m[i * 3 + 1] = t;
-    }
-}
 
 // 1-D lattice moments of the weights: sum_c w(c) c^n = 1, 0, 1/3, 0, 1/3 for n = 0..4
 __device__ constexpr float mu1(int n) { return n == 0 ? 1.0f : ((n & 1) ? 0.0f : 1.0f / 3.0f); }
 __device__ constexpr float emom(int a, int b, int c) { return mu1(a) * mu1(b) * mu1(c); }
 
-// raw moment (a,b,c) of the Guo source S_q = w_q [3 (c-u).F + 9 (c.u)(c.F)] / P
-template <int A, int B, int C>
-__device__ __forceinline__ float guo_moment(const float F[3], float uF, const float uaFb[3][3]) {
+// 2^-(number of non-zero indices): the scale of moment (a, b, c) in the halved form
+__device__ constexpr float half_scale(int a, int b, int c) {
+    return (a ? 0.5f : 1.0f) * (b ? 0.5f : 1.0f) * (c ? 0.5f : 1.0f);
+}
+
+// raw moment (a,b,c) of the Guo source S_q = w_q [3 (c-u).F + 9 (c.u)(c.F)] / P for a
+// force fa along axis Ax: uFa = u_Ax fa, ua_fa[al] = u_al fa (terms of the other
+// force components vanish and are left out)
+template <int A, int B, int C, int Ax>
+__device__ __forceinline__ float guo_moment(float fa, float uFa, const float ua_fa[3]) {
     float v = 0.0f;
-    constexpr int e[3] = {A, B, C};
-#pragma unroll
-    for (int beta = 0; beta < 3; ++beta) {
-        const float k = 3.0f * emom(A + (beta == 0), B + (beta == 1), C + (beta == 2));
-        if (k != 0.0f) v = fmaf(k, F[beta], v);
+    {
+        const float k = 3.0f * emom(A + (Ax == 0), B + (Ax == 1), C + (Ax == 2));
+        if (k != 0.0f) v = fmaf(k, fa, v);
     }
     {
         const float k = -3.0f * emom(A, B, C);
-        if (k != 0.0f) v = fmaf(k, uF, v);
+        if (k != 0.0f) v = fmaf(k, uFa, v);
     }
 #pragma unroll
-    for (int al = 0; al < 3; ++al)
-#pragma unroll
-        for (int beta = 0; beta < 3; ++beta) {
-            const float k = 9.0f * emom(A + (al == 0) + (beta == 0), B + (al == 1) + (beta == 1),
-                                        C + (al == 2) + (beta == 2));
-            if (k != 0.0f) v = fmaf(k, uaFb[al][beta], v);
-        }
-    (void)e;
+    for (int al = 0; al < 3; ++al) {
+        const float k = 9.0f * emom(A + (al == 0) + (Ax == 0), B + (al == 1) + (Ax == 1), C + (al == 2) + (Ax == 2));
+        if (k != 0.0f) v = fmaf(k, ua_fa[al], v);
+    }
     return v;
+}
+
+// adds the Guo source moments (in the halved form) for a force fa along axis Ax
+template <int Ax>
+__device__ __forceinline__ void add_guo(float g[27], float fa, float ux, float uy, float uz) {
+    const float u3[3] = {ux, uy, uz};
+    const float uFa = u3[Ax] * fa;
+    const float ua_fa[3] = {ux * fa, uy * fa, uz * fa};
+#define GM(a, b, c) g[IDX(a, b, c)] += half_scale(a, b, c) * guo_moment<a, b, c, Ax>(fa, uFa, ua_fa);
+    GM(0, 0, 0) GM(0, 0, 1) GM(0, 0, 2) GM(0, 1, 0) GM(0, 1, 1) GM(0, 1, 2) GM(0, 2, 0) GM(0, 2, 1) GM(0, 2, 2)
+    GM(1, 0, 0) GM(1, 0, 1) GM(1, 0, 2) GM(1, 1, 0) GM(1, 1, 1) GM(1, 1, 2) GM(1, 2, 0) GM(1, 2, 1) GM(1, 2, 2)
+    GM(2, 0, 0) GM(2, 0, 1) GM(2, 0, 2) GM(2, 1, 0) GM(2, 1, 1) GM(2, 1, 2) GM(2, 2, 0) GM(2, 2, 1) GM(2, 2, 2)
+#undef GM
 }
 
 struct CollideResult {
@@ -538,27 +484,124 @@ __device__ __forceinline__ void shell_blend(const StepParams& P, const uint16_t*
 }
 
 // g: populations in stored form (f - w when Shifted), overwritten with the post-collision values
+// Raw moments of order <= 2 from the 27 populations (separable, only the needed
+// lines): m000, the three first moments and the six second moments.
+struct LowMoments {
+    float m000, m100, m010, m001, m200, m020, m002, m110, m101, m011;
+};
+__device__ __forceinline__ LowMoments low_moments(const float f[27]) {
+    // z lines of the 9 (x, y) columns: Z0 = sum, Z1 = c_z-weighted, Z2 = c_z^2-weighted
+    float z0[9], z1[9], z2[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float s = f[i * 3 + 0] + f[i * 3 + 2];
+        z1[i] = f[i * 3 + 2] - f[i * 3 + 0];
+        z0[i] = s + f[i * 3 + 1];
+        z2[i] = s;
+    }
+    // y lines per x: (ky, kz) in {(0,0), (1,0), (2,0), (0,1), (1,1), (0,2)}
+    float y00[3], y10[3], y20[3], y01[3], y11[3], y02[3];
+#pragma unroll
+    for (int ix = 0; ix < 3; ++ix) {
+        const int a = ix * 3, b = ix * 3 + 1, c = ix * 3 + 2;
+        const float s0 = z0[a] + z0[c];
+        y00[ix] = s0 + z0[b];
+        y10[ix] = z0[c] - z0[a];
+        y20[ix] = s0;
+        y01[ix] = (z1[a] + z1[c]) + z1[b];
+        y11[ix] = z1[c] - z1[a];
+        y02[ix] = (z2[a] + z2[c]) + z2[b];
+    }
+    LowMoments m;
+    const float sx = y00[0] + y00[2];
+    m.m000 = sx + y00[1];
+    m.m100 = y00[2] - y00[0];
+    m.m200 = sx;
+    m.m010 = (y10[0] + y10[2]) + y10[1];
+    m.m110 = y10[2] - y10[0];
+    m.m001 = (y01[0] + y01[2]) + y01[1];
+    m.m101 = y01[2] - y01[0];
+    m.m020 = (y20[0] + y20[2]) + y20[1];
+    m.m011 = (y11[0] + y11[2]) + y11[1];
+    m.m002 = (y02[0] + y02[2]) + y02[1];
+    return m;
+}
+
+// Central moments about u -> raw moments along one axis, with the outputs of
+// order 1 and 2 halved (the inputs of order 1 and 2 arrive halved already): the
+// factor the next inverse line needs. hu = u / 2, tu = 2 u, hu2 = u^2 / 2.
+__device__ __forceinline__ void unshift_half(float& k0, float& k1, float& k2, float hu, float tu, float hu2) {
+    k2 = fmaf(tu, k1, fmaf(hu2, k0, k2));
+    k1 = fmaf(hu, k0, k1);
+}
+// raw moments (order 1 and 2 halved) -> populations (c = -1, 0, +1) of one line
+__device__ __forceinline__ void inv_half(float& m0, float& m1, float& m2) {
+    const float fm = m2 - m1, fp = m2 + m1;
+    m0 = fmaf(-2.0f, m2, m0);
+    m1 = fm;  // c = -1 value, moved to slot 0 by the caller
+    m2 = fp;
+}
+
+__device__ __forceinline__ void inverse_half(float m[27]) {
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) {
+            inv_half(m[IDX(0, ky, kz)], m[IDX(1, ky, kz)], m[IDX(2, ky, kz)]);
+            const float t = m[IDX(0, ky, kz)];
+            m[IDX(0, ky, kz)] = m[IDX(1, ky, kz)];
+            m[IDX(1, ky, kz)] = t;
+        }
+#pragma unroll
+    for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+        for (int kz = 0; kz < 3; ++kz) {
+            inv_half(m[IDX(ix, 0, kz)], m[IDX(ix, 1, kz)], m[IDX(ix, 2, kz)]);
+            const float t = m[IDX(ix, 0, kz)];
+            m[IDX(ix, 0, kz)] = m[IDX(ix, 1, kz)];
+            m[IDX(ix, 1, kz)] = t;
+        }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        inv_half(m[i * 3 + 0], m[i * 3 + 1], m[i * 3 + 2]);
+        const float t = m[i * 3 + 0];
+        m[i * 3 + 0] = m[i * 3 + 1];
+        m[i * 3 + 1] = t;
+    }
+}
+
+// the same for a force (fx, fy, 0) (gravity along x or y: one of the two is zero)
+__device__ __forceinline__ void add_guo_xy(float g[27], float fx, float fy, float ux, float uy, float uz) {
+    const float ux_fx[3] = {ux * fx, uy * fx, uz * fx};
+    const float uy_fy[3] = {ux * fy, uy * fy, uz * fy};
+    const float uFx = ux * fx, uFy = uy * fy;
+#define GM(a, b, c)                                                                   \
+    g[IDX(a, b, c)] += half_scale(a, b, c) * (guo_moment<a, b, c, 0>(fx, uFx, ux_fx) + \
+                                              guo_moment<a, b, c, 1>(fy, uFy, uy_fy));
+    GM(0, 0, 0) GM(0, 0, 1) GM(0, 0, 2) GM(0, 1, 0) GM(0, 1, 1) GM(0, 1, 2) GM(0, 2, 0) GM(0, 2, 1) GM(0, 2, 2)
+    GM(1, 0, 0) GM(1, 0, 1) GM(1, 0, 2) GM(1, 1, 0) GM(1, 1, 1) GM(1, 1, 2) GM(1, 2, 0) GM(1, 2, 1) GM(1, 2, 2)
+    GM(2, 0, 0) GM(2, 0, 1) GM(2, 0, 2) GM(2, 1, 0) GM(2, 1, 1) GM(2, 1, 2) GM(2, 2, 0) GM(2, 2, 1) GM(2, 2, 2)
+#undef GM
+}
+
+// g: populations in stored form (f - w when Shifted), overwritten with the post-collision values.
+// The collision needs only the raw moments of order <= 2 (density, momentum and the
+// second moments, from which the second central moments follow); the higher central
+// moments are set by the closure. The post-collision central moments go back to
+// populations through raw moments kept in the halved form of unshift_half/inv_half.
 template <bool Shifted>
 __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27], float T) {
     CollideResult r;
-    forward_raw(g);
-    float rho;
+    LowMoments lm = low_moments(g);
     if (Shifted) {
-        const float drho = g[IDX(0, 0, 0)];
-        rho = 1.0f + drho;
-        // add the weights' own raw moments (1, 1/3, 1/9, 1/27 on the even ones)
-        g[IDX(0, 0, 0)] = rho;
-        g[IDX(2, 0, 0)] += 1.0f / 3.0f;
-        g[IDX(0, 2, 0)] += 1.0f / 3.0f;
-        g[IDX(0, 0, 2)] += 1.0f / 3.0f;
-        g[IDX(2, 2, 0)] += 1.0f / 9.0f;
-        g[IDX(2, 0, 2)] += 1.0f / 9.0f;
-        g[IDX(0, 2, 2)] += 1.0f / 9.0f;
-        g[IDX(2, 2, 2)] += 1.0f / 27.0f;
-    } else {
-        rho = g[IDX(0, 0, 0)];
+        // the weights' own raw moments: 1 and 1/3 on the even ones of order <= 2
+        lm.m000 += 1.0f;
+        lm.m200 += 1.0f / 3.0f;
+        lm.m020 += 1.0f / 3.0f;
+        lm.m002 += 1.0f / 3.0f;
     }
-    const float jx = g[IDX(1, 0, 0)], jy = g[IDX(0, 1, 0)], jz = g[IDX(0, 0, 1)];
+    const float rho = lm.m000;
+    const float jx = lm.m100, jy = lm.m010, jz = lm.m001;
     r.ok = finite(rho) && finite(jx) && finite(jy) && finite(jz);
     if (!r.ok) return r;
     const float rho_s = fmaxf(1e-6f, rho);
@@ -585,22 +628,15 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
     r.ux = ux;
     r.uy = uy;
     r.uz = uz;
-    const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
-    // raw -> central
-#pragma unroll
-    for (int ky = 0; ky < 3; ++ky)
-#pragma unroll
-        for (int kz = 0; kz < 3; ++kz) shift_line(g[IDX(0, ky, kz)], g[IDX(1, ky, kz)], g[IDX(2, ky, kz)], ux, ux2);
-#pragma unroll
-    for (int kx = 0; kx < 3; ++kx)
-#pragma unroll
-        for (int kz = 0; kz < 3; ++kz) shift_line(g[IDX(kx, 0, kz)], g[IDX(kx, 1, kz)], g[IDX(kx, 2, kz)], uy, uy2);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) shift_line(g[i * 3 + 0], g[i * 3 + 1], g[i * 3 + 2], uz, uz2);
+    // second central moments about u: sum f (c_a - u_a)(c_b - u_b)
+    const float c200 = fmaf(ux * ux, rho, fmaf(-2.0f * ux, lm.m100, lm.m200));
+    const float c020 = fmaf(uy * uy, rho, fmaf(-2.0f * uy, lm.m010, lm.m020));
+    const float c002 = fmaf(uz * uz, rho, fmaf(-2.0f * uz, lm.m001, lm.m002));
+    const float c110 = fmaf(ux * uy, rho, fmaf(-ux, lm.m010, fmaf(-uy, lm.m100, lm.m110)));
+    const float c101 = fmaf(ux * uz, rho, fmaf(-ux, lm.m001, fmaf(-uz, lm.m100, lm.m101)));
+    const float c011 = fmaf(uy * uz, rho, fmaf(-uy, lm.m001, fmaf(-uz, lm.m010, lm.m011)));
 
     const float cs2 = 1.0f / 3.0f;
-    const float c200 = g[IDX(2, 0, 0)], c020 = g[IDX(0, 2, 0)], c002 = g[IDX(0, 0, 2)];
-    const float c110 = g[IDX(1, 1, 0)], c101 = g[IDX(1, 0, 1)], c011 = g[IDX(0, 1, 1)];
     float ts = P.tau_shear_base, tn = P.tau_normal_base;
     if (P.sgs_on) {
         // ref: jni.cpp:2346-2366
@@ -633,62 +669,61 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
     const float d112 = (d002 * d110 + t101 * d011) * ir;
     const float d222 = (d200 * d020 * d002 + t110 * d110 * d002 + t101 * d101 * d020 + t011 * d011 * d200 +
                         8.0f * d110 * d101 * d011) * ir * ir;
+    // post-collision central moments in the halved form; the others are zero
 #pragma unroll
     for (int k = 0; k < 27; ++k) g[k] = 0.0f;
     g[IDX(0, 0, 0)] = rho_s;
-    g[IDX(2, 0, 0)] = d200;
-    g[IDX(0, 2, 0)] = d020;
-    g[IDX(0, 0, 2)] = d002;
-    g[IDX(1, 1, 0)] = d110;
-    g[IDX(1, 0, 1)] = d101;
-    g[IDX(0, 1, 1)] = d011;
-    g[IDX(2, 2, 0)] = d220;
-    g[IDX(2, 0, 2)] = d202;
-    g[IDX(0, 2, 2)] = d022;
-    g[IDX(2, 1, 1)] = d211;
-    g[IDX(1, 2, 1)] = d121;
-    g[IDX(1, 1, 2)] = d112;
-    g[IDX(2, 2, 2)] = d222;
-    // central -> raw
+    g[IDX(2, 0, 0)] = 0.5f * d200;
+    g[IDX(0, 2, 0)] = 0.5f * d020;
+    g[IDX(0, 0, 2)] = 0.5f * d002;
+    g[IDX(1, 1, 0)] = 0.25f * d110;
+    g[IDX(1, 0, 1)] = 0.25f * d101;
+    g[IDX(0, 1, 1)] = 0.25f * d011;
+    g[IDX(2, 2, 0)] = 0.25f * d220;
+    g[IDX(2, 0, 2)] = 0.25f * d202;
+    g[IDX(0, 2, 2)] = 0.25f * d022;
+    g[IDX(2, 1, 1)] = 0.125f * d211;
+    g[IDX(1, 2, 1)] = 0.125f * d121;
+    g[IDX(1, 1, 2)] = 0.125f * d112;
+    g[IDX(2, 2, 2)] = 0.125f * d222;
+    // central -> raw (halved form)
+    {
+        const float hz = 0.5f * uz, tz = 2.0f * uz, hz2 = 0.5f * uz * uz;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) unshift_line(g[i * 3 + 0], g[i * 3 + 1], g[i * 3 + 2], uz, uz2);
+        for (int i = 0; i < 9; ++i) unshift_half(g[i * 3 + 0], g[i * 3 + 1], g[i * 3 + 2], hz, tz, hz2);
+        const float hy = 0.5f * uy, ty = 2.0f * uy, hy2 = 0.5f * uy * uy;
 #pragma unroll
-    for (int kx = 0; kx < 3; ++kx)
+        for (int kx = 0; kx < 3; ++kx)
 #pragma unroll
-        for (int kz = 0; kz < 3; ++kz) unshift_line(g[IDX(kx, 0, kz)], g[IDX(kx, 1, kz)], g[IDX(kx, 2, kz)], uy, uy2);
+            for (int kz = 0; kz < 3; ++kz)
+                unshift_half(g[IDX(kx, 0, kz)], g[IDX(kx, 1, kz)], g[IDX(kx, 2, kz)], hy, ty, hy2);
+        const float hx = 0.5f * ux, tx = 2.0f * ux, hx2 = 0.5f * ux * ux;
 #pragma unroll
-    for (int ky = 0; ky < 3; ++ky)
+        for (int ky = 0; ky < 3; ++ky)
 #pragma unroll
-        for (int kz = 0; kz < 3; ++kz) unshift_line(g[IDX(0, ky, kz)], g[IDX(1, ky, kz)], g[IDX(2, ky, kz)], ux, ux2);
+            for (int kz = 0; kz < 3; ++kz)
+                unshift_half(g[IDX(0, ky, kz)], g[IDX(1, ky, kz)], g[IDX(2, ky, kz)], hx, tx, hx2);
+    }
     // Guo source as raw moments (ref: jni.cpp:1270-1281, added after the collision)
     if (P.buoyancy_on && b != 0.0f) {
         const float pref = fmaxf(0.0f, 1.0f - 0.5f * clampf(1.0f / ts, 0.0f, 1.999f));
         const float fa = pref * rho_s * b;  // P * F along the axis
-        const float F[3] = {P.g_axis == 0 ? fa : 0.0f, P.g_axis == 1 ? fa : 0.0f, P.g_axis == 2 ? fa : 0.0f};
-        const float uF = ux * F[0] + uy * F[1] + uz * F[2];
-        const float u3[3] = {ux, uy, uz};
-        float uaFb[3][3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) uaFb[a][c] = u3[a] * F[c];
-#define GM(a, b, c) g[IDX(a, b, c)] += guo_moment<a, b, c>(F, uF, uaFb);
-        GM(0, 0, 0) GM(0, 0, 1) GM(0, 0, 2) GM(0, 1, 0) GM(0, 1, 1) GM(0, 1, 2) GM(0, 2, 0) GM(0, 2, 1) GM(0, 2, 2)
-        GM(1, 0, 0) GM(1, 0, 1) GM(1, 0, 2) GM(1, 1, 0) GM(1, 1, 1) GM(1, 1, 2) GM(1, 2, 0) GM(1, 2, 1) GM(1, 2, 2)
-        GM(2, 0, 0) GM(2, 0, 1) GM(2, 0, 2) GM(2, 1, 0) GM(2, 1, 1) GM(2, 1, 2) GM(2, 2, 0) GM(2, 2, 1) GM(2, 2, 2)
-#undef GM
+        // gravity along z (the common case) with the x and y force terms left out
+        if (P.g_axis == 2) add_guo<2>(g, fa, ux, uy, uz);
+        else add_guo_xy(g, P.g_axis == 0 ? fa : 0.0f, P.g_axis == 1 ? fa : 0.0f, ux, uy, uz);
     }
     if (Shifted) {
+        // remove the weights' raw moments (1, 1/3, 1/9, 1/27), in the halved form
         g[IDX(0, 0, 0)] -= 1.0f;
-        g[IDX(2, 0, 0)] -= 1.0f / 3.0f;
-        g[IDX(0, 2, 0)] -= 1.0f / 3.0f;
-        g[IDX(0, 0, 2)] -= 1.0f / 3.0f;
-        g[IDX(2, 2, 0)] -= 1.0f / 9.0f;
-        g[IDX(2, 0, 2)] -= 1.0f / 9.0f;
-        g[IDX(0, 2, 2)] -= 1.0f / 9.0f;
-        g[IDX(2, 2, 2)] -= 1.0f / 27.0f;
+        g[IDX(2, 0, 0)] -= 1.0f / 6.0f;
+        g[IDX(0, 2, 0)] -= 1.0f / 6.0f;
+        g[IDX(0, 0, 2)] -= 1.0f / 6.0f;
+        g[IDX(2, 2, 0)] -= 1.0f / 36.0f;
+        g[IDX(2, 0, 2)] -= 1.0f / 36.0f;
+        g[IDX(0, 2, 2)] -= 1.0f / 36.0f;
+        g[IDX(2, 2, 2)] -= 1.0f / 216.0f;
     }
-    inverse_raw(g);
+    inverse_half(g);
     return r;
 }
 
