@@ -120,6 +120,29 @@ class AdjointConfig(Structure):
                 ("beta", c_float), ("alpha", c_float), ("u0", c_float * 3), ("probe_count", c_int)]
 
 
+class GridDesc(Structure):
+    """AeroGridDesc (include/aero_solver_capi.h)."""
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("dx", c_float), ("dt", c_float)]
+
+
+class BoundaryDesc(Structure):
+    """AeroBoundaryDesc (include/aero_solver_capi.h)."""
+    _fields_ = [("mode", c_int), ("inlet_vx", c_float), ("inlet_vy", c_float), ("inlet_vz", c_float),
+                ("outlet_pressure", c_float), ("density", c_float), ("viscosity", c_float)]
+
+
+class StepInput(Structure):
+    _fields_ = [("solid_mask", POINTER(ctypes.c_uint8)), ("prev_flow", POINTER(c_float)), ("grid", GridDesc),
+                ("boundary", BoundaryDesc), ("steps", c_int)]
+
+
+class StepOutput(Structure):
+    _fields_ = [("flow_out", POINTER(c_float)), ("max_velocity", c_float), ("status", c_int)]
+
+
+SOLVER_WIND_TUNNEL, SOLVER_WIND_TUNNEL_PRESSURE_OUTLET, SOLVER_CLOSED, SOLVER_PERIODIC = 1, 2, 3, 4
+
+
 class AeroLbmError(RuntimeError):
     pass
 
@@ -132,6 +155,19 @@ _I16 = POINTER(ctypes.c_int16)
 
 # name -> (restype, argtypes); mirrors include/aero_lbm_capi.h
 PROTOTYPES = {
+    "aero_solver_default_grid": (None, [POINTER(GridDesc)]),
+    "aero_solver_default_boundary": (None, [POINTER(BoundaryDesc)]),
+    "aero_solver_create": (c_int, [c_int, c_int, c_int, c_float, c_float, POINTER(c_longlong)]),
+    "aero_solver_create_with_grid": (c_int, [POINTER(GridDesc), POINTER(c_longlong)]),
+    "aero_solver_set_solid_mask": (c_int, [c_longlong, POINTER(ctypes.c_uint8), c_int]),
+    "aero_solver_set_flow_state": (c_int, [c_longlong, POINTER(c_float), c_int]),
+    "aero_solver_step_wind_tunnel": (c_int, [c_longlong, POINTER(BoundaryDesc), c_int, POINTER(c_float), c_int]),
+    "aero_solver_advance_wind_tunnel": (c_int, [c_longlong, POINTER(BoundaryDesc), c_int]),
+    "aero_solver_run_wind_tunnel": (c_int, [POINTER(StepInput), POINTER(StepOutput)]),
+    "aero_solver_destroy": (None, [c_longlong]),
+    "aero_solver_last_error": (c_char_p, []),
+    "aero_solver_runtime_info": (c_char_p, []),
+    "aero_solver_finish": (c_int, []),
     "aero_lbm_init": (c_int, [c_int, c_int, c_int]),
     "aero_lbm_step": (c_int, [_F, c_int, c_longlong, _F]),
     "aero_lbm_init_rect": (c_int, [c_int, c_int, c_int, c_int, c_int]),
