@@ -122,13 +122,29 @@ template <class R, class Fn> AD_HD void lines(R m[27], int axis, Fn fn) {
 
 // One fluid cell: f (27 incoming populations) -> post-collision populations; u_out
 // receives the collision velocity. T: the cell's new temperature.
-template <class R> AD_HD void collide(const Params& P, R f[27], R T, R u_out[3]) {
-    lines(f, 2, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
-    lines(f, 1, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
-    lines(f, 0, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
-    const R rho = f[AD_IDX(0, 0, 0)];
+// The collision depends on the populations only through their raw moments of
+// order <= 2 (the closure sets every higher central moment). kLow of them:
+// m000, m100, m010, m001, m200, m020, m002, m110, m101, m011.
+constexpr int kLow = 10;
+AD_HD int low_a(int k) { return k == 1 || k == 4 || k == 7 || k == 8 ? (k == 4 ? 2 : 1) : 0; }
+AD_HD int low_b(int k) { return k == 2 || k == 5 || k == 7 || k == 9 ? (k == 5 ? 2 : 1) : 0; }
+AD_HD int low_c(int k) { return k == 3 || k == 6 || k == 8 || k == 9 ? (k == 6 ? 2 : 1) : 0; }
+
+template <class R> AD_HD void low_moments(const R f[27], R m[kLow]) {
+    R t[27];
+    for (int q = 0; q < 27; ++q) t[q] = f[q];
+    lines(t, 2, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
+    lines(t, 1, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
+    lines(t, 0, [](R& a, R& b, R& c) { fwd_line(a, b, c); });
+    for (int k = 0; k < kLow; ++k) m[k] = t[AD_IDX(low_a(k), low_b(k), low_c(k))];
+}
+
+// post-collision populations f and collision velocity from the low raw moments m
+// and the new temperature T
+template <class R> AD_HD void collide_low(const Params& P, const R m[kLow], R T, R f[27], R u_out[3]) {
+    const R rho = m[0];
     const R rho_s = rmax(R(1e-6f), rho);
-    R u[3] = {f[AD_IDX(1, 0, 0)] / rho_s, f[AD_IDX(0, 1, 0)] / rho_s, f[AD_IDX(0, 0, 1)] / rho_s};
+    R u[3] = {m[1] / rho_s, m[2] / rho_s, m[3] / rho_s};
     // Boussinesq: F / rho = clamp(beta T, +-0.02) against gravity
     R b = rclamp(R(P.beta) * T, -0.02f, 0.02f);
     if (P.g_sign > 0) b = -b;
@@ -144,20 +160,26 @@ template <class R> AD_HD void collide(const Params& P, R f[27], R T, R u_out[3])
     }
     for (int k = 0; k < 3; ++k) u_out[k] = u[k];
     const R ux = u[0], uy = u[1], uz = u[2];
-    lines(f, 0, [ux](R& a, R& b, R& c) { shift_line(a, b, c, ux); });
-    lines(f, 1, [uy](R& a, R& b, R& c) { shift_line(a, b, c, uy); });
-    lines(f, 2, [uz](R& a, R& b, R& c) { shift_line(a, b, c, uz); });
+    // second central moments about u, in the operation order of the x, y, z line
+    // shifts applied to the full moment set
+    const R k100 = m[1] - ux * m[0], k010 = m[2] - uy * m[0];
+    const R c200 = m[4] - R(2) * ux * m[1] + ux * ux * m[0];
+    const R c020 = m[5] - R(2) * uy * m[2] + uy * uy * m[0];
+    const R c002 = m[6] - R(2) * uz * m[3] + uz * uz * m[0];
+    const R c110 = (m[7] - ux * m[2]) - uy * k100;
+    const R c101 = (m[8] - ux * m[3]) - uz * k100;
+    const R c011 = (m[9] - uy * m[3]) - uz * k010;
     const R cs2 = R(1.0 / 3.0);
     // relaxation rates clamped to [0, 1.95] in the closure (ref: jni.cpp:1384-1430);
     // the Guo prefactor takes the unclamped shear rate
     const double om_off = 1.0 / P.tau_shear;
     const R sd = R(fmin(fmax(1.0 / P.tau_normal, 0.0), 1.95)), so = R(fmin(fmax(om_off, 0.0), 1.95));
     const R eq2 = rho_s * cs2;
-    const R d200 = f[AD_IDX(2, 0, 0)] + sd * (eq2 - f[AD_IDX(2, 0, 0)]);
-    const R d020 = f[AD_IDX(0, 2, 0)] + sd * (eq2 - f[AD_IDX(0, 2, 0)]);
-    const R d002 = f[AD_IDX(0, 0, 2)] + sd * (eq2 - f[AD_IDX(0, 0, 2)]);
+    const R d200 = c200 + sd * (eq2 - c200);
+    const R d020 = c020 + sd * (eq2 - c020);
+    const R d002 = c002 + sd * (eq2 - c002);
     const R om = R(1) - so;
-    const R d110 = om * f[AD_IDX(1, 1, 0)], d101 = om * f[AD_IDX(1, 0, 1)], d011 = om * f[AD_IDX(0, 1, 1)];
+    const R d110 = om * c110, d101 = om * c101, d011 = om * c011;
     const R ir = R(1) / rho_s;
     // Gaussian closure of orders 4 and 6, zero odd central moments (ref: jni.cpp:1384-1430)
     const R d220 = (d200 * d020 + R(2) * d110 * d110) * ir;
@@ -201,6 +223,12 @@ template <class R> AD_HD void collide(const Params& P, R f[27], R T, R u_out[3])
             f[q] = f[q] + wq<R>(q) * fa * (R(3) * (c[P.g_axis] - u[P.g_axis]) + R(9) * cu * c[P.g_axis]);
         }
     }
+}
+
+template <class R> AD_HD void collide(const Params& P, R f[27], R T, R u_out[3]) {
+    R m[kLow];
+    low_moments(f, m);
+    collide_low(P, m, T, f, u_out);
 }
 
 AD_HD int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
