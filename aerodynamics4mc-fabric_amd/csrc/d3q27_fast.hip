@@ -272,7 +272,7 @@ __device__ __forceinline__ void face_store(const StepParams& P, int x, int y, in
 // L: the workgroup's tile index within the brick. Shell: the context has a
 // boundary-reference shell (a separate instantiation, so contexts without one keep
 // the shorter kernel and its register budget).
-template <class St, class Uv, int Mode, int StPol, bool Shell, bool XF>
+template <class St, class Uv, int Mode, int StPol, bool Shell, bool XF, bool GravZ = false>
 __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields& C, const uint32_t* __restrict__ mask,
                                           const typename St::T* __restrict__ fin, typename St::T* fout,
                                           const float* __restrict__ t_in, float* __restrict__ t_out,
@@ -344,7 +344,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
         if (xlo) xface_links<0, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
         if (xhi) xface_links<1, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
     }
-    CollideResult r = collide<St::kShifted>(P, g, temp);
+    CollideResult r = collide<St::kShifted, GravZ>(P, g, temp);
     if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
     if (Shell && P.shell_on) {
         const int layer = C.shell[cell];
@@ -381,7 +381,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     }
 }
 
-template <class St, class Uv, int Mode, int StPol, bool Shell>
+template <class St, class Uv, int Mode, int StPol, bool Shell, bool GravZ>
 __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
                                                    const typename St::T* __restrict__ fin, typename St::T* fout,
                                                    const float* __restrict__ t_in, float* __restrict__ t_out,
@@ -389,11 +389,11 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
                                                    const float* __restrict__ store_in, float* __restrict__ store_out,
                                                    int local_first) {
     if (xface_tile(P, blockIdx.x))
-        bulk_body<St, Uv, Mode, StPol, Shell, true>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
-                                                    store_out, local_first, blockIdx.x);
+        bulk_body<St, Uv, Mode, StPol, Shell, true, GravZ>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
+                                                           store_out, local_first, blockIdx.x);
     else
-        bulk_body<St, Uv, Mode, StPol, Shell, false>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
-                                                     store_out, local_first, blockIdx.x);
+        bulk_body<St, Uv, Mode, StPol, Shell, false, GravZ>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
+                                                            store_out, local_first, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- rare-cell kernels
@@ -886,11 +886,16 @@ static void launch_bulk(const StepParams& P, const CellFields& C, const uint32_t
     using T = typename St::T;
     using U = typename Uv::T;
     const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
+    // gravity along z (or no buoyancy): the instantiation without the x/y force branch
+    const bool grav_z = !P.buoyancy_on || P.g_axis == 2;
     if (P.shell_on)
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, true>), grid, block, 0, s, P, C, mask,
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, true, false>), grid, block, 0, s, P, C, mask,
+                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
+    else if (grav_z)
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, false, true>), grid, block, 0, s, P, C, mask,
                            (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
     else
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, false>), grid, block, 0, s, P, C, mask,
+        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, false, false>), grid, block, 0, s, P, C, mask,
                            (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
 }
 

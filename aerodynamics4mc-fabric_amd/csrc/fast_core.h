@@ -589,7 +589,9 @@ __device__ __forceinline__ void add_guo_xy(float g[27], float fx, float fy, floa
 // second moments, from which the second central moments follow); the higher central
 // moments are set by the closure. The post-collision central moments go back to
 // populations through raw moments kept in the halved form of unshift_half/inv_half.
-template <bool Shifted>
+// GravZ: the caller guarantees the gravity axis is z whenever buoyancy is on (a
+// separate instantiation: the x/y branch of the force costs registers)
+template <bool Shifted, bool GravZ = false>
 __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27], float T) {
     CollideResult r;
     LowMoments lm = low_moments(g);
@@ -612,8 +614,8 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
     if (P.buoyancy_on) {
         b = clampf(P.beta * T, -0.02f, 0.02f);
         if (P.g_sign > 0) b = -b;
-        if (P.g_axis == 0) ux = fmaf(0.5f, b, ux);
-        else if (P.g_axis == 1) uy = fmaf(0.5f, b, uy);
+        if (!GravZ && P.g_axis == 0) ux = fmaf(0.5f, b, ux);
+        else if (!GravZ && P.g_axis == 1) uy = fmaf(0.5f, b, uy);
         else uz = fmaf(0.5f, b, uz);
     }
     {
@@ -709,7 +711,7 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
         const float pref = fmaxf(0.0f, 1.0f - 0.5f * clampf(1.0f / ts, 0.0f, 1.999f));
         const float fa = pref * rho_s * b;  // P * F along the axis
         // gravity along z (the common case) with the x and y force terms left out
-        if (P.g_axis == 2) add_guo<2>(g, fa, ux, uy, uz);
+        if (GravZ || P.g_axis == 2) add_guo<2>(g, fa, ux, uy, uz);
         else add_guo_xy(g, P.g_axis == 0 ? fa : 0.0f, P.g_axis == 1 ? fa : 0.0f, ux, uy, uz);
     }
     if (Shifted) {
