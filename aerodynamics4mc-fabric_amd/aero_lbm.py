@@ -33,6 +33,11 @@ SOLVER_D3Q27_FP32_INPLACE = 5  # extension
 
 PRESET_NONE = 0
 PRESET_TAYLOR_GREEN_3D = 1
+PRESET_LID_DRIVEN_CAVITY_2D = 2
+PRESET_LID_DRIVEN_CAVITY_3D = 3
+PRESET_CYLINDER_CROSSFLOW_2D = 4
+PRESET_HEATED_CAVITY_2D = 5
+PRESET_HEATED_CAVITY_3D = 6
 
 HYDRO_INHERIT, HYDRO_PERIODIC, HYDRO_BOUNCE_BACK, HYDRO_MOVING_WALL = 0, 1, 2, 3
 HYDRO_VELOCITY, HYDRO_PRESSURE, HYDRO_CONVECTIVE, HYDRO_SYMMETRY = 4, 5, 6, 7

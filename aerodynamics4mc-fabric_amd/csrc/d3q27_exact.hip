@@ -185,29 +185,35 @@ __device__ __forceinline__ float t_neighbor(const StepParams& P, const float* T,
     return T[c];
 }
 
-__device__ __forceinline__ float update_temperature(const StepParams& P, const float* T, const uint8_t* flags, const float* src, int cell,
-                                    int x, int y, int z, float ux, float uy, float uz) {
-    const float dt = 1.0f;
-    const float tc = T[cell];
-    const float px = (float)x - dt * ux, py = (float)y - dt * uy, pz = (float)z - dt * uz;
+// trilinear sample of `field` at (px, py, pz); corners outside the domain or in solid
+// cells read as in t_neighbor with fallback fb (ref: jni.cpp:1910-1945)
+__device__ __forceinline__ float trilinear(const StepParams& P, const float* field, const uint8_t* flags, float px,
+                                           float py, float pz, float fb) {
     const int x0 = (int)floorf(px), y0 = (int)floorf(py), z0 = (int)floorf(pz);
     const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
     const float fx = px - (float)x0, fy = py - (float)y0, fz = pz - (float)z0;
-    const float c000 = t_neighbor(P, T, flags, x0, y0, z0, tc);
-    const float c100 = t_neighbor(P, T, flags, x1, y0, z0, tc);
-    const float c010 = t_neighbor(P, T, flags, x0, y1, z0, tc);
-    const float c110 = t_neighbor(P, T, flags, x1, y1, z0, tc);
-    const float c001 = t_neighbor(P, T, flags, x0, y0, z1, tc);
-    const float c101 = t_neighbor(P, T, flags, x1, y0, z1, tc);
-    const float c011 = t_neighbor(P, T, flags, x0, y1, z1, tc);
-    const float c111 = t_neighbor(P, T, flags, x1, y1, z1, tc);
+    const float c000 = t_neighbor(P, field, flags, x0, y0, z0, fb);
+    const float c100 = t_neighbor(P, field, flags, x1, y0, z0, fb);
+    const float c010 = t_neighbor(P, field, flags, x0, y1, z0, fb);
+    const float c110 = t_neighbor(P, field, flags, x1, y1, z0, fb);
+    const float c001 = t_neighbor(P, field, flags, x0, y0, z1, fb);
+    const float c101 = t_neighbor(P, field, flags, x1, y0, z1, fb);
+    const float c011 = t_neighbor(P, field, flags, x0, y1, z1, fb);
+    const float c111 = t_neighbor(P, field, flags, x1, y1, z1, fb);
     const float c00 = c000 + (c100 - c000) * fx;
     const float c10 = c010 + (c110 - c010) * fx;
     const float c01 = c001 + (c101 - c001) * fx;
     const float c11 = c011 + (c111 - c011) * fx;
     const float c0 = c00 + (c10 - c00) * fy;
     const float c1 = c01 + (c11 - c01) * fy;
-    const float adv = c0 + (c1 - c0) * fz;
+    return c0 + (c1 - c0) * fz;
+}
+
+__device__ __forceinline__ float update_temperature(const StepParams& P, const float* T, const uint8_t* flags, const float* src, int cell,
+                                    int x, int y, int z, float ux, float uy, float uz) {
+    const float dt = 1.0f;
+    const float tc = T[cell];
+    const float adv = trilinear(P, T, flags, (float)x - dt * ux, (float)y - dt * uy, (float)z - dt * uz, tc);
     const float tsum = t_neighbor(P, T, flags, x + 1, y, z, tc) + t_neighbor(P, T, flags, x - 1, y, z, tc) +
                        t_neighbor(P, T, flags, x, y + 1, z, tc) + t_neighbor(P, T, flags, x, y - 1, z, tc) +
                        t_neighbor(P, T, flags, x, y, z + 1, tc) + t_neighbor(P, T, flags, x, y, z - 1, tc);
@@ -216,6 +222,83 @@ __device__ __forceinline__ float update_temperature(const StepParams& P, const f
     if (P.source_on && P.in_ch > 9) source = clampr(src[cell], -0.006f, 0.006f);
     const float tn = adv + dt * (P.alpha * lap + source - 0.0f * adv);
     return clampr(tn, -1.0f, 1.0f);
+}
+
+// ---------------------------------------------------------------- BFECC temperature (jni.cpp:2050-2139)
+// Three passes over the cells, each reading only fields the previous pass finished:
+// forward advection T -> S, backward advection of S corrected into N = T + (T - B) / 2,
+// forward advection of N clamped to the bounds of T around the cell plus diffusion
+// and source of T. Velocities are those of the previous step (u_state).
+__device__ __forceinline__ void decode_cell(const StepParams& P, int cell, int& x, int& y, int& z) {
+    const int yz = P.ny * P.nz;
+    x = cell / yz;
+    const int rem = cell - x * yz;
+    y = rem / P.nz;
+    z = rem - y * P.nz;
+}
+
+__global__ void __launch_bounds__(256) bfecc_forward_kernel(StepParams P, const uint8_t* __restrict__ flags,
+                                                            const float* __restrict__ T,
+                                                            const float* __restrict__ u_state, float* __restrict__ S) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.cells) return;
+    if (flags[cell] & kCellSolid) { S[cell] = 0.0f; return; }
+    const size_t N = (size_t)P.cells;
+    int x, y, z;
+    decode_cell(P, cell, x, y, z);
+    const float dt = 1.0f;
+    S[cell] = trilinear(P, T, flags, (float)x - dt * u_state[cell], (float)y - dt * u_state[N + cell],
+                        (float)z - dt * u_state[2 * N + cell], T[cell]);
+}
+
+__global__ void __launch_bounds__(256) bfecc_backward_kernel(StepParams P, const uint8_t* __restrict__ flags,
+                                                             const float* __restrict__ T, const float* __restrict__ S,
+                                                             const float* __restrict__ u_state,
+                                                             float* __restrict__ Nf) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.cells) return;
+    if (flags[cell] & kCellSolid) { Nf[cell] = 0.0f; return; }
+    const size_t N = (size_t)P.cells;
+    int x, y, z;
+    decode_cell(P, cell, x, y, z);
+    const float dt = 1.0f;
+    const float tc = T[cell];
+    const float back = trilinear(P, S, flags, (float)x + dt * u_state[cell], (float)y + dt * u_state[N + cell],
+                                 (float)z + dt * u_state[2 * N + cell], S[cell]);
+    Nf[cell] = clampr(tc + 0.5f * (tc - back), -1.0f, 1.0f);
+}
+
+__global__ void __launch_bounds__(256) bfecc_correct_kernel(StepParams P, const uint8_t* __restrict__ flags,
+                                                            const float* __restrict__ src, const float* __restrict__ T,
+                                                            const float* __restrict__ Nf,
+                                                            const float* __restrict__ u_state,
+                                                            float* __restrict__ out) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.cells) return;
+    if (flags[cell] & kCellSolid) { out[cell] = 0.0f; return; }
+    const size_t N = (size_t)P.cells;
+    int x, y, z;
+    decode_cell(P, cell, x, y, z);
+    const float dt = 1.0f;
+    const float tc = T[cell];
+    float adv = trilinear(P, Nf, flags, (float)x - dt * u_state[cell], (float)y - dt * u_state[N + cell],
+                          (float)z - dt * u_state[2 * N + cell], Nf[cell]);
+    // local_temperature_bounds (jni.cpp:1883-1907) of T, centred on the cell
+    const float nb[6] = {t_neighbor(P, T, flags, x + 1, y, z, tc), t_neighbor(P, T, flags, x - 1, y, z, tc),
+                         t_neighbor(P, T, flags, x, y + 1, z, tc), t_neighbor(P, T, flags, x, y - 1, z, tc),
+                         t_neighbor(P, T, flags, x, y, z + 1, tc), t_neighbor(P, T, flags, x, y, z - 1, tc)};
+    float lo = tc, hi = tc;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        lo = rmin(lo, nb[i]);
+        hi = rmax(hi, nb[i]);
+    }
+    adv = clampr(adv, lo, hi);
+    const float tsum = nb[0] + nb[1] + nb[2] + nb[3] + nb[4] + nb[5];
+    const float lap = tsum - 6.0f * tc;
+    float source = 0.0f;
+    if (P.source_on && P.in_ch > 9) source = clampr(src[cell], -0.006f, 0.006f);
+    out[cell] = clampr(adv + dt * (P.alpha * lap + source - 0.0f * adv), -1.0f, 1.0f);
 }
 
 // ---------------------------------------------------------------- cumulant collision (jni.cpp:1335-1473, 2209-2413)
@@ -555,9 +638,13 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     }
     float T = 0.0f;
     if (P.thermal_on) {
-        T = update_temperature(P, t_in, C.flags, C.src, cell, x, y, z, u_state[cell], u_state[N + cell],
-                               u_state[2 * N + cell]);
-        t_out[cell] = T;
+        if (P.thermal_bfecc) {
+            T = t_out[cell];  // the BFECC passes already wrote this step's temperature
+        } else {
+            T = update_temperature(P, t_in, C.flags, C.src, cell, x, y, z, u_state[cell], u_state[N + cell],
+                                   u_state[2 * N + cell]);
+            t_out[cell] = T;
+        }
     }
     float fo[27];
     CollideOut o;
@@ -688,6 +775,16 @@ void launch_exact_step(const StepParams& P, const CellFields& C, const float* fi
                        float* t_out, float* u_state, int local_mode, hipStream_t s) {
     hipLaunchKernelGGL(exact::step_kernel, grid1(P.cells), dim3(256), 0, s, P, C, fin, fout, t_in, t_out, u_state,
                        local_mode);
+}
+
+void launch_exact_bfecc(const StepParams& P, const CellFields& C, const float* t_in, float* t_out, float* t_mid,
+                        const float* u_state, hipStream_t s) {
+    // t_out holds S until the last pass overwrites it with the new temperature
+    hipLaunchKernelGGL(exact::bfecc_forward_kernel, grid1(P.cells), dim3(256), 0, s, P, C.flags, t_in, u_state, t_out);
+    hipLaunchKernelGGL(exact::bfecc_backward_kernel, grid1(P.cells), dim3(256), 0, s, P, C.flags, t_in,
+                       (const float*)t_out, u_state, t_mid);
+    hipLaunchKernelGGL(exact::bfecc_correct_kernel, grid1(P.cells), dim3(256), 0, s, P, C.flags, C.src, t_in,
+                       (const float*)t_mid, u_state, t_out);
 }
 
 void launch_exact_stream(const StepParams& P, const CellFields& C, const float* fin, float* fout, hipStream_t s) {

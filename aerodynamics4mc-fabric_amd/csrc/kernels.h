@@ -30,6 +30,10 @@ void launch_exact_init(const StepParams& P, const CellFields& C, float* f, float
                        float initial_temperature, hipStream_t s);
 void launch_exact_step(const StepParams& P, const CellFields& C, const float* fin, float* fout, const float* t_in,
                        float* t_out, float* u_state, int local_mode, hipStream_t s);
+// BFECC temperature of one step (heated-cavity presets) into t_out; t_mid is a
+// cell-sized scratch field
+void launch_exact_bfecc(const StepParams& P, const CellFields& C, const float* t_in, float* t_out, float* t_mid,
+                        const float* u_state, hipStream_t s);
 void launch_exact_stream(const StepParams& P, const CellFields& C, const float* fin, float* fout, hipStream_t s);
 void launch_exact_readout(const StepParams& P, const CellFields& C, const float* f, int post_collision, int stride,
                           int out_ch, float scale, float* out, hipStream_t s);

@@ -79,6 +79,9 @@ struct StepParams {
     // bounce-back, or its thermal rule is Dirichlet / Neumann). A plane of x is a
     // set of whole wavefront rows, so the rules run as a wave-uniform branch.
     int xface_bulk[2];
+    // temperature by BFECC advection (heated-cavity presets): three passes before
+    // the step leave the new temperature in t_out, which the exact step kernel reads
+    int thermal_bfecc;
     // device copy of this whole struct: rarely taken kernel paths read their
     // parameters from it, so the bulk path keeps no scalar registers for them
     const StepParams* self_dev;

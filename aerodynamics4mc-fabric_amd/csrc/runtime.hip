@@ -217,7 +217,28 @@ static float clampr_(float v, float lo, float hi) { return rminf_(hi, rmaxf_(lo,
 static float clamp_nu(float nu) { return clampr_(nu, 1.0e-8f, (0.95f - 0.5f) / 3.0f); }
 static float tau_from_nu(float nu) { return 0.5f + 3.0f * clamp_nu(nu); }
 
+static bool heated_cavity(const AeroLbmBenchmarkConfig& g) {
+    return g.preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_2D ||
+           g.preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_3D;
+}
+
+// ref: jni.cpp:881-899: span of the Dirichlet face temperatures (1 without any)
+static float dirichlet_temperature_span(const AeroLbmBenchmarkConfig& g) {
+    const AeroLbmBoundaryFaceConfig* faces[6] = {&g.x_min, &g.x_max, &g.y_min, &g.y_max, &g.z_min, &g.z_max};
+    float lo = 1.0e9f, hi = -1.0e9f;
+    bool any = false;
+    for (auto* f : faces) {
+        if (f->thermal_kind != AERO_LBM_THERMAL_BOUNDARY_TEMPERATURE_DIRICHLET) continue;
+        any = true;
+        lo = std::min(lo, f->temperature);
+        hi = std::max(hi, f->temperature);
+    }
+    return any ? rmaxf_(1.0e-6f, hi - lo) : 1.0f;
+}
+
+// ref: jni.cpp:823-859 benchmark_reference_speed (cylinder preset not supported)
 static float reference_speed(const AeroLbmBenchmarkConfig& g) {
+    if (heated_cavity(g)) return clampr_(fin_or(g.mach_number, 0.05f), 1.0e-4f, 0.30f) * std::sqrt(1.0f / 3.0f);
     const float* iv = g.initial_velocity;
     float speed = std::sqrt(iv[0] * iv[0] + iv[1] * iv[1] + iv[2] * iv[2]);
     const AeroLbmBoundaryFaceConfig* faces[6] = {&g.x_min, &g.x_max, &g.y_min, &g.y_max, &g.z_min, &g.z_max};
@@ -236,10 +257,8 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
                   "enable a benchmark configuration (aero_lbm_benchmark_set_config)");
         return false;
     }
-    if (g.preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D ||
-        g.preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_2D ||
-        g.preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_3D) {
-        set_error("benchmark preset not implemented by this build (cylinder / heated cavity)");
+    if (g.preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) {
+        set_error("benchmark preset not implemented by this build (cylinder crossflow)");
         return false;
     }
     std::memset(&P, 0, sizeof(P));
@@ -276,15 +295,22 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
     const float ref_speed = reference_speed(g);
     const float ref_len = rmaxf_(1.0e-6f, fin_or(g.reference_length, 1.0f));
     const float nu_rt = clamp_nu(1.50e-5f * (1.0f / 20.0f) / (1.0f * 1.0f));
+    const bool heated = heated_cavity(g);
+    const float ra = rmaxf_(1.0e-6f, fin_or(g.rayleigh_number, 1.0e5f));
+    const float pr = rmaxf_(1.0e-6f, fin_or(g.prandtl_number, 0.71f));
     float nu;
-    if (re <= 1.0e-6f || ref_speed <= 1.0e-8f) nu = nu_rt;
+    if (heated) nu = clamp_nu(ref_speed * ref_len * std::sqrt(pr / ra));  // ref: jni.cpp:937-943
+    else if (re <= 1.0e-6f || ref_speed <= 1.0e-8f) nu = nu_rt;
     else nu = clamp_nu(ref_speed * ref_len / re);
     P.nu_base = nu;
     const float tau_min = 0.5f + 3.0f * 1.0e-8f;
     P.tau_shear_base = clampr_(tau_from_nu(nu), tau_min, 0.95f);
     P.tau_normal_base = clampr_(tau_from_nu(clamp_nu(nu)), tau_min, 0.95f);
-    P.alpha = 0.035f;
-    P.beta = ((9.81f * (1.0f / 300.0f)) * 20.0f) * (((1.0f / 20.0f) * (1.0f / 20.0f)) / 1.0f);
+    // ref: jni.cpp:966-971 (heated cavity: nu / Pr), 986-993 (heated cavity: U^2 / (L dT))
+    P.alpha = heated ? nu / pr : 0.035f;
+    P.beta = heated ? (ref_speed * ref_speed) / rmaxf_(1.0e-6f, ref_len * dirichlet_temperature_span(g))
+                    : ((9.81f * (1.0f / 300.0f)) * 20.0f) * (((1.0f / 20.0f) * (1.0f / 20.0f)) / 1.0f);
+    P.thermal_bfecc = heated ? 1 : 0;  // ref: jni.cpp:871-873
     float sgs_c = 0.025f;
     P.nut_cap_ratio = 5.0f;
     P.g_axis = 1;

@@ -52,6 +52,7 @@ struct Context {
     float* src = nullptr;
     float* T[2] = {nullptr, nullptr};
     float* u_state = nullptr;
+    float* t_bfecc = nullptr;      // corrected field of the BFECC passes (allocated on first use)
     float* f32[2] = {nullptr, nullptr};  // FP32 A-B populations (27 planes)
     uint16_t* h16 = nullptr;             // FP16/BF16 A-A populations (27 planes, shifted by w_q)
     float* face_store[2] = {nullptr, nullptr};  // A-A: near-face post-collision values per parity

@@ -134,11 +134,20 @@ static bool ensure_face_store(Context& c, const StepParams& P) {
 bool storage_step(Context& c, const StepParams& P) {
     hipStream_t st = rt().stream;
     const CellFields C = cell_fields(c);
+    if (P.thermal_bfecc && c.storage != Storage::F32Exact) {
+        set_error("heated-cavity presets (BFECC temperature) run on the exact FP32 kernels: select "
+                  "AERO_LBM_KERNEL_EXACT with the classic D3Q27 solver mode");
+        return false;
+    }
     if (c.storage == Storage::F32Exact) {
         if (!c.initialized) {
             launch_exact_init(P, C, c.f32[c.f_cur], c.T[c.t_cur], c.u_state, rt().bench.initial_temperature, st);
             c.initialized = true;
             c.post_collision = false;
+        }
+        if (P.thermal_on && P.thermal_bfecc) {
+            if (!c.t_bfecc && !c.alloc((void**)&c.t_bfecc, c.cells * 4)) return false;
+            launch_exact_bfecc(P, C, c.T[c.t_cur], c.T[1 - c.t_cur], c.t_bfecc, c.u_state, st);
         }
         const int local = c.post_collision ? 0 : 1;
         launch_exact_step(P, C, c.f32[c.f_cur], c.f32[1 - c.f_cur], c.T[c.t_cur], c.T[1 - c.t_cur], c.u_state, local,

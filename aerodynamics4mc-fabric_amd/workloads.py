@@ -173,6 +173,40 @@ def config3(n: int = 128, beta: float = 0.1, cs: float = 0.1):
     return cfg, ov, packet, t, solid
 
 
+def heated_cavity_config(three_d: bool = True, reference_length: float = 1.0) -> "A.BenchmarkConfig":
+    """Differentially heated cavity preset (ref: native/src/aero_lbm_jni.cpp:307-338
+    defaults, 384-400 preset): Ra 1e5, Pr 0.71, Ma 0.15, internal source off, no-slip
+    walls, hot x- wall at +0.5 and cold x+ wall at -0.5, other walls adiabatic; the
+    2-D variant is periodic in z. The solver runs it with BFECC thermal advection."""
+    cfg = A.BenchmarkConfig()
+    cfg.abi_version = 1
+    cfg.struct_size = 320
+    cfg.flags = (A.FLAG_DISABLE_FAN_FORCING | A.FLAG_DISABLE_FAN_NOISE | A.FLAG_DISABLE_SPONGE
+                 | A.FLAG_DISABLE_OBSTACLE_BOUNCE_BLEND | A.FLAG_DISABLE_SGS
+                 | A.FLAG_DISABLE_INTERNAL_THERMAL_SOURCE)
+    cfg.enabled = 1
+    cfg.preset = A.PRESET_HEATED_CAVITY_3D if three_d else A.PRESET_HEATED_CAVITY_2D
+    cfg.reynolds_number = 100.0
+    cfg.rayleigh_number = 1.0e5
+    cfg.prandtl_number = 0.71
+    cfg.mach_number = 0.15
+    cfg.reference_density = 1.0
+    cfg.reference_length = reference_length
+    cfg.gravity[1] = -1.0
+    for f in (cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.z_min, cfg.z_max):
+        f.hydrodynamic_kind = A.HYDRO_BOUNCE_BACK
+        f.thermal_kind = A.THERMAL_ADIABATIC
+    cfg.x_min.thermal_kind = A.THERMAL_DIRICHLET
+    cfg.x_min.temperature = 0.5
+    cfg.x_max.thermal_kind = A.THERMAL_DIRICHLET
+    cfg.x_max.temperature = -0.5
+    if not three_d:
+        for f in (cfg.z_min, cfg.z_max):
+            f.hydrodynamic_kind = A.HYDRO_PERIODIC
+            f.thermal_kind = A.THERMAL_PERIODIC
+    return cfg
+
+
 def ctypes_sizeof_overrides() -> int:
     import ctypes
     return ctypes.sizeof(A.SolverOverrides)

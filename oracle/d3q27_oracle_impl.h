@@ -43,7 +43,8 @@ typedef struct SFX(Ctx) {
     R *rho, *ux, *uy, *uz;
     R *ref_ux, *ref_uy, *ref_uz, *ref_p, *ref_T, *src;
     unsigned char* obstacle;
-    R *T, *T_next;
+    R *T, *T_next, *T_scr;
+    int bfecc; /* BFECC thermal advection (heated-cavity presets) */
     R w[27];
     R kmax_speed;
     /* boundary-reference shell (jni.cpp:4535-4552, 6449-6522): per-cell layer, FP16-
@@ -127,8 +128,20 @@ static R SFX(clamp_nu)(R nu) {
 }
 static R SFX(tau_from_nu)(R nu) { return K(0.5) + K(3.0) * SFX(clamp_nu)(nu); }
 
+static int SFX(heated_cavity)(const AeroLbmBenchmarkConfig* g) {
+    /* jni.cpp:865-869 */
+    return g->preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_2D ||
+           g->preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_3D;
+}
+
+static R SFX(finite_or_f)(float v, R fb) { return isfinite(v) ? (R)v : fb; }
+
 static R SFX(reference_speed)(const AeroLbmBenchmarkConfig* g) {
-    /* jni.cpp:819-859, presets other than heated cavity / cylinder */
+    /* jni.cpp:823-859 */
+    if (SFX(heated_cavity)(g)) {
+        const R ma = SFX(clampr)(SFX(finite_or_f)(g->mach_number, K(0.05)), K(1.0e-4), K(0.30));
+        return ma * RSQRT_(K(1.0) / K(3.0));
+    }
     const float* iv = g->initial_velocity;
     R speed = RSQRT_((R)iv[0] * (R)iv[0] + (R)iv[1] * (R)iv[1] + (R)iv[2] * (R)iv[2]);
     if (g->preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) {
@@ -146,14 +159,24 @@ static R SFX(reference_speed)(const AeroLbmBenchmarkConfig* g) {
     return speed;
 }
 
-static R SFX(finite_or_f)(float v, R fb) { return isfinite(v) ? (R)v : fb; }
+/* jni.cpp:881-899: span of the Dirichlet face temperatures (1 without any) */
+static R SFX(temperature_span)(const AeroLbmBenchmarkConfig* g) {
+    const AeroLbmBoundaryFaceConfig* faces[6] = {&g->x_min, &g->x_max, &g->y_min, &g->y_max, &g->z_min, &g->z_max};
+    R lo = K(1.0e9), hi = K(-1.0e9);
+    int any = 0;
+    for (int i = 0; i < 6; ++i) {
+        if (faces[i]->thermal_kind != AERO_LBM_THERMAL_BOUNDARY_TEMPERATURE_DIRICHLET) continue;
+        any = 1;
+        lo = SFX(rmin)(lo, (R)faces[i]->temperature);
+        hi = SFX(rmax)(hi, (R)faces[i]->temperature);
+    }
+    if (!any) return K(1.0);
+    return SFX(rmax)(K(1.0e-6), hi - lo);
+}
 
 static int SFX(resolve)(SFX(Ctx) * c, const AeroLbmSolverOverrides* ov) {
     const AeroLbmBenchmarkConfig* g = &c->cfg;
     if (!g->enabled) return 0; /* runtime mode is not restated here */
-    if (g->preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_2D ||
-        g->preset == AERO_LBM_BENCHMARK_PRESET_DIFFERENTIALLY_HEATED_CAVITY_3D)
-        return 0; /* BFECC thermal path not restated */
     if (g->preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) return 0;
     if (!(g->flags & AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING)) return 0;
     /* jni.cpp:917-929 effective_base_nu_shear */
@@ -162,17 +185,26 @@ static int SFX(resolve)(SFX(Ctx) * c, const AeroLbmSolverOverrides* ov) {
     const R ref_len = SFX(rmax)(K(1.0e-6), SFX(finite_or_f)(g->reference_length, K(1.0)));
     /* runtime molecular nu0 = 1.5e-5 m^2/s * 0.05 s / 1 m^2 (jni.cpp:100-104) */
     const R nu_rt = SFX(clamp_nu)(K(1.50e-5) * (K(1.0) / K(20.0)) / (K(1.0) * K(1.0)));
+    const int heated = SFX(heated_cavity)(g);
+    const R ra = SFX(rmax)(K(1.0e-6), SFX(finite_or_f)(g->rayleigh_number, K(1.0e5)));
+    const R pr = SFX(rmax)(K(1.0e-6), SFX(finite_or_f)(g->prandtl_number, K(0.71)));
     R nu;
-    if (re <= K(1.0e-6) || ref_speed <= K(1.0e-8)) nu = nu_rt;
+    if (heated) nu = SFX(clamp_nu)(ref_speed * ref_len * RSQRT_(pr / ra)); /* jni.cpp:937-943 */
+    else if (re <= K(1.0e-6) || ref_speed <= K(1.0e-8)) nu = nu_rt;
     else nu = SFX(clamp_nu)(ref_speed * ref_len / re);
     c->nu_base = nu;
     const R tau_min = K(0.5) + K(3.0) * K(1.0e-8);
     c->tau_shear_base = SFX(clampr)(SFX(tau_from_nu)(nu), tau_min, K(0.95));
     c->tau_normal_base = SFX(clampr)(SFX(tau_from_nu)(SFX(clamp_nu)(nu)), tau_min, K(0.95));
-    /* jni.cpp:950-955 benchmark thermal diffusivity */
-    c->alpha = K(0.035);
-    /* thermal_core.h:268-279 with jni.cpp:161-167 constants */
-    c->beta = ((K(9.81) * (K(1.0) / K(300.0))) * K(20.0)) * (((K(1.0) / K(20.0)) * (K(1.0) / K(20.0))) / K(1.0));
+    /* jni.cpp:966-971 benchmark thermal diffusivity (heated cavity: nu / Pr) */
+    c->alpha = heated ? nu / pr : K(0.035);
+    /* thermal_core.h:268-279 with jni.cpp:161-167 constants; heated cavity:
+     * U^2 / (L * dT) (jni.cpp:986-993) */
+    if (heated)
+        c->beta = (ref_speed * ref_speed) / SFX(rmax)(K(1.0e-6), ref_len * SFX(temperature_span)(g));
+    else
+        c->beta = ((K(9.81) * (K(1.0) / K(300.0))) * K(20.0)) * (((K(1.0) / K(20.0)) * (K(1.0) / K(20.0))) / K(1.0));
+    c->bfecc = heated; /* jni.cpp:871-873 */
     R sgs_c = K(0.025);
     c->nut_cap_ratio = K(5.0);
     c->g_axis = 1;
@@ -613,7 +645,86 @@ static R SFX(trilinear)(const SFX(Ctx) * c, const R* field, R x, R y, R z, R fb)
     return c0 + (c1 - c0) * fz;
 }
 
+static void SFX(decode)(const SFX(Ctx) * c, size_t cell, int* x, int* y, int* z) {
+    const int yz = c->ny * c->nz;
+    *x = (int)(cell / (size_t)yz);
+    const int rem = (int)(cell - (size_t)*x * (size_t)yz);
+    *y = rem / c->nz;
+    *z = rem - *y * c->nz;
+}
+
+/* local_temperature_bounds (jni.cpp:1883-1907): min / max of the cell and its six
+ * face neighbours (solid or outside points read as the cell itself, or the face value) */
+static void SFX(local_bounds)(const SFX(Ctx) * c, const R* field, int x, int y, int z, R* lo, R* hi) {
+    const R center = field[SFX(cell_index)(c, x, y, z)];
+    const R nb[6] = {SFX(t_neighbor)(c, field, x + 1, y, z, center), SFX(t_neighbor)(c, field, x - 1, y, z, center),
+                     SFX(t_neighbor)(c, field, x, y + 1, z, center), SFX(t_neighbor)(c, field, x, y - 1, z, center),
+                     SFX(t_neighbor)(c, field, x, y, z + 1, center), SFX(t_neighbor)(c, field, x, y, z - 1, center)};
+    R mn = center, mx = center;
+    for (int i = 0; i < 6; ++i) {
+        mn = SFX(rmin)(mn, nb[i]);
+        mx = SFX(rmax)(mx, nb[i]);
+    }
+    *lo = mn;
+    *hi = mx;
+}
+
+/* update_temperature_field, BFECC branch (jni.cpp:2050-2139): advect forward
+ * (T -> S), advect S backward (-> B), correct N = T + (T - B) / 2, advect N forward,
+ * clamp to the bounds of T around the cell, then diffusion and source of T */
+static void SFX(update_temperature_bfecc)(SFX(Ctx) * c) {
+    const R dt = (R)1;
+    const int src_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_INTERNAL_THERMAL_SOURCE);
+    const R* T = c->T;
+    R* S = c->T_scr;
+    R* Nf = c->T_next;
+#pragma omp parallel for schedule(static)
+    for (size_t cell = 0; cell < c->cells; ++cell) {
+        if (c->obstacle[cell]) { S[cell] = K(0.0); continue; }
+        int x, y, z;
+        SFX(decode)(c, cell, &x, &y, &z);
+        S[cell] = SFX(trilinear)(c, T, (R)x - dt * c->ux[cell], (R)y - dt * c->uy[cell], (R)z - dt * c->uz[cell],
+                                 T[cell]);
+    }
+#pragma omp parallel for schedule(static)
+    for (size_t cell = 0; cell < c->cells; ++cell) {
+        if (c->obstacle[cell]) { Nf[cell] = K(0.0); continue; }
+        int x, y, z;
+        SFX(decode)(c, cell, &x, &y, &z);
+        const R tc = T[cell];
+        const R back = SFX(trilinear)(c, S, (R)x + dt * c->ux[cell], (R)y + dt * c->uy[cell], (R)z + dt * c->uz[cell],
+                                      S[cell]);
+        Nf[cell] = SFX(clampr)(tc + K(0.5) * (tc - back), K(-1.0), K(1.0));
+    }
+#pragma omp parallel for schedule(static)
+    for (size_t cell = 0; cell < c->cells; ++cell) {
+        if (c->obstacle[cell]) { S[cell] = K(0.0); continue; }
+        int x, y, z;
+        SFX(decode)(c, cell, &x, &y, &z);
+        const R tc = T[cell];
+        R adv = SFX(trilinear)(c, Nf, (R)x - dt * c->ux[cell], (R)y - dt * c->uy[cell], (R)z - dt * c->uz[cell],
+                               Nf[cell]);
+        R lo, hi;
+        SFX(local_bounds)(c, T, x, y, z, &lo, &hi);
+        adv = SFX(clampr)(adv, lo, hi);
+        const R tsum = SFX(t_neighbor)(c, T, x + 1, y, z, tc) + SFX(t_neighbor)(c, T, x - 1, y, z, tc) +
+                       SFX(t_neighbor)(c, T, x, y + 1, z, tc) + SFX(t_neighbor)(c, T, x, y - 1, z, tc) +
+                       SFX(t_neighbor)(c, T, x, y, z + 1, tc) + SFX(t_neighbor)(c, T, x, y, z - 1, tc);
+        const R lap = tsum - K(6.0) * tc;
+        R src = K(0.0);
+        if (src_on && c->in_ch > 9) src = SFX(clampr)(c->src[cell], K(-0.006), K(0.006));
+        S[cell] = SFX(clampr)(adv + dt * (c->alpha * lap + src - K(0.0) * adv), K(-1.0), K(1.0));
+    }
+    c->T_scr = c->T;
+    c->T = S;
+    memcpy(c->T_next, c->T, c->cells * sizeof(R));
+}
+
 static void SFX(update_temperature)(SFX(Ctx) * c) {
+    if (c->bfecc) {
+        SFX(update_temperature_bfecc)(c);
+        return;
+    }
     const R dt = (R)1; /* benchmark stride 1 (jni.cpp:901-909) */
     const int src_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_INTERNAL_THERMAL_SOURCE);
 #pragma omp parallel for schedule(static)
@@ -732,7 +843,7 @@ ORACLE_API void* SFX(oracle_create)(int nx, int ny, int nz, int in_ch, int out_c
     c->ref_ux = (R*)calloc(N, sizeof(R)); c->ref_uy = (R*)calloc(N, sizeof(R)); c->ref_uz = (R*)calloc(N, sizeof(R));
     c->ref_p = (R*)calloc(N, sizeof(R)); c->ref_T = (R*)calloc(N, sizeof(R)); c->src = (R*)calloc(N, sizeof(R));
     c->obstacle = (unsigned char*)calloc(N, 1);
-    c->T = (R*)calloc(N, sizeof(R)); c->T_next = (R*)calloc(N, sizeof(R));
+    c->T = (R*)calloc(N, sizeof(R)); c->T_next = (R*)calloc(N, sizeof(R)); c->T_scr = (R*)calloc(N, sizeof(R));
     for (size_t i = 0; i < N; ++i) c->rho[i] = K(1.0);
     return c;
 }
@@ -742,7 +853,7 @@ ORACLE_API void SFX(oracle_destroy)(void* h) {
     if (!c) return;
     free(c->f); free(c->f_post); free(c->rho); free(c->ux); free(c->uy); free(c->uz);
     free(c->ref_ux); free(c->ref_uy); free(c->ref_uz); free(c->ref_p); free(c->ref_T); free(c->src);
-    free(c->obstacle); free(c->T); free(c->T_next); free(c->shell_layer); free(c->shell_target); free(c);
+    free(c->obstacle); free(c->T); free(c->T_next); free(c->T_scr); free(c->shell_layer); free(c->shell_target); free(c);
 }
 
 /* jni.cpp:9582-9601 assign_temperature_state */

@@ -63,6 +63,19 @@ def test_benchmark_config_roundtrip_without_gpu():
     lib.aero_lbm_benchmark_reset_config()
 
 
+@pytest.mark.parametrize("three_d", [True, False])
+def test_heated_cavity_preset_matches_reference_preset(three_d):
+    """The library's heated-cavity preset is the reference's (jni.cpp:307-338, 384-400),
+    as restated by the workload helper, byte for byte."""
+    import workloads as W
+    lib = A.load()
+    got = A.BenchmarkConfig()
+    preset = A.PRESET_HEATED_CAVITY_3D if three_d else A.PRESET_HEATED_CAVITY_2D
+    assert lib.aero_lbm_benchmark_default_preset_config(preset, ctypes.byref(got)) == 1
+    want = W.heated_cavity_config(three_d)
+    assert bytes(got) == bytes(want)
+
+
 def test_overrides_validation():
     lib = A.load()
     ov = A.default_overrides()

@@ -73,6 +73,52 @@ def test_exact_boussinesq_blob_gravity_z(oracle_mod, lib):
     ctx.release()
 
 
+def heated_cavity_case(dims, seed=7):
+    """Cavity state for the heated-cavity presets: a solid block, a linear hot-to-cold
+    profile along x with noise, small random velocities (reference velocity)."""
+    nx, ny, nz = dims
+    solid = np.zeros(dims, np.uint8)
+    solid[nx // 2 - 2:nx // 2 + 1, ny // 3:ny // 3 + 3, nz // 3:nz // 3 + 3] = 1
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx, dtype=np.float64)[:, None, None]
+    t = (0.5 - (x + 0.5) / nx) + 0.05 * rng.uniform(-1.0, 1.0, dims)
+    t = np.broadcast_to(t, dims).astype(np.float32).copy()
+    t[solid.astype(bool)] = 0.0
+    packet = W.make_packet(solid, u0=(0.0, 0.0, 0.0), perturb=0.02, seed_u=5, temperature=t)
+    return packet, t
+
+
+@pytest.mark.parametrize("three_d,length,steps", [(True, 1.0, 30), (False, 16.0, 30), (True, 16.0, 200)])
+def test_exact_heated_cavity_bfecc(oracle_mod, lib, three_d, length, steps):
+    dims = (16, 12, 10)
+    log(f"exact heated cavity {'3d' if three_d else '2d'} L={length} {dims} {steps} steps (BFECC temperature)")
+    cfg = W.heated_cavity_config(three_d, reference_length=length)
+    packet, t = heated_cavity_case(dims)
+    ov = W.reference_overrides(A.KERNEL_EXACT)
+    ctx, o = run_both(oracle_mod, lib, cfg, packet, steps, dims=dims, ov=ov, temperature=t)
+    assert np.array_equal(ctx.temperature(), o.temperature())
+    assert np.array_equal(ctx.flow(), o.flow())
+    # buoyancy has set the cavity in motion
+    assert np.abs(ctx.flow().reshape(-1, 4)[:, :3]).max() > 1e-4
+    ctx.release()
+
+
+def test_heated_cavity_rejects_fast_kernels(lib):
+    dims = (12, 8, 8)
+    log("heated cavity with the fast kernels: step is refused")
+    cfg = W.heated_cavity_config(True)
+    packet, t = heated_cavity_case(dims)
+    ov = W.reference_overrides(A.KERNEL_FAST)
+    A.check(lib.aero_lbm_init_rect(*dims, 11, 4), "init_rect")
+    A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "set_config")
+    A.check(lib.aero_lbm_set_solver_overrides(ctypes.byref(ov)), "set_overrides")
+    lib.aero_lbm_set_realtime_solver_mode(A.SOLVER_CLASSIC_D3Q27)
+    ctx = A.Context(*dims, key=12)
+    with pytest.raises(A.AeroLbmError, match="exact FP32"):
+        ctx.step(packet)
+    ctx.release()
+
+
 @pytest.mark.parametrize("variant", ["symmetry", "walls", "pressure", "inherit", "inherit_noconv", "sponge"])
 def test_exact_face_kinds(oracle_mod, lib, variant):
     log(f"exact faces {variant}")
