@@ -103,14 +103,43 @@ __device__ __forceinline__ float convective(const StepParams& P, const Src& s, i
     return f1 + 0.15f * (f1 - f2);
 }
 
-// ref: jni.cpp:1518-1561 (cylinder preset is not supported by this build)
-__device__ __forceinline__ float equilibrium_face(const StepParams& P, const Src& s, int cell, int q, const FaceParams& fc) {
+// cylinder crossflow preset geometry (ref: jni.cpp:190-228), binary32 as the reference
+// defines it: channel height ny - 1, diameter 0.1/2.2 of the length nx - 1 (at least
+// 2.5 cells, at most the height - 4), centre at (0.2/2.2, 0.2/0.41) of (length, height)
+__device__ __forceinline__ float cylinder_inlet_ux(const StepParams& P, int y, float u_max) {
+    const float h = (float)(P.ny - 1 > 1 ? P.ny - 1 : 1);
+    const float denom = rmax(h - 0.0f, 1.0e-6f);
+    const float s = ((float)y - 0.0f) / denom;
+    if (s < 0.0f || s > 1.0f) return 0.0f;
+    return 4.0f * u_max * s * (1.0f - s);
+}
+
+__device__ __forceinline__ bool cylinder_solid(const StepParams& P, int x, int y) {
+    const float len = (float)(P.nx - 1 > 1 ? P.nx - 1 : 1);
+    const float h = (float)(P.ny - 1 > 1 ? P.ny - 1 : 1);
+    float d = rmax(2.5f, 0.1f / 2.2f * len);
+    d = rmin(d, rmax(2.5f, h - 4.0f));
+    const float r = 0.5f * d;
+    const float cx = 0.2f / 2.2f * len;
+    const float cy = 0.0f + 0.2f / 0.41f * h;
+    const float dx = (float)x - cx, dy = (float)y - cy;
+    return dx * dx + dy * dy <= r * r;
+}
+
+// ref: jni.cpp:1518-1561
+__device__ __forceinline__ float equilibrium_face(const StepParams& P, const Src& s, int cell, int y, int q, int sx,
+                                                  const FaceParams& fc) {
     float rho = 1.0f + clampr(s.ref_p[cell], -0.03f, 0.03f);
     float ux = 0.0f, uy = 0.0f, uz = 0.0f;
     if (fc.hydro_kind == 3 || fc.hydro_kind == 4) {
         ux = fc.vel[0];
         uy = fc.vel[1];
         uz = fc.vel[2];
+        if (P.cylinder && sx < 0) {  // parabolic inlet (ref: jni.cpp:1540-1544)
+            ux = cylinder_inlet_ux(P, y, fc.vel[0]);
+            uy = 0.0f;
+            uz = 0.0f;
+        }
     } else if (fc.hydro_kind == 5) {
         rho = 1.0f + clampr(fc.pressure, -0.03f, 0.03f);
     }
@@ -135,7 +164,7 @@ __device__ __forceinline__ float face_value(const StepParams& P, const Src& s, i
         case 3:
         case 4:
         case 5:
-            return equilibrium_face(P, s, cell, q, fc);
+            return equilibrium_face(P, s, cell, y, q, sx, fc);
         case 7: {
             const int src = cell_of(P, iclamp(sx, 0, P.nx - 1), iclamp(sy, 0, P.ny - 1), iclamp(sz, 0, P.nz - 1));
             return (s.flags[src] & kCellSolid) ? bounce(s, cell, q) : s.fp[(size_t)q * s.N + src];
@@ -299,6 +328,72 @@ __global__ void __launch_bounds__(256) bfecc_correct_kernel(StepParams P, const 
     float source = 0.0f;
     if (P.source_on && P.in_ch > 9) source = clampr(src[cell], -0.006f, 0.006f);
     out[cell] = clampr(adv + dt * (P.alpha * lap + source - 0.0f * adv), -1.0f, 1.0f);
+}
+
+// ---------------------------------------------------------------- drag of the cylinder preset (jni.cpp:2452-2492)
+// Momentum exchange 2 f*_q c_q over the in-plane links (c_z = 0) from fluid cells into
+// solid cells of the preset's cylinder, in FP64: per-cell sums in q order, a fixed
+// tree per block, then one block adds the block partials in a fixed order, so the
+// result does not depend on scheduling.
+__device__ __forceinline__ void block_sum3(double v[3], double* lds) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lds[k * 256 + t] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lds[k * 256 + t] += lds[k * 256 + t + w];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = lds[k * 256];
+}
+
+__global__ void __launch_bounds__(256) force_partial_kernel(StepParams P, const uint8_t* __restrict__ flags,
+                                                            const float* __restrict__ fpost,
+                                                            double* __restrict__ part) {
+    __shared__ double lds[3 * 256];
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (cell < P.cells && !(flags[cell] & kCellSolid)) {
+        const size_t N = (size_t)P.cells;
+        int x, y, z;
+        decode_cell(P, cell, x, y, z);
+#pragma unroll
+        for (int q = 1; q < 27; ++q) {
+            if (lat_cz(q) != 0) continue;
+            const int ax = x + lat_cx(q), ay = y + lat_cy(q);
+            if (ax < 0 || ay < 0 || ax >= P.nx || ay >= P.ny) continue;
+            if (!(flags[cell_of(P, ax, ay, z)] & kCellSolid)) continue;
+            if (!cylinder_solid(P, ax, ay)) continue;
+            const double fq = (double)fpost[(size_t)q * N + cell];
+            acc[0] += 2.0 * fq * (double)lat_cx(q);
+            acc[1] += 2.0 * fq * (double)lat_cy(q);
+            acc[2] += 2.0 * fq * (double)lat_cz(q);
+        }
+    }
+    block_sum3(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) part[3 * (size_t)blockIdx.x + k] = acc[k];
+    }
+}
+
+__global__ void __launch_bounds__(256) force_final_kernel(StepParams P, const double* __restrict__ part, int blocks,
+                                                          float* __restrict__ out) {
+    __shared__ double lds[3 * 256];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < blocks; b += 256) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] += part[3 * (size_t)b + k];
+    }
+    block_sum3(acc, lds);
+    if (threadIdx.x < 3) {
+        const double inv_depth = P.nz > 0 ? (1.0 / (double)P.nz) : 1.0;
+        out[threadIdx.x] = (float)(acc[threadIdx.x] * inv_depth);
+    }
 }
 
 // ---------------------------------------------------------------- cumulant collision (jni.cpp:1335-1473, 2209-2413)
@@ -785,6 +880,13 @@ void launch_exact_bfecc(const StepParams& P, const CellFields& C, const float* t
                        (const float*)t_out, u_state, t_mid);
     hipLaunchKernelGGL(exact::bfecc_correct_kernel, grid1(P.cells), dim3(256), 0, s, P, C.flags, C.src, t_in,
                        (const float*)t_mid, u_state, t_out);
+}
+
+void launch_exact_force(const StepParams& P, const CellFields& C, const float* fpost, double* part, hipStream_t s) {
+    const int blocks = (P.cells + 255) / 256;
+    hipLaunchKernelGGL(exact::force_partial_kernel, dim3(blocks), dim3(256), 0, s, P, C.flags, fpost, part);
+    hipLaunchKernelGGL(exact::force_final_kernel, dim3(1), dim3(256), 0, s, P, (const double*)part, blocks,
+                       (float*)(part + 3 * (size_t)blocks));
 }
 
 void launch_exact_stream(const StepParams& P, const CellFields& C, const float* fin, float* fout, hipStream_t s) {

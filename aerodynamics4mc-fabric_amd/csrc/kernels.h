@@ -34,6 +34,9 @@ void launch_exact_step(const StepParams& P, const CellFields& C, const float* fi
 // cell-sized scratch field
 void launch_exact_bfecc(const StepParams& P, const CellFields& C, const float* t_in, float* t_out, float* t_mid,
                         const float* u_state, hipStream_t s);
+// drag integral of the cylinder preset over post-collision populations: part holds
+// 3 doubles per 256-cell block, then the float force (fx, fy, fz)
+void launch_exact_force(const StepParams& P, const CellFields& C, const float* fpost, double* part, hipStream_t s);
 void launch_exact_stream(const StepParams& P, const CellFields& C, const float* fin, float* fout, hipStream_t s);
 void launch_exact_readout(const StepParams& P, const CellFields& C, const float* f, int post_collision, int stride,
                           int out_ch, float scale, float* out, hipStream_t s);

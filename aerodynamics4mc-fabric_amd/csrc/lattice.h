@@ -82,6 +82,9 @@ struct StepParams {
     // temperature by BFECC advection (heated-cavity presets): three passes before
     // the step leave the new temperature in t_out, which the exact step kernel reads
     int thermal_bfecc;
+    // cylinder crossflow preset: parabolic inlet on the x- velocity face and the
+    // drag integral after each step (exact kernels)
+    int cylinder;
     // device copy of this whole struct: rarely taken kernel paths read their
     // parameters from it, so the bulk path keeps no scalar registers for them
     const StepParams* self_dev;

@@ -241,6 +241,11 @@ static float reference_speed(const AeroLbmBenchmarkConfig& g) {
     if (heated_cavity(g)) return clampr_(fin_or(g.mach_number, 0.05f), 1.0e-4f, 0.30f) * std::sqrt(1.0f / 3.0f);
     const float* iv = g.initial_velocity;
     float speed = std::sqrt(iv[0] * iv[0] + iv[1] * iv[1] + iv[2] * iv[2]);
+    if (g.preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) {
+        if (speed > 1.0e-8f) return speed;
+        const float* v = g.x_min.velocity;
+        return (2.0f / 3.0f) * std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
     const AeroLbmBoundaryFaceConfig* faces[6] = {&g.x_min, &g.x_max, &g.y_min, &g.y_max, &g.z_min, &g.z_max};
     for (auto* f : faces) {
         const float* v = f->velocity;
@@ -255,10 +260,6 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
     if (!g.enabled) {
         set_error("runtime (non-benchmark) solver mode is not implemented by this build; "
                   "enable a benchmark configuration (aero_lbm_benchmark_set_config)");
-        return false;
-    }
-    if (g.preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) {
-        set_error("benchmark preset not implemented by this build (cylinder crossflow)");
         return false;
     }
     std::memset(&P, 0, sizeof(P));
@@ -311,6 +312,7 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
     P.beta = heated ? (ref_speed * ref_speed) / rmaxf_(1.0e-6f, ref_len * dirichlet_temperature_span(g))
                     : ((9.81f * (1.0f / 300.0f)) * 20.0f) * (((1.0f / 20.0f) * (1.0f / 20.0f)) / 1.0f);
     P.thermal_bfecc = heated ? 1 : 0;  // ref: jni.cpp:871-873
+    P.cylinder = g.preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D ? 1 : 0;
     float sgs_c = 0.025f;
     P.nut_cap_ratio = 5.0f;
     P.g_axis = 1;
@@ -875,10 +877,23 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_copy_flow_temperature_subrect(int nx, int ny, 
 }
 
 AERO_LBM_CAPI_EXPORT int aero_lbm_get_last_force(long long key, float* fx, float* fy, float* fz) {
-    // the drag integral is only produced for the cylinder preset, which this build does not run
+    // drag integral of the last step: produced for the cylinder preset, zero otherwise
+    // (ref: jni.cpp:2452-2492, ctx.last_force)
     std::lock_guard<std::mutex> lk(rt().mu);
-    if (!find_context(key) || !fx || !fy || !fz) return 0;
-    *fx = *fy = *fz = 0.0f;
+    Context* c = find_context(key);
+    if (!c || !fx || !fy || !fz) return 0;
+    float f[3] = {0.0f, 0.0f, 0.0f};
+    if (c->force_valid && c->params_valid && c->params_host.cylinder) {
+        const int blocks = (int)((c->cells + 255) / 256);
+        if (!hip_ok(hipMemcpyAsync(f, c->force_part + 3 * (size_t)blocks, sizeof(f), hipMemcpyDeviceToHost,
+                                   rt().stream),
+                    "readback") ||
+            !hip_ok(hipStreamSynchronize(rt().stream), "sync"))
+            return 0;
+    }
+    *fx = f[0];
+    *fy = f[1];
+    *fz = f[2];
     return 1;
 }
 

@@ -53,6 +53,8 @@ struct Context {
     float* T[2] = {nullptr, nullptr};
     float* u_state = nullptr;
     float* t_bfecc = nullptr;      // corrected field of the BFECC passes (allocated on first use)
+    double* force_part = nullptr;  // per-block drag partials, then the float force (cylinder preset)
+    bool force_valid = false;      // force_part ends with this context's last drag integral
     float* f32[2] = {nullptr, nullptr};  // FP32 A-B populations (27 planes)
     uint16_t* h16 = nullptr;             // FP16/BF16 A-A populations (27 planes, shifted by w_q)
     float* face_store[2] = {nullptr, nullptr};  // A-A: near-face post-collision values per parity

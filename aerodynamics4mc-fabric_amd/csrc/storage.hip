@@ -134,9 +134,9 @@ static bool ensure_face_store(Context& c, const StepParams& P) {
 bool storage_step(Context& c, const StepParams& P) {
     hipStream_t st = rt().stream;
     const CellFields C = cell_fields(c);
-    if (P.thermal_bfecc && c.storage != Storage::F32Exact) {
-        set_error("heated-cavity presets (BFECC temperature) run on the exact FP32 kernels: select "
-                  "AERO_LBM_KERNEL_EXACT with the classic D3Q27 solver mode");
+    if ((P.thermal_bfecc || P.cylinder) && c.storage != Storage::F32Exact) {
+        set_error("heated-cavity and cylinder presets (BFECC temperature, inlet profile and drag) run on the "
+                  "exact FP32 kernels: select AERO_LBM_KERNEL_EXACT with the classic D3Q27 solver mode");
         return false;
     }
     if (c.storage == Storage::F32Exact) {
@@ -155,6 +155,13 @@ bool storage_step(Context& c, const StepParams& P) {
         c.f_cur ^= 1;
         if (P.thermal_on) c.t_cur ^= 1;
         c.post_collision = true;
+        c.force_valid = false;
+        if (P.cylinder) {
+            const size_t blocks = (c.cells + 255) / 256;
+            if (!c.force_part && !c.alloc((void**)&c.force_part, (3 * blocks + 2) * sizeof(double))) return false;
+            launch_exact_force(P, C, c.f32[c.f_cur], c.force_part, st);
+            c.force_valid = true;
+        }
         return hip_ok(hipGetLastError(), "exact step");
     }
     const FastStorage fk = fast_kind(c.storage);

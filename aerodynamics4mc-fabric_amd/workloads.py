@@ -207,6 +207,48 @@ def heated_cavity_config(three_d: bool = True, reference_length: float = 1.0) ->
     return cfg
 
 
+def cylinder_config() -> "A.BenchmarkConfig":
+    """Cylinder crossflow preset (ref: native/src/aero_lbm_jni.cpp:307-338 defaults,
+    366-380 preset): Re 100, buoyancy and internal source off, no-slip walls, x- velocity
+    inlet with u_max 0.12 (mean 0.08, parabolic in y), x+ convective outflow, periodic
+    z. The solid cylinder comes with the payload; the solver adds the drag integral."""
+    cfg = heated_cavity_config(True)
+    cfg.preset = A.PRESET_CYLINDER_CROSSFLOW_2D
+    cfg.flags = (A.FLAG_DISABLE_FAN_FORCING | A.FLAG_DISABLE_FAN_NOISE | A.FLAG_DISABLE_SPONGE
+                 | A.FLAG_DISABLE_OBSTACLE_BOUNCE_BLEND | A.FLAG_DISABLE_SGS
+                 | A.FLAG_DISABLE_INTERNAL_THERMAL_SOURCE | A.FLAG_DISABLE_BUOYANCY)
+    cfg.reynolds_number = 100.0
+    cfg.mach_number = 0.05
+    for f in (cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.z_min, cfg.z_max):
+        f.hydrodynamic_kind = A.HYDRO_BOUNCE_BACK
+        f.thermal_kind = A.THERMAL_DISABLED
+        f.temperature = 0.0
+    cfg.x_min.hydrodynamic_kind = A.HYDRO_VELOCITY
+    cfg.initial_velocity[0] = 0.08
+    cfg.x_min.velocity[0] = 1.5 * 0.08
+    cfg.x_max.hydrodynamic_kind = A.HYDRO_CONVECTIVE
+    for f in (cfg.z_min, cfg.z_max):
+        f.hydrodynamic_kind = A.HYDRO_PERIODIC
+        f.thermal_kind = A.THERMAL_PERIODIC
+    return cfg
+
+
+def cylinder_mask(nx: int, ny: int, nz: int) -> np.ndarray:
+    """Solid mask of the preset's cylinder (ref: native/src/aero_lbm_jni.cpp:200-228),
+    binary32 geometry, extruded along z."""
+    f32 = np.float32
+    length, height = f32(max(1, nx - 1)), f32(max(1, ny - 1))
+    d = max(f32(2.5), f32(0.1) / f32(2.2) * length)
+    d = min(d, max(f32(2.5), height - f32(4.0)))
+    r = f32(0.5) * d
+    cx = f32(0.2) / f32(2.2) * length
+    cy = f32(0.0) + f32(0.2) / f32(0.41) * height
+    X, Y = np.meshgrid(np.arange(nx, dtype=f32), np.arange(ny, dtype=f32), indexing="ij")
+    dx, dy = X - cx, Y - cy
+    disk = dx * dx + dy * dy <= r * r
+    return np.repeat(disk[:, :, None], nz, axis=2).astype(np.uint8)
+
+
 def ctypes_sizeof_overrides() -> int:
     import ctypes
     return ctypes.sizeof(A.SolverOverrides)

@@ -45,6 +45,8 @@ typedef struct SFX(Ctx) {
     unsigned char* obstacle;
     R *T, *T_next, *T_scr;
     int bfecc; /* BFECC thermal advection (heated-cavity presets) */
+    int cylinder; /* cylinder crossflow preset: parabolic inlet, drag integral */
+    float force[3]; /* last step's drag integral (ctx.last_force) */
     R w[27];
     R kmax_speed;
     /* boundary-reference shell (jni.cpp:4535-4552, 6449-6522): per-cell layer, FP16-
@@ -177,7 +179,6 @@ static R SFX(temperature_span)(const AeroLbmBenchmarkConfig* g) {
 static int SFX(resolve)(SFX(Ctx) * c, const AeroLbmSolverOverrides* ov) {
     const AeroLbmBenchmarkConfig* g = &c->cfg;
     if (!g->enabled) return 0; /* runtime mode is not restated here */
-    if (g->preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D) return 0;
     if (!(g->flags & AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING)) return 0;
     /* jni.cpp:917-929 effective_base_nu_shear */
     const R re = SFX(finite_or_f)(g->reynolds_number, K(0.0));
@@ -205,6 +206,7 @@ static int SFX(resolve)(SFX(Ctx) * c, const AeroLbmSolverOverrides* ov) {
     else
         c->beta = ((K(9.81) * (K(1.0) / K(300.0))) * K(20.0)) * (((K(1.0) / K(20.0)) * (K(1.0) / K(20.0))) / K(1.0));
     c->bfecc = heated; /* jni.cpp:871-873 */
+    c->cylinder = g->preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D;
     R sgs_c = K(0.025);
     c->nut_cap_ratio = K(5.0);
     c->g_axis = 1;
@@ -530,13 +532,29 @@ static R SFX(convective)(const SFX(Ctx) * c, size_t cell, int x, int y, int z, i
     return f1 + K(0.15) * (f1 - f2);
 }
 
-static R SFX(equilibrium_face)(const SFX(Ctx) * c, size_t cell, int q, const AeroLbmBoundaryFaceConfig* fc) {
+/* parabolic inlet of the cylinder preset (jni.cpp:214-220): 4 u_max s (1 - s),
+ * s = y / (ny - 1) */
+static R SFX(cylinder_inlet_ux)(const SFX(Ctx) * c, int y, R u_max) {
+    const R h = (R)(c->ny - 1 > 1 ? c->ny - 1 : 1);
+    const R denom = SFX(rmax)(h, K(1.0e-6));
+    const R s = ((R)y - K(0.0)) / denom;
+    if (s < K(0.0) || s > K(1.0)) return K(0.0);
+    return K(4.0) * u_max * s * (K(1.0) - s);
+}
+
+static R SFX(equilibrium_face)(const SFX(Ctx) * c, size_t cell, int y, int q, int sx,
+                               const AeroLbmBoundaryFaceConfig* fc) {
     R rho = K(1.0) + SFX(clampr)(c->ref_p[cell], K(-0.03), K(0.03));
     R ux = K(0.0), uy = K(0.0), uz = K(0.0);
     switch (fc->hydrodynamic_kind) {
         case AERO_LBM_HYDRO_BOUNDARY_MOVING_WALL:
         case AERO_LBM_HYDRO_BOUNDARY_VELOCITY_DIRICHLET:
             ux = (R)fc->velocity[0]; uy = (R)fc->velocity[1]; uz = (R)fc->velocity[2];
+            if (c->cylinder && sx < 0) { /* jni.cpp:1540-1544 */
+                ux = SFX(cylinder_inlet_ux)(c, y, (R)fc->velocity[0]);
+                uy = K(0.0);
+                uz = K(0.0);
+            }
             break;
         case AERO_LBM_HYDRO_BOUNDARY_PRESSURE_DIRICHLET:
             rho = K(1.0) + SFX(clampr)((R)fc->pressure, K(-0.03), K(0.03));
@@ -561,7 +579,7 @@ static R SFX(face_value)(const SFX(Ctx) * c, size_t cell, int x, int y, int z, i
         case AERO_LBM_HYDRO_BOUNDARY_MOVING_WALL:
         case AERO_LBM_HYDRO_BOUNDARY_VELOCITY_DIRICHLET:
         case AERO_LBM_HYDRO_BOUNDARY_PRESSURE_DIRICHLET:
-            return SFX(equilibrium_face)(c, cell, q, fc);
+            return SFX(equilibrium_face)(c, cell, y, q, sx, fc);
         case AERO_LBM_HYDRO_BOUNDARY_SYMMETRY: {
             const size_t s = SFX(cell_index)(c, SFX(iclamp)(sx, 0, c->nx - 1), SFX(iclamp)(sy, 0, c->ny - 1),
                                              SFX(iclamp)(sz, 0, c->nz - 1));
@@ -870,6 +888,58 @@ ORACLE_API void SFX(oracle_get_temperature)(void* h, float* t) {
     for (size_t i = 0; i < c->cells; ++i) t[i] = (float)c->T[i];
 }
 
+/* cylinder of the crossflow preset (jni.cpp:190-228): geometry in binary32 as the
+ * reference defines it, diameter 0.1/2.2 of the length (at least 2.5 cells, at
+ * most the height - 4), centre at (0.2/2.2 of the length, 0.2/0.41 of the height) */
+static int SFX(cylinder_solid)(const SFX(Ctx) * c, int x, int y) {
+    const float len = (float)(c->nx - 1 > 1 ? c->nx - 1 : 1);
+    const float h = (float)(c->ny - 1 > 1 ? c->ny - 1 : 1);
+    float d = 0.1f / 2.2f * len;
+    d = d < 2.5f ? 2.5f : d;
+    const float dmax = (h - 4.0f) < 2.5f ? 2.5f : (h - 4.0f);
+    d = dmax < d ? dmax : d;
+    const float r = 0.5f * d;
+    const float cx = 0.2f / 2.2f * len;
+    const float cy = 0.0f + 0.2f / 0.41f * h;
+    const float dx = (float)x - cx, dy = (float)y - cy;
+    return dx * dx + dy * dy <= r * r;
+}
+
+/* compute_benchmark_force (jni.cpp:2452-2492): momentum exchange 2 f*_q c_q over the
+ * in-plane links (c_z = 0) from fluid cells into solid cylinder cells, summed in
+ * double in cell and q order, divided by the depth */
+static void SFX(benchmark_force)(SFX(Ctx) * c) {
+    c->force[0] = c->force[1] = c->force[2] = 0.0f;
+    if (!c->cylinder) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int x = 0; x < c->nx; ++x)
+        for (int y = 0; y < c->ny; ++y)
+            for (int z = 0; z < c->nz; ++z) {
+                const size_t cell = SFX(cell_index)(c, x, y, z);
+                if (c->obstacle[cell]) continue;
+                for (int q = 1; q < 27; ++q) {
+                    if (kCZ[q] != 0) continue;
+                    const int ax = x + kCX[q], ay = y + kCY[q], az = z + kCZ[q];
+                    if (ax < 0 || ay < 0 || az < 0 || ax >= c->nx || ay >= c->ny || az >= c->nz) continue;
+                    if (!c->obstacle[SFX(cell_index)(c, ax, ay, az)]) continue;
+                    if (!SFX(cylinder_solid)(c, ax, ay)) continue;
+                    const double fq = (double)c->f_post[(size_t)q * c->cells + cell];
+                    fx += 2.0 * fq * (double)kCX[q];
+                    fy += 2.0 * fq * (double)kCY[q];
+                    fz += 2.0 * fq * (double)kCZ[q];
+                }
+            }
+    const double inv_depth = c->nz > 0 ? (1.0 / (double)c->nz) : 1.0;
+    c->force[0] = (float)(fx * inv_depth);
+    c->force[1] = (float)(fy * inv_depth);
+    c->force[2] = (float)(fz * inv_depth);
+}
+
+ORACLE_API void SFX(oracle_get_force)(void* h, float* out3) {
+    const SFX(Ctx)* c = (const SFX(Ctx)*)h;
+    for (int k = 0; k < 3; ++k) out3[k] = c->force[k];
+}
+
 /* run_cpu_step (jni.cpp:8480-8507). `packet` may be NULL after the first step,
  * meaning "same payload as last step" (the cached-step semantics). */
 ORACLE_API int SFX(oracle_step)(void* h, const float* packet, float* out, float out_scale) {
@@ -879,6 +949,7 @@ ORACLE_API int SFX(oracle_step)(void* h, const float* packet, float* out, float 
     if (!c->initialized) SFX(initialize)(c);
     SFX(update_temperature)(c);
     SFX(collide)(c);
+    SFX(benchmark_force)(c);
     SFX(stream)(c);
     if (out) {
         for (size_t cell = 0; cell < c->cells; ++cell) {

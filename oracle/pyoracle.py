@@ -49,6 +49,7 @@ def load():
         getattr(lib, p + "oracle_get_velocity").argtypes = [c_void_p, R]
         getattr(lib, p + "oracle_get_temperature_r").argtypes = [c_void_p, R]
         getattr(lib, p + "oracle_get_params").argtypes = [c_void_p, R]
+        getattr(lib, p + "oracle_get_force").argtypes = [c_void_p, POINTER(c_float)]
         getattr(lib, p + "oracle_collide_cell").argtypes = [c_void_p, R, real, R]
         getattr(lib, p + "oracle_feq").argtypes = [c_int, real, real, real, real]
         getattr(lib, p + "oracle_feq").restype = real
@@ -147,6 +148,12 @@ class Oracle:
         out = np.zeros(3 * self.cells, dtype=self.real)
         self._f("oracle_get_velocity")(self.h, out.ctypes.data_as(POINTER(np.ctypeslib.as_ctypes_type(self.real))))
         return out.reshape(self.cells, 3)
+
+    def force(self):
+        """Last step's drag integral (fx, fy, fz); zero unless the cylinder preset runs."""
+        out = np.zeros(3, dtype=np.float32)
+        self._f("oracle_get_force")(self.h, _fp(out))
+        return out
 
     def params(self):
         out = np.zeros(6, dtype=self.real)

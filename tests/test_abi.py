@@ -76,6 +76,14 @@ def test_heated_cavity_preset_matches_reference_preset(three_d):
     assert bytes(got) == bytes(want)
 
 
+def test_cylinder_preset_matches_reference_preset():
+    import workloads as W
+    lib = A.load()
+    got = A.BenchmarkConfig()
+    assert lib.aero_lbm_benchmark_default_preset_config(A.PRESET_CYLINDER_CROSSFLOW_2D, ctypes.byref(got)) == 1
+    assert bytes(got) == bytes(W.cylinder_config())
+
+
 def test_overrides_validation():
     lib = A.load()
     ov = A.default_overrides()

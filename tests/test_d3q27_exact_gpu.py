@@ -103,11 +103,45 @@ def test_exact_heated_cavity_bfecc(oracle_mod, lib, three_d, length, steps):
     ctx.release()
 
 
-def test_heated_cavity_rejects_fast_kernels(lib):
-    dims = (12, 8, 8)
-    log("heated cavity with the fast kernels: step is refused")
-    cfg = W.heated_cavity_config(True)
-    packet, t = heated_cavity_case(dims)
+def last_force(lib, key):
+    f = [ctypes.c_float(), ctypes.c_float(), ctypes.c_float()]
+    A.check(lib.aero_lbm_get_last_force(key, *[ctypes.byref(v) for v in f]), "get_last_force")
+    return np.array([v.value for v in f], np.float32)
+
+
+@pytest.mark.parametrize("dims,steps", [((48, 21, 3), 60), ((90, 33, 2), 150)])
+def test_exact_cylinder_crossflow_flow_and_drag(oracle_mod, lib, dims, steps):
+    log(f"exact cylinder crossflow {dims} {steps} steps (parabolic inlet, drag)")
+    cfg = W.cylinder_config()
+    solid = W.cylinder_mask(*dims)
+    packet = W.make_packet(solid, u0=(0.08, 0.0, 0.0), perturb=1e-3, seed_u=3)
+    ov = W.reference_overrides(A.KERNEL_EXACT)
+    ctx = setup(lib, None, cfg, ov=ov, dims=dims)
+    o = oracle_mod.Oracle(*dims, cfg, overrides=ov, precision=32)
+    ctx.step(packet)
+    o.step(packet)
+    fg, fo = last_force(lib, ctx.key), o.force()
+    assert np.allclose(fg, fo, rtol=1e-6, atol=1e-9), (fg, fo)
+    for _ in range(steps - 1):
+        ctx.step(None)
+        o.step(None)
+    assert np.array_equal(ctx.flow(), o.flow())
+    fg, fo = last_force(lib, ctx.key), o.force()
+    assert np.allclose(fg, fo, rtol=1e-6, atol=1e-9), (fg, fo)
+    assert abs(fo[0]) > 1e-5  # the drag integral is live (its sign is transient this early)
+    ctx.release()
+
+
+@pytest.mark.parametrize("preset", ["heated_cavity", "cylinder"])
+def test_presets_reject_fast_kernels(lib, preset):
+    dims = (12, 9, 8)
+    log(f"{preset} with the fast kernels: step is refused")
+    if preset == "heated_cavity":
+        cfg = W.heated_cavity_config(True)
+        packet, t = heated_cavity_case(dims)
+    else:
+        cfg = W.cylinder_config()
+        packet = W.make_packet(W.cylinder_mask(*dims), u0=(0.08, 0.0, 0.0))
     ov = W.reference_overrides(A.KERNEL_FAST)
     A.check(lib.aero_lbm_init_rect(*dims, 11, 4), "init_rect")
     A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "set_config")
