@@ -117,6 +117,26 @@ def test_ragged_z_rows(oracle_mod, lib, dims, variant):
     assert np.max(np.abs(tb.reshape(-1) - o.temperature())) <= 1e-5
 
 
+def test_fast_population_array_size_limit(lib):
+    """The fast kernels address a population array through one buffer resource with
+    32-bit byte offsets: FP32 in place takes 27 planes x 4 B per cell, so a brick of
+    20 x 1024 x 1001 cells (2.2 GB) is refused with a message and 19 x 1024 x 1000
+    (2.10 GB) is accepted."""
+    cfg = W.channel_config(0.04, 40.0, sgs=False, buoyancy=False)
+    ov = W.reference_overrides(A.KERNEL_FAST)
+    for dims, ok in (((20, 1024, 1001), False), ((19, 1024, 1000), True)):
+        log(f"size limit {dims} accepted={ok}")
+        ctx = make_ctx(lib, dims, cfg, ov, A.SOLVER_D3Q27_FP32_INPLACE, key=41)
+        t = np.zeros(ctx.cells, np.float32)
+        if ok:
+            ctx.set_temperature(t)
+            assert np.array_equal(ctx.temperature()[:1000], t[:1000])
+        else:
+            with pytest.raises(A.AeroLbmError, match="too large for the fast kernels"):
+                ctx.set_temperature(t)
+        ctx.release()
+
+
 def velocity_error(flow, ref_u, solid, u_in):
     f = flow.reshape(-1, 4)[:, :3].astype(np.float64)
     fluid = solid.reshape(-1) == 0
