@@ -39,8 +39,7 @@ MODES = {
 }
 
 
-# Algorithmic bytes per lattice update (the survey's figure): every population read
-# once and written once, one flag byte per cell, the temperature scalar read and
+# Algorithmic bytes per lattice update: every population read once and written once, one flag byte per cell, the temperature scalar read and
 # written once. The velocity carried to the next step's temperature advection
 # (3 x 2 B read + written for 16-bit storage) is real traffic this figure leaves out.
 B_LUP = {"fp16": 27 * 2 * 2 + 1 + 8, "bf16": 27 * 2 * 2 + 1 + 8, "fp32": 27 * 4 * 2 + 1 + 8,
