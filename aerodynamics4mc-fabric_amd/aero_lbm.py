@@ -30,6 +30,7 @@ SOLVER_COMPACT_EXPERIMENTAL = 2
 SOLVER_D3Q27_FP16_INPLACE = 3
 SOLVER_D3Q27_BF16_INPLACE = 4  # extension
 SOLVER_D3Q27_FP32_INPLACE = 5  # extension
+SOLVER_D3Q27_S16_INPLACE = 6  # extension: 16-bit fixed point of f - w_q
 
 PRESET_NONE = 0
 PRESET_TAYLOR_GREEN_3D = 1

@@ -68,7 +68,10 @@ std::unordered_map<long long, std::unique_ptr<Batch>> g_batches;
 long long g_next_batch = 1;
 
 FastStorage kind_of(Storage s) {
-    return s == Storage::F16 ? FastStorage::F16 : (s == Storage::BF16 ? FastStorage::BF16 : FastStorage::F32);
+    return s == Storage::F16    ? FastStorage::F16
+           : s == Storage::BF16 ? FastStorage::BF16
+           : s == Storage::S16  ? FastStorage::S16
+                                : FastStorage::F32;
 }
 void* aa_buf(Context& c) { return c.storage == Storage::F32AA ? (void*)c.f32[0] : (void*)c.h16; }
 int flip(int layout) { return layout == 1 ? 2 : 1; }
@@ -339,8 +342,8 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_batch_create(const long long* keys,
             set_error("batch_create: context " + std::to_string(keys[i]) + " missing or never stepped");
             return 0;
         }
-        if (c->storage != Storage::F16 && c->storage != Storage::BF16 && c->storage != Storage::F32AA) {
-            set_error("batch_create: batches take the in-place storages (FP16, BF16, FP32 A-A) only");
+        if (!storage_16bit(c->storage) && c->storage != Storage::F32AA) {
+            set_error("batch_create: batches take the in-place storages (FP16, BF16, S16, FP32 A-A) only");
             return 0;
         }
         if (i > 0 && c->storage != b->members[0].ctx->storage) {

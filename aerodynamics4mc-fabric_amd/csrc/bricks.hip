@@ -154,6 +154,7 @@ bool experimental(const World& w) {
         case AERO_LBM_REALTIME_SOLVER_COMPACT_EXPERIMENTAL:
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL:
         case AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE:
+        case AERO_LBM_REALTIME_SOLVER_D3Q27_S16_INPLACE:
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP32_INPLACE:
             return true;
         case AERO_LBM_REALTIME_SOLVER_AUTO: {

@@ -313,7 +313,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     float g[27];
     if (Mode == 1 || local_first) {
 #pragma unroll
-        for (int q = 0; q < 27; ++q) g[q] = St::template bld<kPolDefault>(rin, zb[1], plane_row<T>(Ni, q, R.row));
+        for (int q = 0; q < 27; ++q) g[q] = St::template bld<kPolDefault>(rin, zb[1], plane_row<T>(Ni, q, R.row), q);
     } else {
         // Each link has two candidate sources: the neighbour's slot (streaming) and
         // the cell's own slot of the opposite direction (halfway bounce-back, when
@@ -330,7 +330,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
             const int so = min(sa, sb);
             const bool sp = (links >> q) & 1u;
             const int voff = sp ? zb[1] + (sb - so) : zb[dz + 1] + (sa - so);
-            g[q] = St::template bld<kPolDefault>(rin, voff, so);
+            g[q] = St::template bld<kPolDefault>(rin, voff, so, q);
         }
     }
     // x face planes flagged xface_bulk (inlet, outflow, symmetry, Dirichlet / Neumann
@@ -361,21 +361,21 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     if (Mode != 2) {
 #pragma unroll
         for (int q = 0; q < 27; ++q)
-            St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, Mode == 0 ? q : lat_opp(q), R2.row), g[q]);
+            St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, Mode == 0 ? q : lat_opp(q), R2.row), g[q], q);
     } else {
         // push to x + c_q; a target outside the domain is dropped (row: skipped, z:
         // out-of-range offset), a solid target receives a value no fluid cell reads
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const int dx = lat_cx(q), dy = lat_cy(q), dz = lat_cz(q);
-            if (R2.ok(dx, dy)) St::template bst<StPol>(rout, zb[dz + 1], plane_row<T>(Ni, q, R2.off(dx, dy)), g[q]);
+            if (R2.ok(dx, dy)) St::template bst<StPol>(rout, zb[dz + 1], plane_row<T>(Ni, q, R2.off(dx, dy)), g[q], q);
         }
         // target solid or outside: bounce back into the cell's own slot opp(q)
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const bool sp = (links >> lat_opp(q)) & 1u;
             if (__builtin_amdgcn_ballot_w64(sp)) {
-                if (sp) St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, lat_opp(q), R2.row), g[q]);
+                if (sp) St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, lat_opp(q), R2.row), g[q], q);
             }
         }
     }
@@ -434,12 +434,12 @@ __device__ __forceinline__ void gather_body(const StepParams& P, const CellField
         const typename St::T* const src = Mode == 0 ? fin : fout;
         const bool sp = (links >> q) & 1u;
         if (Mode == 1 || local_first) {
-            v = St::ld(src + q * Np + cell);
+            v = St::ld(src + q * Np + cell, q);
         } else {
             // special link: the cell's own post-collision value of opp(q) (halfway bounce-back)
             const int qs = Mode == 0 ? (sp ? lat_opp(q) : q) : (sp ? q : lat_opp(q));
             const int s = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
-            v = St::ld(src + qs * Np + s);
+            v = St::ld(src + qs * Np + s, q);
         }
         if ((m & kMaskFace) && !local_first && sp && link_oob(P, x, y, z, q)) {
             const PrevPost<St, Mode != 0> prev{fin, store_in, Np};
@@ -499,17 +499,17 @@ __device__ __forceinline__ void update_body(const StepParams& P, const CellField
     if (Mode != 0 && P.face_store_faces) face_store(P, x, y, z, g, store_out);
     if (Mode == 0) {
 #pragma unroll
-        for (int q = 0; q < 27; ++q) St::st(fout + q * Np + cell, g[q]);
+        for (int q = 0; q < 27; ++q) St::st(fout + q * Np + cell, g[q], q);
     } else if (Mode == 1) {
 #pragma unroll
-        for (int q = 0; q < 27; ++q) St::st(fout + lat_opp(q) * Np + cell, g[q]);
+        for (int q = 0; q < 27; ++q) St::st(fout + lat_opp(q) * Np + cell, g[q], q);
     } else {
         const Nbr nb = make_nbr(P, x, y, z);
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const bool sp = (links >> lat_opp(q)) & 1u;  // target x + c_q solid or out of the domain
             const int d = sp ? cell : cell + nb.off(lat_cx(q), lat_cy(q), lat_cz(q));
-            St::st(fout + (sp ? lat_opp(q) : q) * Np + d, g[q]);
+            St::st(fout + (sp ? lat_opp(q) : q) * Np + d, g[q], q);
         }
     }
 }
@@ -544,12 +544,12 @@ __device__ __forceinline__ void cell_output_fast(const StepParams& P, const Cell
         const bool sp = (links >> q) & 1u;
         if (layout == 0) {
             const int src = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
-            g[q] = St::ld(f + (size_t)(sp ? lat_opp(q) : q) * Np + src);
+            g[q] = St::ld(f + (size_t)(sp ? lat_opp(q) : q) * Np + src, q);
         } else if (layout == 1) {
             const int src = sp ? cell : cell + nb.off(-lat_cx(q), -lat_cy(q), -lat_cz(q));
-            g[q] = St::ld(f + (size_t)(sp ? q : lat_opp(q)) * Np + src);
+            g[q] = St::ld(f + (size_t)(sp ? q : lat_opp(q)) * Np + src, q);
         } else {
-            g[q] = St::ld(f + (size_t)q * Np + cell);
+            g[q] = St::ld(f + (size_t)q * Np + cell, q);
         }
     }
     if ((m & kMaskFace) && layout != 3) {
@@ -714,7 +714,7 @@ __global__ void __launch_bounds__(256) init_kernel(StepParams P, CellFields C, t
         uz = C.ref_u[2 * N + cell];
     }
 #pragma unroll
-    for (int q = 0; q < 27; ++q) St::st(f + (size_t)q * P.plane + cell, feq_s<St::kShifted>(q, rho, ux, uy, uz));
+    for (int q = 0; q < 27; ++q) St::st(f + (size_t)q * P.plane + cell, feq_s<St::kShifted>(q, rho, ux, uy, uz), q);
     float t = T[cell];
     if (fabsf(t) < 1e-8f) t = clampf(t_init, -1.0f, 1.0f);
     else if (!solid) t = clampf(C.ref_t[cell], -1.0f, 1.0f);
@@ -862,6 +862,10 @@ void launch_fast_init(const StepParams& P, const CellFields& C, FastStorage st, 
             hipLaunchKernelGGL((init_kernel<StBF16, UF16>), g1(P.cells), dim3(256), 0, s, P, C, (uint16_t*)f, T,
                                T_other, (uint16_t*)u_state, t_init);
             break;
+        case FastStorage::S16:
+            hipLaunchKernelGGL((init_kernel<StS16, UF16>), g1(P.cells), dim3(256), 0, s, P, C, (uint16_t*)f, T,
+                               T_other, (uint16_t*)u_state, t_init);
+            break;
     }
 }
 
@@ -969,6 +973,10 @@ void launch_fast_step(const StepParams& P, const CellFields& C, FastStorage st, 
             launch_step_t<StBF16, UF16>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, mode,
                                         local_first, rare, s);
             break;
+        case FastStorage::S16:
+            launch_step_t<StS16, UF16>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in, store_out, mode,
+                                       local_first, rare, s);
+            break;
     }
 }
 
@@ -1017,6 +1025,7 @@ void launch_batch_step(FastStorage st, int mode, int store_policy_, bool shell, 
         case FastStorage::F32: batch_step_s<fast::StF32, fast::UF32>(mode, store_policy_, shell, table, g, s); break;
         case FastStorage::F16: batch_step_s<fast::StF16, fast::UF16>(mode, store_policy_, shell, table, g, s); break;
         case FastStorage::BF16: batch_step_s<fast::StBF16, fast::UF16>(mode, store_policy_, shell, table, g, s); break;
+        case FastStorage::S16: batch_step_s<fast::StS16, fast::UF16>(mode, store_policy_, shell, table, g, s); break;
     }
 }
 
@@ -1027,6 +1036,7 @@ void launch_batch_publish(FastStorage st, const BatchBrick* table, const BatchGr
         case FastStorage::F32: hipLaunchKernelGGL(fast::publish_batch<fast::StF32>, grid, dim3(256), 0, s, table, g.publish_owner); break;
         case FastStorage::F16: hipLaunchKernelGGL(fast::publish_batch<fast::StF16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
         case FastStorage::BF16: hipLaunchKernelGGL(fast::publish_batch<fast::StBF16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
+        case FastStorage::S16: hipLaunchKernelGGL(fast::publish_batch<fast::StS16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
     }
 }
 
@@ -1047,6 +1057,10 @@ void launch_fast_readout(const StepParams& P, const CellFields& C, FastStorage s
             break;
         case FastStorage::BF16:
             hipLaunchKernelGGL(readout_kernel<StBF16>, g, dim3(256), 0, s, P, C, mask, (const uint16_t*)f, store,
+                               layout, stride, sx, sy, sz, out_ch, scale, out);
+            break;
+        case FastStorage::S16:
+            hipLaunchKernelGGL(readout_kernel<StS16>, g, dim3(256), 0, s, P, C, mask, (const uint16_t*)f, store,
                                layout, stride, sx, sy, sz, out_ch, scale, out);
             break;
     }

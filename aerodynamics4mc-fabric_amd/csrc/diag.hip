@@ -19,11 +19,13 @@ __device__ __forceinline__ float wgt(int q) {
     return n == 0 ? 8.0f / 27.0f : (n == 1 ? 2.0f / 27.0f : (n == 2 ? 1.0f / 54.0f : 1.0f / 216.0f));
 }
 
-// storage: 0 FP32, 1 FP16 shifted, 2 BF16 shifted; flip: slot s holds direction opp(s)
-__device__ __forceinline__ float slot_value(const void* f, int storage, size_t off) {
+// storage: 0 FP32, 1 FP16 shifted, 2 BF16 shifted, 3 16-bit fixed point shifted
+// (step s16_class_range(|c|^2) * w / 32767); flip: slot s holds direction opp(s)
+__device__ __forceinline__ float slot_value(const void* f, int storage, size_t off, int q) {
     if (storage == 0) return ((const float*)f)[off];
     const uint16_t b = ((const uint16_t*)f)[off];
     if (storage == 1) return (float)__builtin_bit_cast(_Float16, b);
+    if (storage == 3) return (float)(int16_t)b * (s16_class_range(lat_c2(q)) * wgt(q) / 32767.0f);
     return __uint_as_float((uint32_t)b << 16);
 }
 
@@ -41,7 +43,7 @@ __global__ void __launch_bounds__(kBlock) partial_kernel(const void* f, int stor
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
             const int q = flip ? lat_opp(s) : s;  // direction stored in slot s
-            float v = slot_value(f, storage, (size_t)s * plane + cell);
+            float v = slot_value(f, storage, (size_t)s * plane + cell, q);
             if (storage != 0) v += wgt(q);
             const double d = (double)v;
             acc[0] += d;
@@ -94,9 +96,10 @@ bool storage_conservation(Context& c, double* out4) {
             break;
         case Storage::F16:
         case Storage::BF16:
+        case Storage::S16:
         case Storage::F32AA:
             f = c.storage == Storage::F32AA ? (const void*)c.f32[0] : (const void*)c.h16;
-            storage = c.storage == Storage::F16 ? 1 : (c.storage == Storage::BF16 ? 2 : 0);
+            storage = c.storage == Storage::F16 ? 1 : (c.storage == Storage::BF16 ? 2 : (c.storage == Storage::S16 ? 3 : 0));
             plane = (size_t)fast_plane_stride((int)c.cells);
             flip = c.aa_layout == 1 ? 1 : 0;  // after an even step, slot opp(q) holds direction q
             break;

@@ -29,37 +29,40 @@ constexpr int kPolDefault = 0;
 constexpr int kPolStream = 2;
 
 // ---------------------------------------------------------------- storage traits
+// Every access names the population slot q it touches. Only the 16-bit fixed-point
+// format uses it (its step depends on the direction class of q, and a slot and
+// its opposite belong to the same class, so a value may sit in either).
 struct StF32 {
     using T = float;
     static constexpr bool kShifted = false;
-    static __device__ __forceinline__ float ld(const T* p) { return __builtin_nontemporal_load(p); }
-    static __device__ __forceinline__ void st(T* p, float v) { __builtin_nontemporal_store(v, p); }
+    static __device__ __forceinline__ float ld(const T* p, int) { return __builtin_nontemporal_load(p); }
+    static __device__ __forceinline__ void st(T* p, float v, int) { __builtin_nontemporal_store(v, p); }
     template <int Pol>
-    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff, int) {
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, Pol));
     }
     template <int Pol>
-    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v, int) {
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, soff, Pol);
     }
 };
 struct StF16 {
     using T = uint16_t;
     static constexpr bool kShifted = true;
-    static __device__ __forceinline__ float ld(const T* p) {
+    static __device__ __forceinline__ float ld(const T* p, int) {
         const uint16_t b = __builtin_nontemporal_load(p);
         return (float)__builtin_bit_cast(_Float16, b);
     }
-    static __device__ __forceinline__ void st(T* p, float v) {
+    static __device__ __forceinline__ void st(T* p, float v, int) {
         __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, (_Float16)v), p);
     }
     template <int Pol>
-    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff, int) {
         const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, Pol);
         return (float)__builtin_bit_cast(_Float16, b);
     }
     template <int Pol>
-    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v, int) {
         __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), r, off, soff, Pol);
     }
 };
@@ -71,19 +74,52 @@ struct StBF16 {
         u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
         return (uint16_t)(u >> 16);
     }
-    static __device__ __forceinline__ float ld(const T* p) {
+    static __device__ __forceinline__ float ld(const T* p, int) {
         const uint32_t b = __builtin_nontemporal_load(p);
         return __uint_as_float(b << 16);
     }
-    static __device__ __forceinline__ void st(T* p, float v) { __builtin_nontemporal_store(pack(v), p); }
+    static __device__ __forceinline__ void st(T* p, float v, int) { __builtin_nontemporal_store(pack(v), p); }
     template <int Pol>
-    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff, int) {
         const uint32_t b = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, Pol);
         return __uint_as_float(b << 16);
     }
     template <int Pol>
-    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v, int) {
         __builtin_amdgcn_raw_buffer_store_b16(pack(v), r, off, soff, Pol);
+    }
+};
+// 16-bit signed fixed point of the shifted value g = f - w_q: g = k * R_c * w_q / 32767
+// with k in [-32767, 32767] and a range R_c per direction class c = |c_q|^2
+// (s16_range). The step is uniform inside the range, so a small deviation from
+// w_q keeps an absolute error of R_c w_q / 65534 instead of FP16's relative
+// 2^-11; values beyond the range saturate. Encoding is one v_cvt_pknorm_i16_f32
+// (round to nearest of clamp(x, -1, 1) * 32767) of g / (R_c w_q).
+__device__ constexpr float s16_range(int q) { return s16_class_range(lat_c2(q)); }
+__device__ constexpr float s16_w(int q) {
+    return lat_c2(q) == 0 ? 8.0f / 27.0f : (lat_c2(q) == 1 ? 2.0f / 27.0f : (lat_c2(q) == 2 ? 1.0f / 54.0f : 1.0f / 216.0f));
+}
+struct StS16 {
+    using T = uint16_t;
+    static constexpr bool kShifted = true;
+    static __device__ __forceinline__ float step(int q) { return s16_range(q) * s16_w(q) / 32767.0f; }
+    static __device__ __forceinline__ float inv_range(int q) { return 1.0f / (s16_range(q) * s16_w(q)); }
+    static __device__ __forceinline__ uint16_t pack(float v, int q) {
+        const auto k = __builtin_amdgcn_cvt_pknorm_i16(v * inv_range(q), 0.0f);
+        return (uint16_t)k.x;
+    }
+    static __device__ __forceinline__ float unpack(uint32_t b, int q) {
+        return (float)(int)(int16_t)(uint16_t)b * step(q);
+    }
+    static __device__ __forceinline__ float ld(const T* p, int q) { return unpack(__builtin_nontemporal_load(p), q); }
+    static __device__ __forceinline__ void st(T* p, float v, int q) { __builtin_nontemporal_store(pack(v, q), p); }
+    template <int Pol>
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff, int q) {
+        return unpack(__builtin_amdgcn_raw_buffer_load_b16(r, off, soff, Pol), q);
+    }
+    template <int Pol>
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v, int q) {
+        __builtin_amdgcn_raw_buffer_store_b16(pack(v, q), r, off, soff, Pol);
     }
 };
 
@@ -169,7 +205,7 @@ struct PrevPost {
     const float* store;       // A-A: face store of the previous step
     size_t N;
     __device__ __forceinline__ float get(const StepParams& P, int face, int q, int sx, int sy, int sz) const {
-        if (!AA) return St::ld(f + (size_t)q * N + (size_t)((sx * P.ny + sy) * P.nz + sz));
+        if (!AA) return St::ld(f + (size_t)q * N + (size_t)((sx * P.ny + sy) * P.nz + sz), q);
         const int axis = face >> 1;
         const int layer = (face & 1) ? ((axis == 0 ? P.nx : axis == 1 ? P.ny : P.nz) - 1 -
                                         (axis == 0 ? sx : axis == 1 ? sy : sz))

@@ -5,7 +5,9 @@
 
 namespace aero {
 
-enum class FastStorage { F32, F16, BF16 };
+// population storage of the fast kernels: FP32, and 16-bit shifted (f - w_q) as
+// IEEE half, bfloat16, or signed fixed point with a range per direction class
+enum class FastStorage { F32, F16, BF16, S16 };
 
 // per-cell link mask: bit q (0..26) = the pull source x - c_q is solid or out of the
 // domain after periodic wrapping; plus the two flags below

@@ -19,6 +19,18 @@ __host__ __device__ constexpr int lat_cx(int q) { return q / 9 - 1; }
 __host__ __device__ constexpr int lat_cy(int q) { return (q / 3) % 3 - 1; }
 __host__ __device__ constexpr int lat_cz(int q) { return q % 3 - 1; }
 __host__ __device__ constexpr int lat_opp(int q) { return 26 - q; }
+__host__ __device__ constexpr int lat_c2(int q) {
+    return lat_cx(q) * lat_cx(q) + lat_cy(q) * lat_cy(q) + lat_cz(q) * lat_cz(q);
+}
+
+// Range of the 16-bit fixed-point population storage per direction class |c|^2
+// (rest, axis, face diagonal, corner), in units of the class weight: a stored
+// shifted value f - w_q covers [-R w_q, R w_q]. An equilibrium at density 1 stays
+// inside it up to a speed of about 0.19 along a corner diagonal (Mach 0.33) and
+// 0.26 along an axis.
+__host__ __device__ constexpr float s16_class_range(int c2) {
+    return c2 == 0 ? 0.25f : (c2 == 1 ? 1.0f : (c2 == 2 ? 1.25f : 1.5f));
+}
 
 // cell classification bits (one byte per cell)
 enum : uint8_t {

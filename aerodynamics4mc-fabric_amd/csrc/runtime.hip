@@ -432,6 +432,7 @@ static Storage storage_for_mode(int mode, int variant) {
     switch (mode) {
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL: return Storage::F16;
         case AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE: return Storage::BF16;
+        case AERO_LBM_REALTIME_SOLVER_D3Q27_S16_INPLACE: return Storage::S16;
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP32_INPLACE: return Storage::F32AA;
         default: return variant == AERO_LBM_KERNEL_EXACT ? Storage::F32Exact : Storage::F32Fast;
     }
@@ -958,6 +959,7 @@ AERO_LBM_CAPI_EXPORT void aero_lbm_set_realtime_solver_mode(int mode) {
         case AERO_LBM_REALTIME_SOLVER_COMPACT_EXPERIMENTAL:
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL:
         case AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE:
+        case AERO_LBM_REALTIME_SOLVER_D3Q27_S16_INPLACE:
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP32_INPLACE:
             rt().solver_mode = mode;
             break;

@@ -14,7 +14,10 @@
 
 namespace aero {
 
-enum class Storage { F32Exact, F32Fast, F16, BF16, F32AA };
+enum class Storage { F32Exact, F32Fast, F16, BF16, S16, F32AA };
+
+// 16-bit shifted populations (FP16, BF16, fixed point)
+inline bool storage_16bit(Storage s) { return s == Storage::F16 || s == Storage::BF16 || s == Storage::S16; }
 
 constexpr int kMaskSigLen = 22;
 
@@ -56,7 +59,7 @@ struct Context {
     double* force_part = nullptr;  // per-block drag partials, then the float force (cylinder preset)
     bool force_valid = false;      // force_part ends with this context's last drag integral
     float* f32[2] = {nullptr, nullptr};  // FP32 A-B populations (27 planes)
-    uint16_t* h16 = nullptr;             // FP16/BF16 A-A populations (27 planes, shifted by w_q)
+    uint16_t* h16 = nullptr;             // 16-bit A-A populations (27 planes, shifted by w_q)
     float* face_store[2] = {nullptr, nullptr};  // A-A: near-face post-collision values per parity
     size_t face_store_floats = 0;
     float* payload_dev = nullptr;

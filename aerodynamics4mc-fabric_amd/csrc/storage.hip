@@ -8,7 +8,10 @@
 namespace aero {
 
 static FastStorage fast_kind(Storage s) {
-    return s == Storage::F16 ? FastStorage::F16 : (s == Storage::BF16 ? FastStorage::BF16 : FastStorage::F32);
+    return s == Storage::F16    ? FastStorage::F16
+           : s == Storage::BF16 ? FastStorage::BF16
+           : s == Storage::S16  ? FastStorage::S16
+                                : FastStorage::F32;
 }
 
 // link mask and rare-cell list of the fast kernels
@@ -28,7 +31,7 @@ bool storage_alloc(Context& c) {
     const size_t Np = (size_t)fast_plane_stride((int)c.cells);  // fast kernels' plane stride
     if (c.storage != Storage::F32Exact) {
         // the fast kernels address each population array through one buffer resource
-        const size_t bytes = 27 * Np * (c.storage == Storage::F16 || c.storage == Storage::BF16 ? 2 : 4);
+        const size_t bytes = 27 * Np * (storage_16bit(c.storage) ? 2 : 4);
         if (bytes > (size_t)kFastMaxBufferBytes) {
             set_error("brick too large for the fast kernels (population array above 2 GiB); "
                       "use the exact kernel variant or smaller bricks");
@@ -43,6 +46,7 @@ bool storage_alloc(Context& c) {
                    alloc_fast_aux(c);
         case Storage::F16:
         case Storage::BF16:
+        case Storage::S16:
             return c.alloc((void**)&c.h16, 27 * Np * 2) && alloc_fast_aux(c);
         case Storage::F32AA:
             return c.alloc((void**)&c.f32[0], 27 * Np * 4) && alloc_fast_aux(c);
@@ -61,7 +65,8 @@ size_t context_device_bytes(const Context& c) {
         case Storage::F32Exact: b += 2 * 27 * N * 4; break;
         case Storage::F32Fast: b += 2 * 27 * Np * 4 + N * 4; break;
         case Storage::F16:
-        case Storage::BF16: b += 27 * Np * 2 + N * 4; break;
+        case Storage::BF16:
+        case Storage::S16: b += 27 * Np * 2 + N * 4; break;
         case Storage::F32AA: b += 27 * Np * 4 + N * 4; break;
     }
     b += 2 * c.face_store_floats * 4;
@@ -207,6 +212,7 @@ bool storage_readout(Context& c, const StepParams& P, int stride, int out_ch, fl
             break;
         case Storage::F16:
         case Storage::BF16:
+        case Storage::S16:
         case Storage::F32AA:
             launch_fast_readout(P, C, fast_kind(c.storage), c.mask, aa_buffer(c), c.face_store[c.store_cur],
                                 c.aa_layout, stride, out_ch, scale, out_dev, st);

@@ -34,6 +34,7 @@ MODES = {
     # name: (solver mode, kernel variant, bytes per population)
     "fp16": ("fp16", 2),
     "bf16": ("bf16", 2),
+    "s16": ("s16", 2),
     "fp32": ("fp32_fast", 4),
     "fp32_exact": ("fp32_exact", 4),
 }
@@ -42,7 +43,7 @@ MODES = {
 # Algorithmic bytes per lattice update: every population read once and written once, one flag byte per cell, the temperature scalar read and
 # written once. The velocity carried to the next step's temperature advection
 # (3 x 2 B read + written for 16-bit storage) is real traffic this figure leaves out.
-B_LUP = {"fp16": 27 * 2 * 2 + 1 + 8, "bf16": 27 * 2 * 2 + 1 + 8, "fp32": 27 * 4 * 2 + 1 + 8,
+B_LUP = {"fp16": 27 * 2 * 2 + 1 + 8, "bf16": 27 * 2 * 2 + 1 + 8, "s16": 27 * 2 * 2 + 1 + 8, "fp32": 27 * 4 * 2 + 1 + 8,
          "fp32_exact": 27 * 4 * 2 + 1 + 8}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "d3q27_step_128_traffic.json")
 
@@ -66,8 +67,8 @@ def measured_traffic(mode: str):
 def set_mode(lib, A, W, mode: str, ov):
     ov.kernel_variant = A.KERNEL_EXACT if mode == "fp32_exact" else A.KERNEL_FAST
     A.check(lib.aero_lbm_set_solver_overrides(ctypes.byref(ov)), "set_solver_overrides")
-    lib.aero_lbm_set_realtime_solver_mode({"fp16": A.SOLVER_D3Q27_FP16_INPLACE, "bf16": A.SOLVER_D3Q27_BF16_INPLACE}
-                                          .get(mode, A.SOLVER_CLASSIC_D3Q27))
+    lib.aero_lbm_set_realtime_solver_mode({"fp16": A.SOLVER_D3Q27_FP16_INPLACE, "bf16": A.SOLVER_D3Q27_BF16_INPLACE,
+                                           "s16": A.SOLVER_D3Q27_S16_INPLACE}.get(mode, A.SOLVER_CLASSIC_D3Q27))
 
 
 def run_mode(lib, A, W, mode, n, steps, warmup, inputs):
@@ -299,7 +300,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--modes", default="fp16,fp32")
+    ap.add_argument("--modes", default="fp16,fp32,s16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-meso", action="store_true", help="skip the D3Q19 mesoscale line")
     ap.add_argument("--meso-steps", type=int, default=500)

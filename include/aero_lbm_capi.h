@@ -36,15 +36,18 @@ enum {
     AERO_LBM_SIMULATION_BRICK_RUNTIME_STATUS_FIELDS = 8
 };
 
-/* ref: native/include/aero_lbm_capi.h:28-33; value 4 is an extension
- * (BF16 equilibrium-shifted in-place storage). */
+/* ref: native/include/aero_lbm_capi.h:28-33; values 4-6 are extensions
+ * (BF16, FP32 and 16-bit fixed-point in-place storage). */
 enum {
     AERO_LBM_REALTIME_SOLVER_AUTO = 0,
     AERO_LBM_REALTIME_SOLVER_CLASSIC_D3Q27 = 1,
     AERO_LBM_REALTIME_SOLVER_COMPACT_EXPERIMENTAL = 2,
     AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL = 3,
     AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE = 4,
-    AERO_LBM_REALTIME_SOLVER_D3Q27_FP32_INPLACE = 5  /* extension: FP32 unshifted A-A */
+    AERO_LBM_REALTIME_SOLVER_D3Q27_FP32_INPLACE = 5, /* extension: FP32 unshifted A-A */
+    /* extension: 16-bit signed fixed point of f - w_q, range per direction class
+     * (rest 0.25, axis 1.0, face diagonal 1.25, corner 1.5 times w_q) */
+    AERO_LBM_REALTIME_SOLVER_D3Q27_S16_INPLACE = 6
 };
 
 /* ref: native/include/aero_lbm_capi.h:35-43 */

@@ -204,3 +204,42 @@ def test_fp32_tracks_fp64_on_terrain(oracle_mod):
     assert np.max(np.abs(a[:, :3] - b[:, :3])) < 2e-3      # kelvin
     assert np.max(np.abs(a[:, 3:] - b[:, 3:])) < 1e-4       # m/s (wind ~0.5 m/s)
     assert np.max(np.abs(b[:, 3:])) > 0.05                  # the wind field is not trivial
+
+
+@pytest.mark.parametrize("tau", [0.65, 1.3])
+def test_trt_shear_wave_decays_with_even_viscosity(oracle_mod, tau):
+    """Two-relaxation-time collision away from unit relaxation (meso.cpp:548-738):
+    a shear wave u_x = A sin(k z) in a quiescent atmosphere decays as exp(-nu k^2 t)
+    with nu = (tau_even - 1/2)/3, whatever the odd rate (magic parameter 1/4 gives
+    tau_odd = 0.5 + 0.25/(tau_even - 0.5), which would give a very different decay).
+    The background wind of the forcing equals the initial wave (so the momentum
+    relaxation toward it, dt/60 per step, acts only on the decay itself, a 1e-7
+    effect at dt = 1 ms), the seeding puts the equilibrium of that wind in every
+    cell, and the amplitude is read at the domain centre after steps 10 and 30:
+    populations move at most one cell per step, so a centre more than 30 cells from
+    every face and the floor sees no boundary."""
+    nx, ny, nz, lam = 64, 64, 128, 64
+    dx, dt = 1.0, 1.0e-3
+    nu_l = (tau - 0.5) / 3.0
+    o = oracle_mod.MesoOracle(nx, ny, nz, dx=dx, dt=dt, nu=nu_l * dx * dx / dt, pr=0.71, prt=0.85, tau_min=0.51,
+                              precision=64)
+    vs = dx / dt
+    k = 2.0 * np.pi / lam
+    z = np.arange(nz)
+    F = np.zeros((nx, ny, nz, 25), np.float32)
+    F[..., 2] = T0
+    F[..., 3] = T0
+    F[..., 4] = T0
+    F[..., 6] = (0.01 * np.sin(k * (z - 32)) * vs)[None, None, :]
+    zs = np.arange(32, 32 + lam)
+    basis = np.sin(k * (zs - 32))
+    amp = {}
+    for s in range(1, 31):
+        out = o.step(F).reshape(nx, ny, nz, 6)
+        if s in (10, 30):
+            ux = out[nx // 2, ny // 2, zs, 3].astype(np.float64) / vs
+            amp[s] = float(np.dot(ux, basis) / np.dot(basis, basis))
+    nu_eff = -np.log(amp[30] / amp[10]) / (20.0 * k * k)
+    assert nu_eff == pytest.approx(nu_l, rel=5e-3)
+    tau_odd = 0.5 + 0.25 / (tau - 0.5)
+    assert abs(nu_eff - (tau_odd - 0.5) / 3.0) > 0.1 * nu_l
