@@ -233,11 +233,7 @@ def run_config3_128(lib, mode, steps, key=41):
 H128 = {"fp32": A.SOLVER_CLASSIC_D3Q27, "fp16": A.SOLVER_D3Q27_FP16_INPLACE, "s16": A.SOLVER_D3Q27_S16_INPLACE}
 
 
-@pytest.mark.parametrize("name,bar", [("fp32", 1e-4),
-                                      pytest.param("fp16", 1e-3, marks=pytest.mark.xfail(
-                                          reason="IEEE half storage rounding misses 1e-3 of the inflow speed",
-                                          strict=False)),
-                                      ("s16", 1e-3)])
+@pytest.mark.parametrize("name,bar", [("fp32", 1e-4), ("fp16", 1e-3), ("s16", 1e-3)])
 def test_config3_128_horizon_vs_fp64_fixture(lib, name, bar):
     """The 128^3 workload (C=0.1, beta 0.1, gravity -z, 10% solid boxes, hot blob on
     the floor, inflow 0.05) against the FP64 oracle at 4087 seeded probes after 100,
