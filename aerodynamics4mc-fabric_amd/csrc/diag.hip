@@ -4,6 +4,8 @@
 // workgroup through LDS, and one final workgroup adds the per-workgroup
 // partials in a fixed order, so the result does not depend on scheduling.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "fast_kernels.h"
 #include "runtime_internal.h"
@@ -80,44 +82,106 @@ __global__ void __launch_bounds__(64) final_kernel(const double* __restrict__ pa
 
 }  // namespace diag
 
-// mass and momentum of a context's populations; out4 receives {mass, jx, jy, jz}
-bool storage_conservation(Context& c, double* out4) {
-    hipStream_t st = rt().stream;
+// Fixed-size device scratch of the allocation-free entry point: the per-workgroup
+// partials (at most kMaxBlocks x 4) followed by the four totals. One copy per
+// device, held until the process ends. In host code `scratch` names a host-side
+// stand-in; its device address comes from hipGetSymbolAddress (scratch_ptr).
+constexpr int kScratchDoubles = diag::kMaxBlocks * 4 + 4;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// device address of `scratch` on the current device, looked up once per device
+static double* scratch_ptr() {
+    constexpr int kMaxDevices = 64;
+    static double* ptrs[kMaxDevices] = {};
+    static std::mutex mu;
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("get_conservation_noalloc: device index out of range");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return nullptr;
+        ptrs[dev] = (double*)p;
+    }
+    return ptrs[dev];
+}
+
+struct SlotView {
     const void* f = nullptr;
     int storage = 0, flip = 0;
-    size_t plane = c.cells;
+    size_t plane = 0;
+};
+
+static SlotView slot_view(const Context& c) {
+    SlotView v;
+    v.plane = c.cells;
     switch (c.storage) {
         case Storage::F32Exact:
-            f = c.f32[c.f_cur];
+            v.f = c.f32[c.f_cur];
             break;
         case Storage::F32Fast:
-            f = c.f32[c.f_cur];
-            plane = (size_t)fast_plane_stride((int)c.cells);
+            v.f = c.f32[c.f_cur];
+            v.plane = (size_t)fast_plane_stride((int)c.cells);
             break;
         case Storage::F16:
         case Storage::BF16:
         case Storage::S16:
         case Storage::F32AA:
-            f = c.storage == Storage::F32AA ? (const void*)c.f32[0] : (const void*)c.h16;
-            storage = c.storage == Storage::F16 ? 1 : (c.storage == Storage::BF16 ? 2 : (c.storage == Storage::S16 ? 3 : 0));
-            plane = (size_t)fast_plane_stride((int)c.cells);
-            flip = c.aa_layout == 1 ? 1 : 0;  // after an even step, slot opp(q) holds direction q
+            v.f = c.storage == Storage::F32AA ? (const void*)c.f32[0] : (const void*)c.h16;
+            v.storage = c.storage == Storage::F16 ? 1 : (c.storage == Storage::BF16 ? 2 : (c.storage == Storage::S16 ? 3 : 0));
+            v.plane = (size_t)fast_plane_stride((int)c.cells);
+            v.flip = c.aa_layout == 1 ? 1 : 0;  // after an even step, slot opp(q) holds direction q
             break;
     }
-    const int blocks = (int)std::min<size_t>(diag::kMaxBlocks, (c.cells + diag::kBlock - 1) / diag::kBlock);
+    return v;
+}
+
+static int conservation_blocks(const Context& c) {
+    return (int)std::min<size_t>(diag::kMaxBlocks, (c.cells + diag::kBlock - 1) / diag::kBlock);
+}
+
+// partial sums into `partial` (blocks x 4), totals into `out_dev` (4), copy to out4
+static bool conservation_into(Context& c, double* partial, double* out_dev, double* out4) {
+    hipStream_t st = rt().stream;
+    const SlotView v = slot_view(c);
+    const int blocks = conservation_blocks(c);
+    hipLaunchKernelGGL(diag::partial_kernel, dim3(blocks), dim3(diag::kBlock), 0, st, v.f, v.storage, v.flip, v.plane,
+                       (int)c.cells, c.flags, partial);
+    hipLaunchKernelGGL(diag::final_kernel, dim3(1), dim3(64), 0, st, partial, blocks, out_dev);
+    return hip_ok(hipGetLastError(), "conservation") &&
+           hip_ok(hipMemcpyAsync(out4, out_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, st), "readback");
+}
+
+// mass and momentum of a context's populations; out4 receives {mass, jx, jy, jz}
+bool storage_conservation(Context& c, double* out4) {
+    hipStream_t st = rt().stream;
+    const int blocks = conservation_blocks(c);
     double* partial = nullptr;
     double* out_dev = nullptr;
     if (!hip_ok(hipMallocAsync((void**)&partial, (size_t)blocks * 4 * sizeof(double), st), "hipMallocAsync") ||
         !hip_ok(hipMallocAsync((void**)&out_dev, 4 * sizeof(double), st), "hipMallocAsync"))
         return false;
-    hipLaunchKernelGGL(diag::partial_kernel, dim3(blocks), dim3(diag::kBlock), 0, st, f, storage, flip, plane,
-                       (int)c.cells, c.flags, partial);
-    hipLaunchKernelGGL(diag::final_kernel, dim3(1), dim3(64), 0, st, partial, blocks, out_dev);
-    bool ok = hip_ok(hipGetLastError(), "conservation") &&
-              hip_ok(hipMemcpyAsync(out4, out_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, st), "readback");
+    const bool ok = conservation_into(c, partial, out_dev, out4);
     (void)hipFreeAsync(partial, st);
     (void)hipFreeAsync(out_dev, st);
     return ok && hip_ok(hipStreamSynchronize(st), "sync");
+}
+
+// the same sums through the fixed device scratch: no allocation
+static bool storage_conservation_noalloc(Context& c, double* out4) {
+    const int blocks = conservation_blocks(c);
+    if (blocks * 4 + 4 > kScratchDoubles) {
+        set_error("get_conservation_noalloc: " + std::to_string(blocks) + " partial sums exceed the scratch of " +
+                  std::to_string(diag::kMaxBlocks));
+        return false;
+    }
+    double* base = scratch_ptr();
+    if (!base) return false;
+    return conservation_into(c, base, base + diag::kMaxBlocks * 4, out4) &&
+           hip_ok(hipStreamSynchronize(rt().stream), "sync");
 }
 
 }  // namespace aero
@@ -138,4 +202,20 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_get_conservation_rect(int nx, int n
         return 0;
     }
     return storage_conservation(*c, out4) ? 1 : 0;
+}
+
+extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_get_conservation_rect_noalloc(int nx, int ny, int nz, long long key,
+                                                                           double* out4) {
+    std::lock_guard<std::mutex> lk(rt().mu);
+    clear_error();
+    if (!out4) {
+        set_error("get_conservation_noalloc: null output");
+        return 0;
+    }
+    Context* c = find_context(key);
+    if (!c || !c->initialized || c->nx != nx || c->ny != ny || c->nz != nz) {
+        set_error("get_conservation_noalloc: no initialised context of these dimensions for this key");
+        return 0;
+    }
+    return storage_conservation_noalloc(*c, out4) ? 1 : 0;
 }

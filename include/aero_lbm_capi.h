@@ -262,6 +262,14 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_get_timing_snapshot(AeroLbmTimingSnapshot* out
  * Total mass and momentum {m, jx, jy, jz} of the fluid cells' populations as
  * stored (lattice units), summed in FP64 on the device in a fixed order. ---- */
 AERO_LBM_CAPI_EXPORT int aero_lbm_get_conservation_rect(int nx, int ny, int nz, long long context_key, double* out4);
+/* No counterpart in the reference. The same sums without any device allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the partial sums live in a fixed 64 KiB device
+ * array (2048 x 4 partials + 4 totals) that each GPU holds until the process ends.
+ * Any context size fits (the partial kernel loops over cells). Calls that overlap on
+ * one GPU share that array and race; every partial is rewritten by each call. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_get_conservation_rect_noalloc(int nx, int ny, int nz, long long context_key,
+                                                               double* out4);
 
 /* ---- boundary-reference shell of the in-place paths
  * (ref: native/include/aero_lbm_capi.h:183-194, native/src/aero_lbm_jni.cpp:9962-10030).
