@@ -21,7 +21,7 @@ __device__ __forceinline__ float finite_or0(float v) { return __builtin_isfinite
 __global__ void __launch_bounds__(256) ingest_kernel(int cells, int ch, const float* __restrict__ payload,
                                                      uint8_t* __restrict__ flags, float* __restrict__ ref_p,
                                                      float* __restrict__ ref_u, float* __restrict__ ref_t,
-                                                     float* __restrict__ src) {
+                                                     float* __restrict__ src, float* __restrict__ fan) {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= cells) return;
     const float* p = payload + (size_t)cell * ch;
@@ -32,6 +32,12 @@ __global__ void __launch_bounds__(256) ingest_kernel(int cells, int ch, const fl
     ref_p[cell] = clampf_ref(p[8], -0.03f, 0.03f);
     ref_t[cell] = clampf_ref(ch > 10 ? p[10] : 0.0f, -1.0f, 1.0f);
     src[cell] = clampf_ref(ch > 9 ? p[9] : 0.0f, -0.006f, 0.006f);
+    if (fan) {
+        fan[cell] = clampf_ref(p[1], 0.0f, 1.0f);
+        fan[(size_t)cells + cell] = finite_or0(p[2]);
+        fan[2 * (size_t)cells + cell] = finite_or0(p[3]);
+        fan[3 * (size_t)cells + cell] = finite_or0(p[4]);
+    }
 }
 
 __device__ __forceinline__ bool wrap_coord(int& c, int n, int wrap_lo, int wrap_hi) {
@@ -67,9 +73,9 @@ __global__ void __launch_bounds__(256) classify_kernel(StepParams P, uint8_t* __
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
 void launch_ingest(const StepParams& P, const float* payload, uint8_t* flags, float* ref_p, float* ref_u,
-                   float* ref_t, float* src, hipStream_t s) {
+                   float* ref_t, float* src, float* fan, hipStream_t s) {
     hipLaunchKernelGGL(ingest_kernel, grid_for(P.cells), dim3(256), 0, s, P.cells, P.in_ch, payload, flags, ref_p,
-                       ref_u, ref_t, src);
+                       ref_u, ref_t, src, fan);
 }
 
 void launch_classify(const StepParams& P, uint8_t* flags, hipStream_t s) {

@@ -266,6 +266,81 @@ __device__ __forceinline__ void decode_cell(const StepParams& P, int cell, int& 
     z = rem - y * P.nz;
 }
 
+// distance of a cell to the nearest face of the brick
+__device__ __forceinline__ int face_distance(const StepParams& P, int x, int y, int z) {
+    int d = x;
+    d = y < d ? y : d;
+    d = z < d ? z : d;
+    d = (P.nx - 1 - x) < d ? (P.nx - 1 - x) : d;
+    d = (P.ny - 1 - y) < d ? (P.ny - 1 - y) : d;
+    d = (P.nz - 1 - z) < d ? (P.nz - 1 - z) : d;
+    return d;
+}
+
+// effective_runtime_state_nudge (ref: jni.cpp:683-693): 0.08 (1 - d/8)^2 within 8 cells
+// of a face in runtime mode, 0 in benchmark mode
+__device__ __forceinline__ float state_nudge(const StepParams& P, int x, int y, int z) {
+    if (!P.runtime_mode) return 0.0f;
+    const int d = face_distance(P, x, y, z);
+    if (d >= 8) return 0.0f;
+    const float eta = (float)(8 - d) / (float)8;
+    return 0.08f * eta * eta;
+}
+
+// velocity_component_or_self (ref: jni.cpp:1952-1969): a face neighbour's collision
+// velocity, or the cell's own outside the domain (no periodic wrap) or in a solid
+__device__ __forceinline__ float vel_or_self(const StepParams& P, const uint8_t* flags, const float* u, int x, int y,
+                                             int z, float self) {
+    if (x < 0 || y < 0 || z < 0 || x >= P.nx || y >= P.ny || z >= P.nz) return self;
+    const int n = cell_of(P, x, y, z);
+    if (flags[n] & kCellSolid) return self;
+    return u[n];
+}
+
+// runtime thermal diffusivity (ref: jni.cpp:1971-2048): alpha0 + nu_t / 0.85 with the
+// Smagorinsky viscosity C^2 |S| (capped at 5 nu0) of the central-difference strain of
+// the previous step's collision velocities; nu_t = 0 next to a solid cell
+__device__ __forceinline__ float runtime_alpha(const StepParams& P, const uint8_t* flags, const float* u_state,
+                                               int cell, int x, int y, int z) {
+    float nu_t = 0.0f;
+    bool adjacent = false;
+    {
+        const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int a = x + off[k][0], b = y + off[k][1], d = z + off[k][2];
+            if (a < 0 || b < 0 || d < 0 || a >= P.nx || b >= P.ny || d >= P.nz) continue;
+            if (flags[cell_of(P, a, b, d)] & kCellSolid) adjacent = true;
+        }
+    }
+    if (P.sgs_on && !adjacent) {
+        const size_t N = (size_t)P.cells;
+        const float* ux = u_state;
+        const float* uy = u_state + N;
+        const float* uz = u_state + 2 * N;
+        const float cx = ux[cell], cy = uy[cell], cz = uz[cell];
+        const float dux_dx = 0.5f * (vel_or_self(P, flags, ux, x + 1, y, z, cx) - vel_or_self(P, flags, ux, x - 1, y, z, cx));
+        const float dux_dy = 0.5f * (vel_or_self(P, flags, ux, x, y + 1, z, cx) - vel_or_self(P, flags, ux, x, y - 1, z, cx));
+        const float dux_dz = 0.5f * (vel_or_self(P, flags, ux, x, y, z + 1, cx) - vel_or_self(P, flags, ux, x, y, z - 1, cx));
+        const float duy_dx = 0.5f * (vel_or_self(P, flags, uy, x + 1, y, z, cy) - vel_or_self(P, flags, uy, x - 1, y, z, cy));
+        const float duy_dy = 0.5f * (vel_or_self(P, flags, uy, x, y + 1, z, cy) - vel_or_self(P, flags, uy, x, y - 1, z, cy));
+        const float duy_dz = 0.5f * (vel_or_self(P, flags, uy, x, y, z + 1, cy) - vel_or_self(P, flags, uy, x, y, z - 1, cy));
+        const float duz_dx = 0.5f * (vel_or_self(P, flags, uz, x + 1, y, z, cz) - vel_or_self(P, flags, uz, x - 1, y, z, cz));
+        const float duz_dy = 0.5f * (vel_or_self(P, flags, uz, x, y + 1, z, cz) - vel_or_self(P, flags, uz, x, y - 1, z, cz));
+        const float duz_dz = 0.5f * (vel_or_self(P, flags, uz, x, y, z + 1, cz) - vel_or_self(P, flags, uz, x, y, z - 1, cz));
+        const float sxx = dux_dx, syy = duy_dy, szz = duz_dz;
+        const float sxy = 0.5f * (dux_dy + duy_dx);
+        const float sxz = 0.5f * (dux_dz + duz_dx);
+        const float syz = 0.5f * (duy_dz + duz_dy);
+        const float smag = __builtin_sqrtf(rmax(0.0f, 2.0f * (sxx * sxx + syy * syy + szz * szz) +
+                                                          4.0f * (sxy * sxy + sxz * sxz + syz * syz)));
+        nu_t = P.sgs_c2 * smag;
+        nu_t = rmin(nu_t, P.nut_cap_ratio * P.nu_rt0);
+        nu_t = rmax(0.0f, nu_t);
+    }
+    return P.alpha + nu_t / rmax(1.0e-6f, 0.85f);
+}
+
 __global__ void __launch_bounds__(256) bfecc_forward_kernel(StepParams P, const uint8_t* __restrict__ flags,
                                                             const float* __restrict__ T,
                                                             const float* __restrict__ u_state, float* __restrict__ S) {
@@ -275,7 +350,7 @@ __global__ void __launch_bounds__(256) bfecc_forward_kernel(StepParams P, const 
     const size_t N = (size_t)P.cells;
     int x, y, z;
     decode_cell(P, cell, x, y, z);
-    const float dt = 1.0f;
+    const float dt = P.thermal_dt;
     S[cell] = trilinear(P, T, flags, (float)x - dt * u_state[cell], (float)y - dt * u_state[N + cell],
                         (float)z - dt * u_state[2 * N + cell], T[cell]);
 }
@@ -290,7 +365,7 @@ __global__ void __launch_bounds__(256) bfecc_backward_kernel(StepParams P, const
     const size_t N = (size_t)P.cells;
     int x, y, z;
     decode_cell(P, cell, x, y, z);
-    const float dt = 1.0f;
+    const float dt = P.thermal_dt;
     const float tc = T[cell];
     const float back = trilinear(P, S, flags, (float)x + dt * u_state[cell], (float)y + dt * u_state[N + cell],
                                  (float)z + dt * u_state[2 * N + cell], S[cell]);
@@ -308,7 +383,7 @@ __global__ void __launch_bounds__(256) bfecc_correct_kernel(StepParams P, const 
     const size_t N = (size_t)P.cells;
     int x, y, z;
     decode_cell(P, cell, x, y, z);
-    const float dt = 1.0f;
+    const float dt = P.thermal_dt;
     const float tc = T[cell];
     float adv = trilinear(P, Nf, flags, (float)x - dt * u_state[cell], (float)y - dt * u_state[N + cell],
                           (float)z - dt * u_state[2 * N + cell], Nf[cell]);
@@ -327,7 +402,26 @@ __global__ void __launch_bounds__(256) bfecc_correct_kernel(StepParams P, const 
     const float lap = tsum - 6.0f * tc;
     float source = 0.0f;
     if (P.source_on && P.in_ch > 9) source = clampr(src[cell], -0.006f, 0.006f);
-    out[cell] = clampr(adv + dt * (P.alpha * lap + source - 0.0f * adv), -1.0f, 1.0f);
+    const float alpha = P.runtime_mode ? runtime_alpha(P, flags, u_state, cell, x, y, z) : P.alpha;
+    out[cell] = clampr(adv + dt * (alpha * lap + source - 0.0f * adv), -1.0f, 1.0f);
+}
+
+// runtime mode, every step after the first (ref: jni.cpp:8037-8064): solid cells to 0,
+// cells within 8 of a face relaxed toward the payload temperature by the state nudge
+__global__ void __launch_bounds__(256) t_reference_kernel(StepParams P, CellFields C, float* __restrict__ t) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= P.cells) return;
+    if (C.flags[cell] & kCellSolid) {
+        t[cell] = 0.0f;
+        return;
+    }
+    int x, y, z;
+    decode_cell(P, cell, x, y, z);
+    const float tn = state_nudge(P, x, y, z);
+    if (tn <= 0.0f) return;
+    const float rt = clampr(C.ref_t[cell], -1.0f, 1.0f);
+    const float keep = 1.0f - tn;
+    t[cell] = clampr(t[cell] * keep + rt * tn, -1.0f, 1.0f);
 }
 
 // ---------------------------------------------------------------- drag of the cylinder preset (jni.cpp:2452-2492)
@@ -481,6 +575,19 @@ __device__ __forceinline__ float tau_from_nu(float nu) {
     return 0.5f + 3.0f * clampr(nu, 1.0e-8f, nu_max);
 }
 
+// hash_signed_noise (ref: jni.cpp:1318-1327): SplitMix64 finalizer of cell and step
+// counter, the low 24 bits mapped to [-1, 1]
+__device__ __forceinline__ float fan_noise(unsigned long long cell, unsigned long long tick) {
+    unsigned long long x = (cell * 0x9E3779B97F4A7C15ull) ^ (tick * 0xD1B54A32D192ED03ull);
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    const float u = (float)(x & 0x00FFFFFFull) * (1.0f / 16777215.0f);
+    return 2.0f * u - 1.0f;
+}
+
 struct CollideOut {
     float ux, uy, uz;
     bool reseeded;
@@ -536,19 +643,64 @@ __device__ __forceinline__ void collide(const StepParams& P, const CellFields& C
     const float inv_rho = 1.0f / rho_safe;
     float ux = mx * inv_rho, uy = my * inv_rho, uz = mz * inv_rho;
     if (!fin(rho_safe) || !fin(inv_rho) || !fin(ux) || !fin(uy) || !fin(uz)) { reseed(P, C, cell, fo, o); return; }
-    // buoyancy along the gravity axis; the other components stay +0.0f as in the reference
     float F[3] = {0.0f, 0.0f, 0.0f};
+    bool fan_force = false;
+    if (P.fan_on) {
+        // fan forcing (ref: jni.cpp:2258-2286): push toward the fan speed along its axis,
+        // damp the perpendicular velocity, soften above a speed cap
+        const size_t N = (size_t)P.cells;
+        const float mask = C.fan[cell];
+        if (mask > 0.0f) {
+            const float fu = C.fan[N + cell], fv = C.fan[2 * N + cell], fw = C.fan[3 * N + cell];
+            const float speed_pre = __builtin_sqrtf(ux * ux + uy * uy + uz * uz);
+            const float fan_norm = __builtin_sqrtf(fu * fu + fv * fv + fw * fw);
+            if (fan_norm > 1e-8f) {
+                const float inv_norm = 1.0f / fan_norm;
+                const float noise = 1.0f + P.fan_noise_amp * fan_noise((unsigned long long)cell, P.tick);
+                const float target = clampr(fan_norm * 0.05f * rmax(0.0f, noise), 0.0f, 0.28f);
+                const float nx = fu * inv_norm, ny = fv * inv_norm, nz = fw * inv_norm;
+                const float u_para = ux * nx + uy * ny + uz * nz;
+                const float upx = ux - u_para * nx, upy = uy - u_para * ny, upz = uz - u_para * nz;
+                const float push = rmax(0.0f, target - u_para);
+                float damp = 1.0f;
+                if (speed_pre > 0.24f) {
+                    const float r = (speed_pre - 0.24f) / rmax(1e-4f, 0.04f);
+                    damp = 1.0f / (1.0f + r * r);
+                }
+                const float fb = mask * 0.07f * damp;
+                F[0] = fb * rho_safe * (push * nx - 1.0f * upx);
+                F[1] = fb * rho_safe * (push * ny - 1.0f * upy);
+                F[2] = fb * rho_safe * (push * nz - 1.0f * upz);
+                fan_force = true;
+            }
+        }
+    }
+    // buoyancy along the gravity axis, added to the fan force; components the
+    // reference never touches stay +0.0f
     if (P.buoyancy_on) {
         const float b = clampr(P.beta * T, -0.02f, 0.02f);
-        const float fa = 0.0f + (P.g_sign < 0 ? rho_safe * b : rho_safe * (-b));
-        F[0] = P.g_axis == 0 ? fa : 0.0f;
-        F[1] = P.g_axis == 1 ? fa : 0.0f;
-        F[2] = P.g_axis == 2 ? fa : 0.0f;
+        const float fa = P.g_sign < 0 ? rho_safe * b : rho_safe * (-b);
+        if (fan_force) {
+            F[P.g_axis == 0 ? 0 : (P.g_axis == 1 ? 1 : 2)] += fa;
+        } else {
+            const float fz0 = 0.0f + fa;
+            F[0] = P.g_axis == 0 ? fz0 : 0.0f;
+            F[1] = P.g_axis == 1 ? fz0 : 0.0f;
+            F[2] = P.g_axis == 2 ? fz0 : 0.0f;
+        }
     }
     ux += 0.5f * (F[0] * inv_rho);
     uy += 0.5f * (F[1] * inv_rho);
     uz += 0.5f * (F[2] * inv_rho);
-    // benchmark mode: the state nudge is zero and adds nothing (ref: jni.cpp:679-681, 2306-2314)
+    if (P.runtime_mode) {
+        // state nudge toward the reference velocity (ref: jni.cpp:2306-2314); in
+        // benchmark mode it is zero and adds nothing
+        const size_t N = (size_t)P.cells;
+        const float sn = state_nudge(P, x, y, z);
+        ux = (1.0f - sn) * ux + sn * C.ref_u[cell];
+        uy = (1.0f - sn) * uy + sn * C.ref_u[N + cell];
+        uz = (1.0f - sn) * uz + sn * C.ref_u[2 * N + cell];
+    }
     if (!fin(ux) || !fin(uy) || !fin(uz)) { reseed(P, C, cell, fo, o); return; }
     const float s2 = ux * ux + uy * uy + uz * uz;
     if (!fin(s2)) { reseed(P, C, cell, fo, o); return; }
@@ -692,7 +844,9 @@ __global__ void __launch_bounds__(256) init_kernel(StepParams P, CellFields C, f
 #pragma unroll
     for (int q = 0; q < 27; ++q) f[(size_t)q * N + cell] = feq(q, rho, ux, uy, uz);
     float t = T[cell];
-    if (fabsf(t) < 1e-8f) t = clampr(t_init, -1.0f, 1.0f);
+    // benchmark mode seeds a zero temperature with the configured initial one; otherwise a
+    // fluid cell takes the payload's temperature (ref: jni.cpp:1771-1776)
+    if (!P.runtime_mode && fabsf(t) < 1e-8f) t = clampr(t_init, -1.0f, 1.0f);
     else if (!solid) t = clampr(C.ref_t[cell], -1.0f, 1.0f);
     T[cell] = clampr(t, -1.0f, 1.0f);
     u_state[cell] = ux;
@@ -733,7 +887,10 @@ __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, c
     }
     float T = 0.0f;
     if (P.thermal_on) {
-        if (P.thermal_bfecc) {
+        if (!P.thermal_update) {
+            T = t_in[cell];  // runtime mode: no temperature update this step (stride 2)
+            t_out[cell] = T;
+        } else if (P.thermal_bfecc) {
             T = t_out[cell];  // the BFECC passes already wrote this step's temperature
         } else {
             T = update_temperature(P, t_in, C.flags, C.src, cell, x, y, z, u_state[cell], u_state[N + cell],
@@ -880,6 +1037,10 @@ void launch_exact_bfecc(const StepParams& P, const CellFields& C, const float* t
                        (const float*)t_out, u_state, t_mid);
     hipLaunchKernelGGL(exact::bfecc_correct_kernel, grid1(P.cells), dim3(256), 0, s, P, C.flags, C.src, t_in,
                        (const float*)t_mid, u_state, t_out);
+}
+
+void launch_exact_t_reference(const StepParams& P, const CellFields& C, float* t, hipStream_t s) {
+    hipLaunchKernelGGL(exact::t_reference_kernel, grid1(P.cells), dim3(256), 0, s, P, C, t);
 }
 
 void launch_exact_force(const StepParams& P, const CellFields& C, const float* fpost, double* part, hipStream_t s) {

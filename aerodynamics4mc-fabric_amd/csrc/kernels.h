@@ -18,11 +18,12 @@ struct CellFields {
     const float* src;      // payload channel 9 (clamped)
     const uint8_t* shell;         // boundary-reference shell layer per cell (0: outside)
     const uint16_t* shell_state;  // FP16 (u_x, u_y, u_z, p) targets of the shell, 4 per cell
+    const float* fan;             // payload channels 1-4 (mask, velocity), four planes; null without fans
 };
 
 // ---- payload ingest and cell classification (d3q27_common.hip) ----
 void launch_ingest(const StepParams& P, const float* payload, uint8_t* flags, float* ref_p, float* ref_u,
-                   float* ref_t, float* src, hipStream_t s);
+                   float* ref_t, float* src, float* fan, hipStream_t s);
 void launch_classify(const StepParams& P, uint8_t* flags, hipStream_t s);
 
 // ---- exact FP32 A-B path (d3q27_exact.hip): operation order of the reference CPU step ----
@@ -34,6 +35,9 @@ void launch_exact_step(const StepParams& P, const CellFields& C, const float* fi
 // cell-sized scratch field
 void launch_exact_bfecc(const StepParams& P, const CellFields& C, const float* t_in, float* t_out, float* t_mid,
                         const float* u_state, hipStream_t s);
+// runtime mode, every step after the first: solid cells' temperature to 0 and the
+// temperature within 8 cells of a face nudged toward the payload's (in place)
+void launch_exact_t_reference(const StepParams& P, const CellFields& C, float* t, hipStream_t s);
 // drag integral of the cylinder preset over post-collision populations: part holds
 // 3 doubles per 256-cell block, then the float force (fx, fy, fz)
 void launch_exact_force(const StepParams& P, const CellFields& C, const float* fpost, double* part, hipStream_t s);

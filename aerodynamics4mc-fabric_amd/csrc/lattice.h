@@ -97,6 +97,17 @@ struct StepParams {
     // cylinder crossflow preset: parabolic inlet on the x- velocity face and the
     // drag integral after each step (exact kernels)
     int cylinder;
+    // runtime (non-benchmark) mode of the reference (jni.cpp:675-681): velocity and
+    // temperature nudged toward the payload reference within 8 cells of every face,
+    // thermal diffusivity alpha0 + nu_t / Pr_t, temperature every second step (dt 2)
+    int runtime_mode;
+    int thermal_update;   // this step updates the temperature (step counter % stride == 0)
+    float thermal_dt;     // temperature time step: the update stride
+    float nu_rt0;         // runtime molecular viscosity (caps the thermal turbulent viscosity)
+    // fan forcing (jni.cpp:2258-2286): enabled and the payload has fan cells
+    int fan_on;
+    float fan_noise_amp;
+    unsigned long long tick;  // the context's step counter while fan forcing runs (noise seed)
     // device copy of this whole struct: rarely taken kernel paths read their
     // parameters from it, so the bulk path keeps no scalar registers for them
     const StepParams* self_dev;

@@ -257,11 +257,7 @@ static float reference_speed(const AeroLbmBenchmarkConfig& g) {
 bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
     RuntimeState& s = rt();
     const AeroLbmBenchmarkConfig& g = s.bench;
-    if (!g.enabled) {
-        set_error("runtime (non-benchmark) solver mode is not implemented by this build; "
-                  "enable a benchmark configuration (aero_lbm_benchmark_set_config)");
-        return false;
-    }
+    const bool runtime = !g.enabled;
     std::memset(&P, 0, sizeof(P));
     P.nx = nx;
     P.ny = ny;
@@ -286,7 +282,8 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
         P.thermal_wrap_hi[a] = faces[2 * a + 1]->thermal_kind == AERO_LBM_THERMAL_BOUNDARY_PERIODIC;
         P.hydro_periodic[a] = P.hydro_wrap_lo[a] && P.hydro_wrap_hi[a];
     }
-    auto flag = [&](uint32_t b) { return (g.flags & b) != 0; };
+    // benchmark flags apply in benchmark mode only (ref: jni.cpp:722-724)
+    auto flag = [&](uint32_t b) { return !runtime && (g.flags & b) != 0; };
     P.sgs_on = !flag(AERO_LBM_BENCHMARK_FLAG_DISABLE_SGS);
     P.buoyancy_on = !flag(AERO_LBM_BENCHMARK_FLAG_DISABLE_BUOYANCY);
     P.sponge_on = !flag(AERO_LBM_BENCHMARK_FLAG_DISABLE_SPONGE);
@@ -307,12 +304,33 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
     const float tau_min = 0.5f + 3.0f * 1.0e-8f;
     P.tau_shear_base = clampr_(tau_from_nu(nu), tau_min, 0.95f);
     P.tau_normal_base = clampr_(tau_from_nu(clamp_nu(nu)), tau_min, 0.95f);
+    P.nu_rt0 = nu_rt;
+    P.thermal_dt = 1.0f;
+    P.thermal_update = 1;
+    // ref: jni.cpp:799-809: fan forcing and its noise follow the flags in benchmark mode
+    P.fan_on = !flag(AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING);
+    P.fan_noise_amp = flag(AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_NOISE) ? 0.0f : 0.02f;
     // ref: jni.cpp:966-971 (heated cavity: nu / Pr), 986-993 (heated cavity: U^2 / (L dT))
     P.alpha = heated ? nu / pr : 0.035f;
     P.beta = heated ? (ref_speed * ref_speed) / rmaxf_(1.0e-6f, ref_len * dirichlet_temperature_span(g))
                     : ((9.81f * (1.0f / 300.0f)) * 20.0f) * (((1.0f / 20.0f) * (1.0f / 20.0f)) / 1.0f);
     P.thermal_bfecc = heated ? 1 : 0;  // ref: jni.cpp:871-873
     P.cylinder = g.preset == AERO_LBM_BENCHMARK_PRESET_CYLINDER_CROSSFLOW_2D ? 1 : 0;
+    if (runtime) {
+        // runtime mode (ref: jni.cpp:896-993): molecular viscosity nu0 and diffusivity of
+        // air at 1 m cells and 0.05 s steps, tau_normal = 0.5 + 3 nu0, BFECC every second
+        // step with dt 2, the Boussinesq beta above; cylinder and heated-cavity presets
+        // belong to benchmark mode
+        P.runtime_mode = 1;
+        P.nu_base = nu_rt;
+        P.tau_shear_base = clampr_(tau_from_nu(nu_rt), tau_min, 0.95f);
+        P.tau_normal_base = clampr_(0.5f + 3.0f * (1.50e-5f * (1.0f / 20.0f) / (1.0f * 1.0f)), tau_min, 0.95f);
+        P.alpha = rmaxf_(1.0e-8f, (1.50e-5f / 0.71f) * (1.0f / 20.0f) / (1.0f * 1.0f));
+        P.beta = ((9.81f * (1.0f / 300.0f)) * 20.0f) * (((1.0f / 20.0f) * (1.0f / 20.0f)) / 1.0f);
+        P.thermal_bfecc = 1;
+        P.thermal_dt = 2.0f;
+        P.cylinder = 0;
+    }
     float sgs_c = 0.025f;
     P.nut_cap_ratio = 5.0f;
     P.g_axis = 1;
@@ -429,6 +447,9 @@ bool Context::alloc(void** p, size_t bytes) {
 }
 
 static Storage storage_for_mode(int mode, int variant) {
+    // runtime (non-benchmark) mode runs on the exact FP32 kernels
+    if (!rt().bench.enabled && (mode == AERO_LBM_REALTIME_SOLVER_AUTO || mode == AERO_LBM_REALTIME_SOLVER_CLASSIC_D3Q27))
+        return Storage::F32Exact;
     switch (mode) {
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL: return Storage::F16;
         case AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE: return Storage::BF16;
@@ -494,7 +515,7 @@ Context* get_context(long long key, int nx, int ny, int nz, bool create) {
 }
 
 CellFields cell_fields(const Context& c) {
-    return CellFields{c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, c.shell, c.shell_state};
+    return CellFields{c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, c.shell, c.shell_state, c.fan};
 }
 
 static bool host_any_nonzero(const float* p, size_t n, size_t stride, size_t off) {
@@ -520,10 +541,12 @@ void update_thermal_activity(Context& c, const StepParams& P) {
 static bool ingest_payload(Context& c, const StepParams& P, const float* packet) {
     const size_t n = c.cells * (size_t)rt().cfg.in_ch;
     if (c.payload_host.size() == n && std::memcmp(c.payload_host.data(), packet, n * sizeof(float)) == 0) return true;
-    const AeroLbmBenchmarkConfig& g = rt().bench;
     const int ch = rt().cfg.in_ch;
-    if (!(g.flags & AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING) && host_any_nonzero(packet, c.cells, ch, 1)) {
-        set_error("fan forcing is not implemented by this build (set AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING)");
+    // fan cells in a payload while fan forcing is on: the force needs the fan planes
+    const bool fans = P.fan_on && host_any_nonzero(packet, c.cells, ch, 1);
+    if (fans && c.storage != Storage::F32Exact) {
+        set_error("fan forcing runs on the exact FP32 kernels: select AERO_LBM_KERNEL_EXACT with the classic D3Q27 "
+                  "solver mode, or set AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING");
         return false;
     }
     if (c.storage == Storage::F32Exact && c.initialized && c.post_collision) {
@@ -542,7 +565,9 @@ static bool ingest_payload(Context& c, const StepParams& P, const float* packet)
     hipStream_t st = rt().stream;
     if (!hip_ok(hipMemcpyAsync(c.payload_dev, packet, n * sizeof(float), hipMemcpyHostToDevice, st), "upload payload"))
         return false;
-    launch_ingest(P, c.payload_dev, c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, st);
+    if (fans && !c.fan && !c.alloc((void**)&c.fan, 4 * c.cells * sizeof(float))) return false;
+    c.fan_active = fans;
+    launch_ingest(P, c.payload_dev, c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, fans ? c.fan : nullptr, st);
     launch_classify(P, c.flags, st);
     if (!storage_rebuild_masks(c, P)) return false;
     const int w[6] = {P.hydro_wrap_lo[0], P.hydro_wrap_hi[0], P.hydro_wrap_lo[1],
@@ -551,6 +576,15 @@ static bool ingest_payload(Context& c, const StepParams& P, const float* packet)
     c.has_payload = true;
     c.timing_upload_bytes += n * sizeof(float);
     return hip_ok(hipGetLastError(), "ingest");
+}
+
+// Per-step switches that depend on the context's step counter: the temperature
+// update stride (ref: jni.cpp:2191-2196) and the fan forcing's noise seed.
+void step_switches(const Context& c, StepParams& P) {
+    const uint64_t stride = P.runtime_mode ? 2u : 1u;
+    P.thermal_update = (c.steps % stride) == 0 ? 1 : 0;
+    P.fan_on = (P.fan_on && c.fan_active) ? 1 : 0;
+    P.tick = P.fan_on ? (unsigned long long)c.steps : 0ull;
 }
 
 // One solver step of a context. packet == nullptr means "resident payload".
@@ -565,6 +599,7 @@ bool step_context(Context& c, const float* packet, float* out, int out_ch, float
     }
     update_thermal_activity(c, P);
     P.thermal_on = c.thermal_on ? 1 : 0;
+    step_switches(c, P);
     if (!sync_params(c, P)) return false;  // the step runs with (and records) the thermal switch just decided
     if (!storage_step(c, P)) return false;
     c.steps += 1;

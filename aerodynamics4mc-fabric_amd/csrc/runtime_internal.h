@@ -53,6 +53,8 @@ struct Context {
     float* ref_u = nullptr;
     float* ref_t = nullptr;
     float* src = nullptr;
+    float* fan = nullptr;          // payload fan mask and velocity (four planes), allocated when fans appear
+    bool fan_active = false;       // the resident payload has fan cells and fan forcing is on
     float* T[2] = {nullptr, nullptr};
     float* u_state = nullptr;
     float* t_bfecc = nullptr;      // corrected field of the BFECC passes (allocated on first use)
@@ -133,6 +135,7 @@ Context* find_context(long long key);
 Context* get_context(long long key, int nx, int ny, int nz, bool create);
 CellFields cell_fields(const Context& c);
 bool step_context(Context& c, const float* packet, float* out, int out_ch, float scale);
+void step_switches(const Context& c, StepParams& P);
 
 // storage-specific operations (storage.hip)
 bool storage_alloc(Context& c);

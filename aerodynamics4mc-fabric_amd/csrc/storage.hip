@@ -139,6 +139,11 @@ static bool ensure_face_store(Context& c, const StepParams& P) {
 bool storage_step(Context& c, const StepParams& P) {
     hipStream_t st = rt().stream;
     const CellFields C = cell_fields(c);
+    if (P.runtime_mode && c.storage != Storage::F32Exact) {
+        set_error("runtime (non-benchmark) mode runs on the exact FP32 kernels of the classic D3Q27 solver mode; "
+                  "this context was created for another storage: release it or set a benchmark configuration");
+        return false;
+    }
     if ((P.thermal_bfecc || P.cylinder) && c.storage != Storage::F32Exact) {
         set_error("heated-cavity and cylinder presets (BFECC temperature, inlet profile and drag) run on the "
                   "exact FP32 kernels: select AERO_LBM_KERNEL_EXACT with the classic D3Q27 solver mode");
@@ -150,7 +155,12 @@ bool storage_step(Context& c, const StepParams& P) {
             c.initialized = true;
             c.post_collision = false;
         }
-        if (P.thermal_on && P.thermal_bfecc) {
+        // runtime mode, after the first step: the payload temperature nudged in at the faces
+        // (ref: jni.cpp:8499-8503)
+        else if (P.runtime_mode && P.thermal_on && P.in_ch > 10) {
+            launch_exact_t_reference(P, C, c.T[c.t_cur], st);
+        }
+        if (P.thermal_on && P.thermal_bfecc && P.thermal_update) {
             if (!c.t_bfecc && !c.alloc((void**)&c.t_bfecc, c.cells * 4)) return false;
             launch_exact_bfecc(P, C, c.T[c.t_cur], c.T[1 - c.t_cur], c.t_bfecc, c.u_state, st);
         }

@@ -44,7 +44,14 @@ typedef struct SFX(Ctx) {
     R *ref_ux, *ref_uy, *ref_uz, *ref_p, *ref_T, *src;
     unsigned char* obstacle;
     R *T, *T_next, *T_scr;
-    int bfecc; /* BFECC thermal advection (heated-cavity presets) */
+    int bfecc; /* BFECC thermal advection (heated-cavity presets; runtime mode) */
+    /* runtime (non-benchmark) mode, jni.cpp:675-681: state and temperature nudge near
+     * the faces, fan forcing, turbulent thermal diffusivity, temperature every
+     * second step with dt 2 */
+    int runtime;
+    int thermal_stride;
+    R nu_rt0;   /* runtime molecular viscosity (lattice) */
+    R *fan_mask, *fan_ux, *fan_uy, *fan_uz;
     int cylinder; /* cylinder crossflow preset: parabolic inlet, drag integral */
     float force[3]; /* last step's drag integral (ctx.last_force) */
     R w[27];
@@ -178,14 +185,46 @@ static R SFX(temperature_span)(const AeroLbmBenchmarkConfig* g) {
 
 static int SFX(resolve)(SFX(Ctx) * c, const AeroLbmSolverOverrides* ov) {
     const AeroLbmBenchmarkConfig* g = &c->cfg;
-    if (!g->enabled) return 0; /* runtime mode is not restated here */
-    if (!(g->flags & AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING)) return 0;
+    /* runtime molecular nu0 = 1.5e-5 m^2/s * 0.05 s / 1 m^2 (jni.cpp:100-104) */
+    c->nu_rt0 = SFX(clamp_nu)(K(1.50e-5) * (K(1.0) / K(20.0)) / (K(1.0) * K(1.0)));
+    c->thermal_stride = 1;
+    if (!g->enabled) {
+        /* runtime mode (jni.cpp:675-681, 896-993): molecular viscosity and thermal
+         * diffusivity of air at 1 m cells and 0.05 s steps, tau_normal = tau(nu0),
+         * BFECC every second step (dt 2), Boussinesq beta of thermal_core.h:268-279 */
+        c->runtime = 1;
+        c->nu_base = c->nu_rt0;
+        const R tau_min = K(0.5) + K(3.0) * K(1.0e-8);
+        c->tau_shear_base = SFX(clampr)(SFX(tau_from_nu)(c->nu_base), tau_min, K(0.95));
+        c->tau_normal_base = SFX(clampr)(K(0.5) + K(3.0) * (K(1.50e-5) * (K(1.0) / K(20.0)) / (K(1.0) * K(1.0))),
+                                         tau_min, K(0.95));
+        const R alpha_phys = K(1.50e-5) / K(0.71); /* thermal_core.h:22-24 nu / Pr */
+        c->alpha = SFX(rmax)(K(1.0e-8), alpha_phys * (K(1.0) / K(20.0)) / (K(1.0) * K(1.0)));
+        c->beta = ((K(9.81) * (K(1.0) / K(300.0))) * K(20.0)) * (((K(1.0) / K(20.0)) * (K(1.0) / K(20.0))) / K(1.0));
+        c->bfecc = 1;
+        c->thermal_stride = 2;
+        c->cylinder = 0;
+        R sgs_c = K(0.025);
+        c->nut_cap_ratio = K(5.0);
+        c->g_axis = 1;
+        c->g_sign = -1;
+        if (ov) {
+            if (!isnan(ov->sgs_constant)) sgs_c = (R)ov->sgs_constant;
+            if (!isnan(ov->sgs_nut_cap_ratio)) c->nut_cap_ratio = (R)ov->sgs_nut_cap_ratio;
+            if (!isnan(ov->boussinesq_beta)) c->beta = (R)ov->boussinesq_beta;
+            if (!isnan(ov->thermal_diffusivity)) c->alpha = (R)ov->thermal_diffusivity;
+            if (ov->gravity_axis >= 0 && ov->gravity_axis <= 2) c->g_axis = ov->gravity_axis;
+            if (ov->gravity_sign == 1 || ov->gravity_sign == -1) c->g_sign = ov->gravity_sign;
+        }
+        c->sgs_c2 = sgs_c * sgs_c;
+        c->kmax_speed = K(0.57735026919) * K(0.60);
+        return 1;
+    }
     /* jni.cpp:917-929 effective_base_nu_shear */
     const R re = SFX(finite_or_f)(g->reynolds_number, K(0.0));
     const R ref_speed = SFX(reference_speed)(g);
     const R ref_len = SFX(rmax)(K(1.0e-6), SFX(finite_or_f)(g->reference_length, K(1.0)));
-    /* runtime molecular nu0 = 1.5e-5 m^2/s * 0.05 s / 1 m^2 (jni.cpp:100-104) */
-    const R nu_rt = SFX(clamp_nu)(K(1.50e-5) * (K(1.0) / K(20.0)) / (K(1.0) * K(1.0)));
+    const R nu_rt = c->nu_rt0;
     const int heated = SFX(heated_cavity)(g);
     const R ra = SFX(rmax)(K(1.0e-6), SFX(finite_or_f)(g->rayleigh_number, K(1.0e5)));
     const R pr = SFX(rmax)(K(1.0e-6), SFX(finite_or_f)(g->prandtl_number, K(0.71)));
@@ -371,11 +410,50 @@ static R SFX(sponge_alpha)(const SFX(Ctx) * c, int x, int y, int z) {
     return SFX(clampr)(K(0.03) * eta * eta, K(0.0), K(0.95));
 }
 
-/* ---------------- collide (jni.cpp:2209-2413, benchmark mode) ---------------- */
+static void SFX(decode)(const SFX(Ctx) * c, size_t cell, int* x, int* y, int* z) {
+    const int yz = c->ny * c->nz;
+    *x = (int)(cell / (size_t)yz);
+    const int rem = (int)(cell - (size_t)*x * (size_t)yz);
+    *y = rem / c->nz;
+    *z = rem - *y * c->nz;
+}
+
+/* hash_signed_noise (jni.cpp:1318-1327): SplitMix64 finalizer of cell and tick,
+ * 24 low bits mapped to [-1, 1] */
+static R SFX(hash_noise)(uint64_t cell, uint64_t tick) {
+    uint64_t x = (cell * 0x9E3779B97F4A7C15ull) ^ (tick * 0xD1B54A32D192ED03ull);
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    const R u = (R)(float)(x & 0x00FFFFFFull) * (K(1.0) / K(16777215.0));
+    return K(2.0) * u - K(1.0);
+}
+
+/* effective_runtime_state_nudge (jni.cpp:683-693): 0.08 (1 - d/8)^2 within 8 cells
+ * of a face in runtime mode, 0 in benchmark mode */
+static R SFX(state_nudge)(const SFX(Ctx) * c, int x, int y, int z) {
+    if (!c->runtime) return K(0.0);
+    int d = x;
+    if (y < d) d = y;
+    if (z < d) d = z;
+    if (c->nx - 1 - x < d) d = c->nx - 1 - x;
+    if (c->ny - 1 - y < d) d = c->ny - 1 - y;
+    if (c->nz - 1 - z < d) d = c->nz - 1 - z;
+    if (d >= 8) return K(0.0);
+    const R eta = (R)(8 - d) / (R)8;
+    return K(0.08) * eta * eta;
+}
+
+/* ---------------- collide (jni.cpp:2209-2413) ---------------- */
 static void SFX(collide)(SFX(Ctx) * c) {
     const size_t N = c->cells;
     const int sgs_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_SGS);
     const int buoy_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_BUOYANCY);
+    /* effective_enable_fan_forcing / effective_fan_noise_amp (jni.cpp:799-809) */
+    const int fan_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING);
+    const R noise_amp = SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_NOISE) ? K(0.0) : K(0.02);
     const R cs2 = K(1.0) / K(3.0);
     /* cells are independent: threads change nothing but wall time */
 #pragma omp parallel for schedule(static)
@@ -409,7 +487,32 @@ static void SFX(collide)(SFX(Ctx) * c) {
             SFX(reseed)(c, cell);
             continue;
         }
+        const R speed_pre = RSQRT_(ux * ux + uy * uy + uz * uz);
         R F[3] = {K(0.0), K(0.0), K(0.0)};
+        /* fan forcing (jni.cpp:2258-2286): push toward the fan speed along its axis,
+         * damp the perpendicular velocity, soften above a speed cap */
+        if (fan_on && c->fan_mask[cell] > K(0.0)) {
+            const R fu = c->fan_ux[cell], fv = c->fan_uy[cell], fw = c->fan_uz[cell];
+            const R fan_norm = RSQRT_(fu * fu + fv * fv + fw * fw);
+            if (fan_norm > K(1e-8)) {
+                const R inv_norm = K(1.0) / fan_norm;
+                const R noise = K(1.0) + noise_amp * SFX(hash_noise)((uint64_t)cell, c->step_counter);
+                const R target = SFX(clampr)(fan_norm * K(0.05) * SFX(rmax)(K(0.0), noise), K(0.0), K(0.28));
+                const R nx = fu * inv_norm, ny = fv * inv_norm, nz = fw * inv_norm;
+                const R u_para = ux * nx + uy * ny + uz * nz;
+                const R upx = ux - u_para * nx, upy = uy - u_para * ny, upz = uz - u_para * nz;
+                const R push = SFX(rmax)(K(0.0), target - u_para);
+                R damp = K(1.0);
+                if (speed_pre > K(0.24)) {
+                    const R r = (speed_pre - K(0.24)) / SFX(rmax)(K(1e-4), K(0.04));
+                    damp = K(1.0) / (K(1.0) + r * r);
+                }
+                const R fb = c->fan_mask[cell] * K(0.07) * damp;
+                F[0] = fb * rho_safe * (push * nx - K(1.0) * upx);
+                F[1] = fb * rho_safe * (push * ny - K(1.0) * upy);
+                F[2] = fb * rho_safe * (push * nz - K(1.0) * upz);
+            }
+        }
         if (buoy_on) {
             const R b = SFX(clampr)(c->beta * c->T[cell], K(-0.02), K(0.02));
             if (c->g_sign < 0) F[c->g_axis] += rho_safe * b;
@@ -419,10 +522,15 @@ static void SFX(collide)(SFX(Ctx) * c) {
         ux += K(0.5) * dux;
         uy += K(0.5) * duy;
         uz += K(0.5) * duz;
-        /* benchmark mode: state nudge 0 (jni.cpp:679-681) */
-        ux = (K(1.0) - K(0.0)) * ux + K(0.0) * c->ref_ux[cell];
-        uy = (K(1.0) - K(0.0)) * uy + K(0.0) * c->ref_uy[cell];
-        uz = (K(1.0) - K(0.0)) * uz + K(0.0) * c->ref_uz[cell];
+        {
+            /* state nudge toward the reference velocity (jni.cpp:2306-2314); 0 in benchmark mode */
+            int x, y, z;
+            SFX(decode)(c, cell, &x, &y, &z);
+            const R sn = SFX(state_nudge)(c, x, y, z);
+            ux = (K(1.0) - sn) * ux + sn * c->ref_ux[cell];
+            uy = (K(1.0) - sn) * uy + sn * c->ref_uy[cell];
+            uz = (K(1.0) - sn) * uz + sn * c->ref_uz[cell];
+        }
         if (!RISFINITE_(ux) || !RISFINITE_(uy) || !RISFINITE_(uz)) { SFX(reseed)(c, cell); continue; }
         const R s2 = ux * ux + uy * uy + uz * uz;
         if (!RISFINITE_(s2)) { SFX(reseed)(c, cell); continue; }
@@ -663,13 +771,6 @@ static R SFX(trilinear)(const SFX(Ctx) * c, const R* field, R x, R y, R z, R fb)
     return c0 + (c1 - c0) * fz;
 }
 
-static void SFX(decode)(const SFX(Ctx) * c, size_t cell, int* x, int* y, int* z) {
-    const int yz = c->ny * c->nz;
-    *x = (int)(cell / (size_t)yz);
-    const int rem = (int)(cell - (size_t)*x * (size_t)yz);
-    *y = rem / c->nz;
-    *z = rem - *y * c->nz;
-}
 
 /* local_temperature_bounds (jni.cpp:1883-1907): min / max of the cell and its six
  * face neighbours (solid or outside points read as the cell itself, or the face value) */
@@ -687,11 +788,60 @@ static void SFX(local_bounds)(const SFX(Ctx) * c, const R* field, int x, int y, 
     *hi = mx;
 }
 
+/* velocity_component_or_self (jni.cpp:1952-1969): a neighbour's collision velocity,
+ * or the cell's own outside the domain (no periodic wrap) or in a solid */
+static R SFX(vel_or_self)(const SFX(Ctx) * c, const R* u, int x, int y, int z, R self) {
+    if (x < 0 || y < 0 || z < 0 || x >= c->nx || y >= c->ny || z >= c->nz) return self;
+    const size_t n = SFX(cell_index)(c, x, y, z);
+    if (c->obstacle[n]) return self;
+    return u[n];
+}
+
+/* obstacle_adjacent (jni.cpp:1297-1316): a face neighbour inside the domain is solid */
+static int SFX(obstacle_adjacent)(const SFX(Ctx) * c, int x, int y, int z) {
+    static const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int k = 0; k < 6; ++k) {
+        const int a = x + off[k][0], b = y + off[k][1], d = z + off[k][2];
+        if (a < 0 || b < 0 || d < 0 || a >= c->nx || b >= c->ny || d >= c->nz) continue;
+        if (c->obstacle[SFX(cell_index)(c, a, b, d)]) return 1;
+    }
+    return 0;
+}
+
+/* thermal diffusivity of a cell (jni.cpp:1971-2048): benchmark mode the resolved
+ * constant; runtime mode alpha0 + nu_t / Pr_t with the Smagorinsky viscosity of the
+ * central-difference strain of the collision velocities (0 next to a solid) */
+static R SFX(local_alpha)(const SFX(Ctx) * c, int x, int y, int z, size_t cell) {
+    if (!c->runtime) return c->alpha;
+    R nu_t = K(0.0);
+    if (!SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_SGS) && !SFX(obstacle_adjacent)(c, x, y, z)) {
+        const R* U[3] = {c->ux, c->uy, c->uz};
+        R g[3][3]; /* g[i][j] = d u_i / d x_j */
+        for (int i = 0; i < 3; ++i) {
+            const R uc = U[i][cell];
+            g[i][0] = K(0.5) * (SFX(vel_or_self)(c, U[i], x + 1, y, z, uc) - SFX(vel_or_self)(c, U[i], x - 1, y, z, uc));
+            g[i][1] = K(0.5) * (SFX(vel_or_self)(c, U[i], x, y + 1, z, uc) - SFX(vel_or_self)(c, U[i], x, y - 1, z, uc));
+            g[i][2] = K(0.5) * (SFX(vel_or_self)(c, U[i], x, y, z + 1, uc) - SFX(vel_or_self)(c, U[i], x, y, z - 1, uc));
+        }
+        const R sxx = g[0][0], syy = g[1][1], szz = g[2][2];
+        const R sxy = K(0.5) * (g[0][1] + g[1][0]);
+        const R sxz = K(0.5) * (g[0][2] + g[2][0]);
+        const R syz = K(0.5) * (g[1][2] + g[2][1]);
+        const R smag = RSQRT_(SFX(rmax)(K(0.0), K(2.0) * (sxx * sxx + syy * syy + szz * szz) +
+                                                   K(4.0) * (sxy * sxy + sxz * sxz + syz * syz)));
+        nu_t = c->sgs_c2 * smag;
+        nu_t = SFX(rmin)(nu_t, c->nut_cap_ratio * c->nu_rt0);
+        nu_t = SFX(rmax)(K(0.0), nu_t);
+    }
+    return c->alpha + nu_t / SFX(rmax)(K(1.0e-6), K(0.85));
+}
+
 /* update_temperature_field, BFECC branch (jni.cpp:2050-2139): advect forward
  * (T -> S), advect S backward (-> B), correct N = T + (T - B) / 2, advect N forward,
- * clamp to the bounds of T around the cell, then diffusion and source of T */
+ * clamp to the bounds of T around the cell, then diffusion and source of T; the
+ * step dt is the update stride (1 heated cavity, 2 runtime mode, jni.cpp:901-909) */
 static void SFX(update_temperature_bfecc)(SFX(Ctx) * c) {
-    const R dt = (R)1;
+    const R dt = (R)c->thermal_stride;
     const int src_on = !SFX(flag)(c, AERO_LBM_BENCHMARK_FLAG_DISABLE_INTERNAL_THERMAL_SOURCE);
     const R* T = c->T;
     R* S = c->T_scr;
@@ -731,7 +881,8 @@ static void SFX(update_temperature_bfecc)(SFX(Ctx) * c) {
         const R lap = tsum - K(6.0) * tc;
         R src = K(0.0);
         if (src_on && c->in_ch > 9) src = SFX(clampr)(c->src[cell], K(-0.006), K(0.006));
-        S[cell] = SFX(clampr)(adv + dt * (c->alpha * lap + src - K(0.0) * adv), K(-1.0), K(1.0));
+        const R alpha = SFX(local_alpha)(c, x, y, z, cell);
+        S[cell] = SFX(clampr)(adv + dt * (alpha * lap + src - K(0.0) * adv), K(-1.0), K(1.0));
     }
     c->T_scr = c->T;
     c->T = S;
@@ -769,12 +920,38 @@ static void SFX(update_temperature)(SFX(Ctx) * c) {
     c->T_next = tmp;
 }
 
+/* apply_runtime_temperature_reference (jni.cpp:8037-8064), run before the update on
+ * every step after the first: solid cells to 0, cells within 8 of a face nudged
+ * toward the payload temperature (runtime mode; the nudge is 0 in benchmark mode) */
+static void SFX(temperature_reference)(SFX(Ctx) * c) {
+    if (c->in_ch <= 10) return;
+    for (size_t i = 0; i < c->cells; ++i) {
+        if (c->obstacle[i]) {
+            c->T[i] = K(0.0);
+            c->T_next[i] = K(0.0);
+            continue;
+        }
+        int x, y, z;
+        SFX(decode)(c, i, &x, &y, &z);
+        const R tn = SFX(state_nudge)(c, x, y, z);
+        if (tn <= K(0.0)) continue;
+        const R rt = SFX(clampr)(c->ref_T[i], K(-1.0), K(1.0));
+        const R keep = K(1.0) - tn;
+        c->T[i] = SFX(clampr)(c->T[i] * keep + rt * tn, K(-1.0), K(1.0));
+        c->T_next[i] = c->T[i];
+    }
+}
+
 /* ---------------- ingest / init (jni.cpp:1671-1694, 1752-1781) ---------------- */
 static void SFX(ingest)(SFX(Ctx) * c, const float* payload) {
     const int ch = c->in_ch;
     for (size_t cell = 0; cell < c->cells; ++cell) {
         const float* p = payload + cell * (size_t)ch;
         c->obstacle[cell] = p[0] > 0.5f ? 1 : 0;
+        c->fan_mask[cell] = SFX(clampr)((R)p[1], K(0.0), K(1.0));
+        c->fan_ux[cell] = SFX(finite_or)((R)p[2], K(0.0));
+        c->fan_uy[cell] = SFX(finite_or)((R)p[3], K(0.0));
+        c->fan_uz[cell] = SFX(finite_or)((R)p[4], K(0.0));
         c->ref_ux[cell] = SFX(finite_or)((R)p[5], K(0.0));
         c->ref_uy[cell] = SFX(finite_or)((R)p[6], K(0.0));
         c->ref_uz[cell] = SFX(finite_or)((R)p[7], K(0.0));
@@ -802,7 +979,7 @@ static void SFX(initialize)(SFX(Ctx) * c) {
             c->f[(size_t)q * c->cells + cell] = e;
             c->f_post[(size_t)q * c->cells + cell] = e;
         }
-        if (RFABS_(c->T[cell]) < K(1e-8)) {
+        if (!c->runtime && RFABS_(c->T[cell]) < K(1e-8)) {
             c->T[cell] = SFX(clampr)((R)c->cfg.initial_temperature, K(-1.0), K(1.0));
         } else if (!c->obstacle[cell]) {
             c->T[cell] = SFX(clampr)(c->ref_T[cell], K(-1.0), K(1.0));
@@ -861,6 +1038,8 @@ ORACLE_API void* SFX(oracle_create)(int nx, int ny, int nz, int in_ch, int out_c
     c->ref_ux = (R*)calloc(N, sizeof(R)); c->ref_uy = (R*)calloc(N, sizeof(R)); c->ref_uz = (R*)calloc(N, sizeof(R));
     c->ref_p = (R*)calloc(N, sizeof(R)); c->ref_T = (R*)calloc(N, sizeof(R)); c->src = (R*)calloc(N, sizeof(R));
     c->obstacle = (unsigned char*)calloc(N, 1);
+    c->fan_mask = (R*)calloc(N, sizeof(R)); c->fan_ux = (R*)calloc(N, sizeof(R));
+    c->fan_uy = (R*)calloc(N, sizeof(R)); c->fan_uz = (R*)calloc(N, sizeof(R));
     c->T = (R*)calloc(N, sizeof(R)); c->T_next = (R*)calloc(N, sizeof(R)); c->T_scr = (R*)calloc(N, sizeof(R));
     for (size_t i = 0; i < N; ++i) c->rho[i] = K(1.0);
     return c;
@@ -871,6 +1050,7 @@ ORACLE_API void SFX(oracle_destroy)(void* h) {
     if (!c) return;
     free(c->f); free(c->f_post); free(c->rho); free(c->ux); free(c->uy); free(c->uz);
     free(c->ref_ux); free(c->ref_uy); free(c->ref_uz); free(c->ref_p); free(c->ref_T); free(c->src);
+    free(c->fan_mask); free(c->fan_ux); free(c->fan_uy); free(c->fan_uz);
     free(c->obstacle); free(c->T); free(c->T_next); free(c->T_scr); free(c->shell_layer); free(c->shell_target); free(c);
 }
 
@@ -947,7 +1127,9 @@ ORACLE_API int SFX(oracle_step)(void* h, const float* packet, float* out, float 
     if (packet) SFX(ingest)(c, packet);
     else if (!c->initialized) return 0;
     if (!c->initialized) SFX(initialize)(c);
-    SFX(update_temperature)(c);
+    else SFX(temperature_reference)(c);
+    /* should_update_temperature (jni.cpp:2191-2196) */
+    if (c->thermal_stride <= 1 || c->step_counter % (uint64_t)c->thermal_stride == 0) SFX(update_temperature)(c);
     SFX(collide)(c);
     SFX(benchmark_force)(c);
     SFX(stream)(c);

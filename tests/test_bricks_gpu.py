@@ -232,3 +232,61 @@ def test_shell_blend_vs_oracle(lib, oracle_mod):
     assert float(np.max(np.abs(o2.flow().reshape(-1, 4)[fluid, :3] - ref[fluid, :3]))) > 1e-3
     ctx.release()
     lib.aero_lbm_set_realtime_solver_mode(0)
+
+
+def test_classic_world_runtime_mode_matches_oracle_chain(oracle_mod, lib):
+    """The mod's default classic solver mode: every step rebuilds the brick's packet
+    from its dynamic state (fan disks from fan surface kinds, heat from an emitter),
+    steps the context in runtime mode with the benchmark configuration reset, reads
+    the flow and air temperature back and applies the static constraints
+    (native/src/aero_lbm_simulation_bridge.cpp:1706-1832). The same chain on the FP32
+    oracle gives bit-identical dynamic state after every step."""
+    n = 16
+    ob = brick_static(n, 9)
+    kind = np.zeros(n ** 3, np.uint8)
+    em = np.zeros(n ** 3, np.float32)
+    for x in range(4, 7):
+        for y in range(6, 9):
+            for z in range(6, 9):
+                c = B.cell(n, x, y, z)
+                if not ob[c]:
+                    kind[c] = 33   # fan blowing toward x+
+    for x in range(9, 11):
+        c = B.cell(n, x, 2, 8)
+        if not ob[c]:
+            em[c] = 4000.0        # watts
+    rng = np.random.default_rng(4)
+    flow = np.tile(np.array([0.03, 0.0, 0.01, 0.0], np.float32), n ** 3)
+    flow[0::4] += rng.uniform(-2e-3, 2e-3, n ** 3).astype(np.float32)
+    air = (0.05 * rng.uniform(-1.0, 1.0, n ** 3)).astype(np.float32)
+    zeros16 = np.zeros(n ** 3, np.uint16)
+    surf = np.zeros(n ** 3, np.float32)
+
+    def constrain(f, a):
+        f, a, s = f.copy(), a.copy(), surf.copy()
+        B.apply_constraints(ob, zeros16, kind, em, f, a, s)
+        return f, a
+
+    lib.aero_lbm_benchmark_reset_config()  # the mod's classic bricks step with no benchmark configuration
+    w = A.BrickWorld(world=13, size=n, mode=A.SOLVER_CLASSIC_D3Q27)
+    try:
+        w.hints([(0, 0, 0)], exact=True)
+        w.upload_static((0, 0, 0), ob, surface_kind=kind, emitter=em)
+        w.upload_dynamic((0, 0, 0), flow, air)
+        cfg = A.default_config()  # the classic brick steps with the configuration reset: runtime mode
+        o = oracle_mod.Oracle(n, n, n, cfg, precision=32)
+        f, a = constrain(flow, air)
+        for step in range(1, 9):
+            log(f"classic world step {step}")
+            w.step(1)
+            pk = B.build_packet(n, ob, kind, em, f, a, [True] * 6)
+            o.step(pk.reshape(-1))
+            f, a = constrain(o.flow(), o.temperature())
+            got_f, got_a, _ = w.dynamic((0, 0, 0))
+            assert np.array_equal(got_f, f), (step, float(np.max(np.abs(got_f - f))))
+            assert np.array_equal(got_a, a), step
+        st = w.status()
+        assert st[3] == 1
+        assert np.abs(f[0::4]).max() > 0.04    # the fan disk has pushed the flow (0.032 at the start)
+    finally:
+        w.release()
