@@ -1,14 +1,16 @@
 // JNI bridge of libaero_lbm.so: the native methods that the mod's Java classes
 // declare (src/main/java/com/aerodynamics4mc/runtime/NativeSimulationBridge.java:1239-1656,
 // MesoscaleNativeBridge.java:129-165), each a thin wrapper over the C ABI in
-// include/aero_lbm_capi.h. Java arrays are pinned with Get<Type>ArrayElements and
-// released with mode 0 when the call wrote them and succeeded, JNI_ABORT otherwise
-// (inputs are never written back), as the reference bridge does
-// (native/src/aero_lbm_simulation_bridge.cpp:7717-7727).
+// include/aero_lbm_capi.h. Every array argument must be present and have exactly the
+// length the call needs, checked before any array is pinned (the reference's
+// contract, native/src/aero_lbm_simulation_bridge.cpp:7026-8985); arrays are then
+// pinned with Get<Type>ArrayElements and released with mode 0 when the call wrote
+// them and succeeded, JNI_ABORT otherwise (inputs are never written back,
+// simulation_bridge.cpp:7717-7727).
 //
-// Built into the library only where a JDK provides <jni.h>; the library itself has
-// no Java dependency.
-#include <jni.h>
+// Built into the library unconditionally against jni.h beside this file (the JNI
+// specification's types and function table); the library needs no JDK to build or load.
+#include "jni.h"
 
 #include <cstddef>
 #include <cstdint>
@@ -28,7 +30,6 @@ struct Pinned {
     bool write_back = false;
     Pinned(JNIEnv* e, JArray a) : env(e), array(a) {
         if (!a) return;
-        length = e->GetArrayLength(a);
         data = get(e, a);
     }
     ~Pinned() {
@@ -53,6 +54,9 @@ const uint8_t* u8(const Bytes& b) { return reinterpret_cast<const uint8_t*>(b.da
 const uint16_t* u16(const Shorts& s) { return reinterpret_cast<const uint16_t*>(s.data); }
 
 jboolean jbool(int ok) { return ok ? JNI_TRUE : JNI_FALSE; }
+
+// length of a Java array, -1 for null (every length test below then fails)
+jsize len(JNIEnv* env, jarray a) { return a ? env->GetArrayLength(a) : -1; }
 
 jintArray coords_array(JNIEnv* env, long long service, long long world, bool active) {
     const int n = active ? aero_lbm_simulation_get_brick_world_active_brick_count(service, world)
@@ -105,11 +109,13 @@ JNIEXPORT jboolean JNICALL NSB(nativeSetFocus)(JNIEnv*, jclass, jlong service, j
 
 JNIEXPORT jboolean JNICALL NSB(nativeSubmitWorldDeltas)(JNIEnv* env, jclass, jlong service, jintArray ints,
                                                        jfloatArray floats) {
+    if (!ints || !floats) return JNI_FALSE;
+    const jsize ni = len(env, ints), nf = len(env, floats);
+    if (ni % kDeltaInts != 0 || nf % kDeltaFloats != 0 || ni / kDeltaInts != nf / kDeltaFloats) return JNI_FALSE;
+    const int count = ni / kDeltaInts;
     Ints i(env, ints);
     Floats f(env, floats);
-    if (!ints || !floats) return JNI_FALSE;
-    const int count = i.length / kDeltaInts;
-    if (i.length % kDeltaInts || f.length != count * kDeltaFloats) return JNI_FALSE;
+    if (!i || !f) return JNI_FALSE;
     std::vector<AeroLbmWorldDelta> deltas((std::size_t)count);
     for (int k = 0; k < count; ++k) {
         const jint* a = i.data + k * kDeltaInts;
@@ -130,23 +136,27 @@ JNIEXPORT jboolean JNICALL NSB(nativeSetBrickWorldSolverMode)(JNIEnv*, jclass, j
 
 JNIEXPORT jboolean JNICALL NSB(nativeSetBrickWorldActiveHints)(JNIEnv* env, jclass, jlong service, jlong world,
                                                               jint size, jintArray coords, jint count) {
+    if (!coords || count < 0 || len(env, coords) < count * 3) return JNI_FALSE;
     Ints c(env, coords);
-    if (count < 0 || (count > 0 && (!c || c.length < count * 3))) return JNI_FALSE;
+    if (!c) return JNI_FALSE;
     return jbool(aero_lbm_simulation_set_brick_world_active_hints(service, world, size, c.data, count));
 }
 
 JNIEXPORT jboolean JNICALL NSB(nativeSetBrickWorldExactActiveHints)(JNIEnv* env, jclass, jlong service, jlong world,
                                                                    jint size, jintArray coords, jint count) {
+    if (!coords || count < 0 || len(env, coords) < count * 3) return JNI_FALSE;
     Ints c(env, coords);
-    if (count < 0 || (count > 0 && (!c || c.length < count * 3))) return JNI_FALSE;
+    if (!c) return JNI_FALSE;
     return jbool(aero_lbm_simulation_set_brick_world_exact_active_hints(service, world, size, c.data, count));
 }
 
 JNIEXPORT jboolean JNICALL NSB(nativeGetBrickWorldRuntimeStatus)(JNIEnv* env, jclass, jlong service, jlong world,
                                                                 jintArray out) {
+    if (len(env, out) != AERO_LBM_SIMULATION_BRICK_RUNTIME_STATUS_FIELDS) return JNI_FALSE;
     Ints o(env, out);
     if (!o) return JNI_FALSE;
-    const int ok = aero_lbm_simulation_get_brick_world_runtime_status(service, world, o.data, o.length);
+    const int ok = aero_lbm_simulation_get_brick_world_runtime_status(service, world, o.data,
+                                                                     AERO_LBM_SIMULATION_BRICK_RUNTIME_STATUS_FIELDS);
     o.write_back = ok != 0;
     return jbool(ok);
 }
@@ -167,18 +177,20 @@ JNIEXPORT jintArray JNICALL NSB(nativeGetBrickWorldActiveBrickCoords)(JNIEnv* en
 
 static jboolean upload_static(JNIEnv* env, jlong service, jlong world, jint size, jint bx, jint by, jint bz,
                               jbyteArray obstacle, jbyteArray kind, jshortArray open, jfloatArray emitter,
-                              jbyteArray fan_src, jfloatArray emitter_src, jbyteArray sky, jbyteArray direct) {
+                              bool with_sources, jbyteArray fan_src, jfloatArray emitter_src, jbyteArray sky,
+                              jbyteArray direct) {
     const long long cells = (long long)size * size * size;
+    if (cells <= 0 || len(env, obstacle) != cells || len(env, kind) != cells || len(env, open) != cells ||
+        len(env, emitter) != cells || len(env, sky) != cells * 6 || len(env, direct) != cells * 6)
+        return JNI_FALSE;
+    if (with_sources && (len(env, fan_src) != cells || len(env, emitter_src) != cells)) return JNI_FALSE;
     Bytes ob(env, obstacle), kd(env, kind), sk(env, sky), dr(env, direct), fs(env, fan_src);
     Shorts op(env, open);
     Floats em(env, emitter), es(env, emitter_src);
-    if (!ob || !kd || !op || !em || !sk || !dr || ob.length < cells || kd.length < cells || op.length < cells ||
-        em.length < cells || sk.length < cells * 6 || dr.length < cells * 6)
-        return JNI_FALSE;
-    if ((fan_src && fs.length < cells) || (emitter_src && es.length < cells)) return JNI_FALSE;
+    if (!ob || !kd || !op || !em || !sk || !dr || (with_sources && (!fs || !es))) return JNI_FALSE;
     return jbool(aero_lbm_simulation_upload_brick_world_static_brick_with_sources(
-        service, world, size, bx, by, bz, u8(ob), u8(kd), u16(op), em.data, fan_src ? u8(fs) : nullptr,
-        emitter_src ? es.data : nullptr, u8(sk), u8(dr)));
+        service, world, size, bx, by, bz, u8(ob), u8(kd), u16(op), em.data, with_sources ? u8(fs) : nullptr,
+        with_sources ? es.data : nullptr, u8(sk), u8(dr)));
 }
 
 JNIEXPORT jboolean JNICALL NSB(nativeUploadBrickWorldStaticBrick)(JNIEnv* env, jclass, jlong service, jlong world,
@@ -186,16 +198,16 @@ JNIEXPORT jboolean JNICALL NSB(nativeUploadBrickWorldStaticBrick)(JNIEnv* env, j
                                                                  jbyteArray obstacle, jbyteArray kind,
                                                                  jshortArray open, jfloatArray emitter,
                                                                  jbyteArray sky, jbyteArray direct) {
-    return upload_static(env, service, world, size, bx, by, bz, obstacle, kind, open, emitter, nullptr, nullptr, sky,
-                         direct);
+    return upload_static(env, service, world, size, bx, by, bz, obstacle, kind, open, emitter, false, nullptr, nullptr,
+                         sky, direct);
 }
 
 JNIEXPORT jboolean JNICALL NSB(nativeUploadBrickWorldStaticBrickWithSources)(
     JNIEnv* env, jclass, jlong service, jlong world, jint size, jint bx, jint by, jint bz, jbyteArray obstacle,
     jbyteArray kind, jshortArray open, jfloatArray emitter, jbyteArray fan_src, jfloatArray emitter_src,
     jbyteArray sky, jbyteArray direct) {
-    return upload_static(env, service, world, size, bx, by, bz, obstacle, kind, open, emitter, fan_src, emitter_src,
-                         sky, direct);
+    return upload_static(env, service, world, size, bx, by, bz, obstacle, kind, open, emitter, true, fan_src,
+                         emitter_src, sky, direct);
 }
 
 JNIEXPORT jboolean JNICALL NSB(nativeQueueBrickWorldStaticBrickUpload)(JNIEnv* env, jclass, jlong service, jlong world,
@@ -204,12 +216,13 @@ JNIEXPORT jboolean JNICALL NSB(nativeQueueBrickWorldStaticBrickUpload)(JNIEnv* e
                                                                       jshortArray open, jfloatArray emitter,
                                                                       jbyteArray sky, jbyteArray direct) {
     const long long cells = (long long)size * size * size;
+    if (cells <= 0 || len(env, obstacle) != cells || len(env, kind) != cells || len(env, open) != cells ||
+        len(env, emitter) != cells || len(env, sky) != cells * 6 || len(env, direct) != cells * 6)
+        return JNI_FALSE;
     Bytes ob(env, obstacle), kd(env, kind), sk(env, sky), dr(env, direct);
     Shorts op(env, open);
     Floats em(env, emitter);
-    if (!ob || !kd || !op || !em || !sk || !dr || ob.length < cells || kd.length < cells || op.length < cells ||
-        em.length < cells || sk.length < cells * 6 || dr.length < cells * 6)
-        return JNI_FALSE;
+    if (!ob || !kd || !op || !em || !sk || !dr) return JNI_FALSE;
     return jbool(aero_lbm_simulation_queue_brick_world_static_brick_upload(service, world, size, bx, by, bz, u8(ob),
                                                                           u8(kd), u16(op), em.data, u8(sk), u8(dr)) >
                  0);
@@ -218,8 +231,11 @@ JNIEXPORT jboolean JNICALL NSB(nativeQueueBrickWorldStaticBrickUpload)(JNIEnv* e
 static jboolean dynamic_io(JNIEnv* env, int op, jlong service, jlong world, jint size, jint bx, jint by, jint bz,
                            jfloatArray flow, jfloatArray air, jfloatArray surface) {
     const long long cells = (long long)size * size * size;
+    if (cells <= 0 || len(env, flow) != cells * AERO_LBM_SIMULATION_FLOW_STATE_CHANNELS || len(env, air) != cells ||
+        len(env, surface) != cells)
+        return JNI_FALSE;
     Floats f(env, flow), a(env, air), s(env, surface);
-    if (!f || !a || !s || f.length < cells * 4 || a.length < cells || s.length < cells) return JNI_FALSE;
+    if (!f || !a || !s) return JNI_FALSE;
     int ok = 0;
     if (op == 0) ok = aero_lbm_simulation_upload_brick_world_dynamic_brick(service, world, size, bx, by, bz, f.data,
                                                                           a.data, s.data);
@@ -257,10 +273,14 @@ JNIEXPORT jboolean JNICALL NSB(nativeCopyBrickWorldDynamicBrick)(JNIEnv* env, jc
 JNIEXPORT jboolean JNICALL NSB(nativeCopyBrickWorldPackedFlowAtlas)(JNIEnv* env, jclass, jlong service, jlong world,
                                                                    jint size, jint bx, jint by, jint bz, jint stride,
                                                                    jshortArray out) {
+    if (!out || stride <= 0) return JNI_FALSE;
+    const int res = (size + stride - 1) / stride;
+    const long long values = (long long)res * res * res * AERO_LBM_SIMULATION_PACKED_ATLAS_CHANNELS;
+    if (res <= 0 || len(env, out) != values) return JNI_FALSE;
     Shorts o(env, out);
     if (!o) return JNI_FALSE;
     const int ok = aero_lbm_simulation_copy_brick_world_packed_flow_atlas(
-        service, world, size, bx, by, bz, stride, reinterpret_cast<int16_t*>(o.data), o.length);
+        service, world, size, bx, by, bz, stride, reinterpret_cast<int16_t*>(o.data), (int)values);
     o.write_back = ok != 0;
     return jbool(ok);
 }
@@ -301,8 +321,11 @@ JNIEXPORT jboolean JNICALL MNB(nativeStepContext)(JNIEnv* env, jclass, jlong key
                                                  jfloatArray forcing, jfloatArray state) {
     const AeroLbmMesoscaleConfig cfg = meso_config(nx, ny, nz, dx, dt, nu, pr, prt);
     const long long cells = (long long)nx * ny * nz;
+    if (len(env, forcing) != cells * AERO_LBM_MESOSCALE_FORCING_CHANNELS ||
+        len(env, state) != cells * AERO_LBM_MESOSCALE_STATE_CHANNELS)
+        return JNI_FALSE;
     Floats f(env, forcing), s(env, state);
-    if (!f || !s || f.length != cells * 25 || s.length != cells * 6) return JNI_FALSE;
+    if (!f || !s) return JNI_FALSE;
     const int ok = aero_lbm_mesoscale_step_context(key, &cfg, f.data, s.data);
     s.write_back = ok != 0;
     return jbool(ok);
