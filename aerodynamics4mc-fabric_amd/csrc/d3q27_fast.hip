@@ -284,9 +284,17 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     const RowNbr R = row_nbr(P, x, y);
     const int cell = R.row + z;
     const uint32_t m = mask[cell];
-    if (m & (kMaskSolid | kMaskRare)) return;
+    // Solid cells take part in the population loads and stores of the local (A-A
+    // even) and A-B steps without colliding: the even step swaps their slots q and
+    // opp(q), the A-B step writes f_q <- f_opp(q), as the reference's solid cells keep
+    // f_post = f and stream it back onto themselves. No fluid cell reads a solid
+    // cell's slots, and the wave's stores then fill whole 64-byte sectors (a sector
+    // with unwritten bytes costs the memory a read-modify-write). The A-A odd step
+    // pushes into the neighbours, so solid cells stay out of it.
+    const bool solid = (m & kMaskSolid) != 0;
+    if ((m & kMaskRare) || (Mode == 2 && solid)) return;
     const size_t N = (size_t)P.cells;
-    const uint32_t links = m & kMaskLinks;
+    const uint32_t links = solid ? kMaskLinks : (m & kMaskLinks);
     using T = typename St::T;
     int zb[3];
     z_bytes<T>(P, z, zb);
@@ -299,7 +307,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     // population loads, which keeps the two sets of values from being live together
     float temp = 0.0f;
     float ux0 = 0.0f, uy0 = 0.0f, uz0 = 0.0f;
-    if (P.thermal_on) {
+    if (P.thermal_on && !solid) {
         ux0 = Uv::ld(u_state + cell);
         uy0 = Uv::ld(u_state + N + cell);
         uz0 = Uv::ld(u_state + 2 * N + cell);
@@ -337,26 +345,28 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     // temperature): the 9 links pulled from outside the domain take the face rule,
     // with the loaded value (the cell's own opposite slot) as the bounce-back
     // candidate. x is wave-uniform, so only the waves of these planes branch here.
-    if (XF && !local_first && (xlo || xhi)) {
-        const StepParams& Q = *P.self_dev;
-        const PrevPost<St, Mode != 0> prev{fin, store_in, (size_t)Q.plane};
-        const float rp = C.ref_p[cell];
-        if (xlo) xface_links<0, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
-        if (xhi) xface_links<1, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
+    if (!solid) {
+        if (XF && !local_first && (xlo || xhi)) {
+            const StepParams& Q = *P.self_dev;
+            const PrevPost<St, Mode != 0> prev{fin, store_in, (size_t)Q.plane};
+            const float rp = C.ref_p[cell];
+            if (xlo) xface_links<0, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
+            if (xhi) xface_links<1, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
+        }
+        CollideResult r = collide<St::kShifted, GravZ>(P, g, temp);
+        if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
+        if (Shell && P.shell_on) {
+            const int layer = C.shell[cell];
+            if (layer) shell_blend<St::kShifted>(P, C.shell_state, cell, layer, g, r);
+        }
+        if (P.thermal_on) {
+            t_out[cell] = temp;
+            Uv::st(u_state + cell, r.ux);
+            Uv::st(u_state + N + cell, r.uy);
+            Uv::st(u_state + 2 * N + cell, r.uz);
+        }
+        if (Mode != 0 && P.face_store_faces) face_store(*P.self_dev, x, y, z, g, store_out);
     }
-    CollideResult r = collide<St::kShifted, GravZ>(P, g, temp);
-    if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
-    if (Shell && P.shell_on) {
-        const int layer = C.shell[cell];
-        if (layer) shell_blend<St::kShifted>(P, C.shell_state, cell, layer, g, r);
-    }
-    if (P.thermal_on) {
-        t_out[cell] = temp;
-        Uv::st(u_state + cell, r.ux);
-        Uv::st(u_state + N + cell, r.uy);
-        Uv::st(u_state + 2 * N + cell, r.uz);
-    }
-    if (Mode != 0 && P.face_store_faces) face_store(*P.self_dev, x, y, z, g, store_out);
     const RowNbr R2 = row_nbr(P, fresh_s(x), fresh_s(y));
     if (Mode != 2) {
 #pragma unroll
