@@ -538,6 +538,35 @@ void update_thermal_activity(Context& c, const StepParams& P) {
     c.thermal_on = on;
 }
 
+// The device half of a payload ingest, with the packet in c.payload_dev: cell
+// flags, reference fields and fan planes from the packet, then the cell classes
+// and the fast kernels' link masks.
+static bool ingest_resident_payload(Context& c, const StepParams& P, bool fans) {
+    hipStream_t st = rt().stream;
+    if (c.storage == Storage::F32Exact && c.initialized && c.post_collision) {
+        // the reference streams step n with step n's payload; run that stream now,
+        // before the new payload replaces flags and reference pressure
+        launch_exact_stream(P, cell_fields(c), c.f32[c.f_cur], c.f32[1 - c.f_cur], st);
+        c.f_cur ^= 1;
+        c.post_collision = false;
+    }
+    if (fans && !c.fan && !c.alloc((void**)&c.fan, 4 * c.cells * sizeof(float))) return false;
+    c.fan_active = fans;
+    launch_ingest(P, c.payload_dev, c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, fans ? c.fan : nullptr, st);
+    launch_classify(P, c.flags, st);
+    if (!storage_rebuild_masks(c, P)) return false;
+    const int w[6] = {P.hydro_wrap_lo[0], P.hydro_wrap_hi[0], P.hydro_wrap_lo[1],
+                      P.hydro_wrap_hi[1], P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]};
+    std::memcpy(c.class_wrap, w, sizeof(w));
+    c.has_payload = true;
+    return hip_ok(hipGetLastError(), "ingest");
+}
+
+static bool ensure_payload_buffer(Context& c) {
+    return c.payload_dev || c.alloc((void**)&c.payload_dev, c.cells * (size_t)rt().cfg.in_ch * sizeof(float));
+}
+
+// A host packet: skipped when it equals the last one, else uploaded and ingested.
 static bool ingest_payload(Context& c, const StepParams& P, const float* packet) {
     const size_t n = c.cells * (size_t)rt().cfg.in_ch;
     if (c.payload_host.size() == n && std::memcmp(c.payload_host.data(), packet, n * sizeof(float)) == 0) return true;
@@ -549,33 +578,27 @@ static bool ingest_payload(Context& c, const StepParams& P, const float* packet)
                   "solver mode, or set AERO_LBM_BENCHMARK_FLAG_DISABLE_FAN_FORCING");
         return false;
     }
-    if (c.storage == Storage::F32Exact && c.initialized && c.post_collision) {
-        // the reference streams step n with step n's payload; run that stream now,
-        // before the new payload replaces flags and reference pressure
-        launch_exact_stream(P, cell_fields(c), c.f32[c.f_cur], c.f32[1 - c.f_cur], rt().stream);
-        c.f_cur ^= 1;
-        c.post_collision = false;
-    }
     c.payload_host.assign(packet, packet + n);
     if (ch > 9 && host_any_nonzero(packet, c.cells, ch, 9)) c.payload_src_nonzero = true;
     if (ch > 10 && host_any_nonzero(packet, c.cells, ch, 10)) c.payload_t_nonzero = true;
-    if (!c.payload_dev) {
-        if (!c.alloc((void**)&c.payload_dev, n * sizeof(float))) return false;
-    }
-    hipStream_t st = rt().stream;
-    if (!hip_ok(hipMemcpyAsync(c.payload_dev, packet, n * sizeof(float), hipMemcpyHostToDevice, st), "upload payload"))
+    if (!ensure_payload_buffer(c)) return false;
+    if (!hip_ok(hipMemcpyAsync(c.payload_dev, packet, n * sizeof(float), hipMemcpyHostToDevice, rt().stream),
+                "upload payload"))
         return false;
-    if (fans && !c.fan && !c.alloc((void**)&c.fan, 4 * c.cells * sizeof(float))) return false;
-    c.fan_active = fans;
-    launch_ingest(P, c.payload_dev, c.flags, c.ref_p, c.ref_u, c.ref_t, c.src, fans ? c.fan : nullptr, st);
-    launch_classify(P, c.flags, st);
-    if (!storage_rebuild_masks(c, P)) return false;
-    const int w[6] = {P.hydro_wrap_lo[0], P.hydro_wrap_hi[0], P.hydro_wrap_lo[1],
-                      P.hydro_wrap_hi[1], P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]};
-    std::memcpy(c.class_wrap, w, sizeof(w));
-    c.has_payload = true;
     c.timing_upload_bytes += n * sizeof(float);
-    return hip_ok(hipGetLastError(), "ingest");
+    return ingest_resident_payload(c, P, fans);
+}
+
+// A packet already in device memory whose fan, source and temperature channels are
+// zero (the wind-tunnel layer's packets): copied into the context and ingested.
+static bool ingest_device_packet(Context& c, const StepParams& P, const float* packet_dev) {
+    c.payload_host.clear();  // the host copy no longer describes the resident payload
+    if (!ensure_payload_buffer(c)) return false;
+    const size_t bytes = c.cells * (size_t)rt().cfg.in_ch * sizeof(float);
+    if (!hip_ok(hipMemcpyAsync(c.payload_dev, packet_dev, bytes, hipMemcpyDeviceToDevice, rt().stream),
+                "copy payload"))
+        return false;
+    return ingest_resident_payload(c, P, false);
 }
 
 // Per-step switches that depend on the context's step counter: the temperature
@@ -587,12 +610,13 @@ void step_switches(const Context& c, StepParams& P) {
     P.tick = P.fan_on ? (unsigned long long)c.steps : 0ull;
 }
 
-// One solver step of a context. packet == nullptr means "resident payload".
-bool step_context(Context& c, const float* packet, float* out, int out_ch, float scale) {
+// One solver step of a context. packet == nullptr means "resident payload";
+// packet_on_device says the packet is in device memory.
+bool step_context(Context& c, const float* packet, float* out, int out_ch, float scale, bool packet_on_device) {
     StepParams P;
     if (!prepare_params(c, P)) return false;
     if (packet) {
-        if (!ingest_payload(c, P, packet)) return false;
+        if (!(packet_on_device ? ingest_device_packet(c, P, packet) : ingest_payload(c, P, packet))) return false;
     } else if (!c.has_payload) {
         set_error("cached step requested before any payload was uploaded");
         return false;
@@ -678,7 +702,8 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_init(int grid_size, int input_channels, int ou
     return aero_lbm_init_rect(grid_size, grid_size, grid_size, input_channels, output_channels);
 }
 
-static int step_impl(const float* packet, int nx, int ny, int nz, long long key, float scale, float* out) {
+static int step_impl(const float* packet, int nx, int ny, int nz, long long key, float scale, float* out,
+                     bool packet_on_device = false) {
     auto t0 = Clock::now();
     std::lock_guard<std::mutex> lk(rt().mu);
     clear_error();
@@ -696,11 +721,27 @@ static int step_impl(const float* packet, int nx, int ny, int nz, long long key,
         if (rt().last_error.empty()) set_error("cached solver step requested before context initialization");
         return 0;
     }
-    const bool ok = step_context(*c, packet, out, rt().cfg.out_ch, scale);
+    const bool ok = step_context(*c, packet, out, rt().cfg.out_ch, scale, packet_on_device);
     const double total = ms_since(t0);
     record_timing(total, total, 0.0, 0.0);
     return ok ? 1 : 0;
 }
+
+}  // extern "C"
+
+namespace aero {
+bool context_resident(long long key, int nx, int ny, int nz) {
+    std::lock_guard<std::mutex> lk(rt().mu);
+    const Context* c = find_context(key);
+    return c && c->has_payload && c->nx == nx && c->ny == ny && c->nz == nz;
+}
+
+int step_rect_device_packet(const float* packet_dev, int nx, int ny, int nz, long long key, float scale, float* out) {
+    return step_impl(packet_dev, nx, ny, nz, key, scale, out, packet_dev != nullptr);
+}
+}  // namespace aero
+
+extern "C" {
 
 AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect(const float* packet, int nx, int ny, int nz, long long key, float* out) {
     if (!packet) {

@@ -134,7 +134,12 @@ void sanitize_benchmark_config(AeroLbmBenchmarkConfig& c);
 Context* find_context(long long key);
 Context* get_context(long long key, int nx, int ny, int nz, bool create);
 CellFields cell_fields(const Context& c);
-bool step_context(Context& c, const float* packet, float* out, int out_ch, float scale);
+bool step_context(Context& c, const float* packet, float* out, int out_ch, float scale, bool packet_on_device = false);
+// aero_lbm_step_rect_scaled for a packet in device memory (fan, source and
+// temperature channels zero); packet_dev == nullptr steps the resident payload
+int step_rect_device_packet(const float* packet_dev, int nx, int ny, int nz, long long key, float scale, float* out);
+// context `key` exists with these dimensions and holds an ingested payload
+bool context_resident(long long key, int nx, int ny, int nz);
 void step_switches(const Context& c, StepParams& P);
 
 // storage-specific operations (storage.hip)

@@ -70,7 +70,7 @@ size_t context_device_bytes(const Context& c) {
         case Storage::F32AA: b += 27 * Np * 4 + N * 4; break;
     }
     b += 2 * c.face_store_floats * 4;
-    if (c.payload_dev) b += c.payload_host.size() * 4;
+    if (c.payload_dev) b += c.cells * (size_t)rt().cfg.in_ch * 4;
     return b;
 }
 
