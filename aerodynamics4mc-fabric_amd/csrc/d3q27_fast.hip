@@ -327,18 +327,33 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
         // the cell's own slot of the opposite direction (halfway bounce-back, when
         // the mask flags the link). Both lie in one buffer; the scalar offset is the
         // lower of the two and each lane picks its vector offset, so a bounced link
-        // costs no second load.
+        // costs no second load. Which one is lower is known here: the two lie in
+        // different planes (the rest link aside), and a row offset is smaller than
+        // the plane stride, so the plane index decides. The pick is a bitfield
+        // select on the link's bit spread over the word (0 or all ones).
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const int dx = -lat_cx(q), dy = -lat_cy(q), dz = -lat_cz(q);
             const int qa = Mode == 0 ? q : lat_opp(q);  // streamed: plane at the neighbour
             const int qb = Mode == 0 ? lat_opp(q) : q;  // bounced: plane at the cell itself
-            const int sa = plane_row<T>(Ni, qa, R.ok(dx, dy) ? R.off(dx, dy) : R.row);
             const int sb = plane_row<T>(Ni, qb, R.row);
-            const int so = min(sa, sb);
-            const bool sp = (links >> q) & 1u;
-            const int voff = sp ? zb[1] + (sb - so) : zb[dz + 1] + (sa - so);
-            g[q] = St::template bld<kPolDefault>(rin, voff, so, q);
+            if (qa == qb) {  // rest link: the cell's own slot
+                g[q] = St::template bld<kPolDefault>(rin, zb[1], sb, q);
+                continue;
+            }
+            const int sa = plane_row<T>(Ni, qa, R.ok(dx, dy) ? R.off(dx, dy) : R.row);
+            const int bounce = (int)(links << (31 - q)) >> 31;
+            int so, va, vb;  // scalar offset; vector offsets of the streamed and the bounced value
+            if (qa < qb) {
+                so = sa;
+                va = zb[dz + 1];
+                vb = zb[1] + (sb - sa);
+            } else {
+                so = sb;
+                va = zb[dz + 1] + (sa - sb);
+                vb = zb[1];
+            }
+            g[q] = St::template bld<kPolDefault>(rin, (bounce & vb) | (~bounce & va), so, q);
         }
     }
     // x face planes flagged xface_bulk (inlet, outflow, symmetry, Dirichlet / Neumann
