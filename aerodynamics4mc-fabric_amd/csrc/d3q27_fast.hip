@@ -178,7 +178,10 @@ __device__ __forceinline__ float temperature_fast(const StepParams& P, const flo
                                                   const float* __restrict__ src, int cell, int x, int y, int z,
                                                   const Nbr& nb, uint32_t links, float ux, float uy, float uz,
                                                   bool xlo = false, bool xhi = false) {
-    const float tc = T[cell];
+    // T, the temperature field, is read as a buffer with 32-bit byte offsets
+    const Rsrc rT = make_rsrc(T, P.cells * 4);
+    const int cb = cell * 4;
+    const float tc = bld_f32(rT, cb);
     float tlo = tc, thi = tc;
     if (XFace) {
         const FaceParams* fp = P.self_dev->faces_dev;
@@ -190,20 +193,19 @@ __device__ __forceinline__ float temperature_fast(const StepParams& P, const flo
     const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
     const bool bx = fx0 < (float)x, by = fy0 < (float)y, bz = fz0 < (float)z;  // box starts at -1 on that axis
     // The 2x2x2 box of the departure point starts at (ax, ay, az) in {-1, 0}^3 relative
-    // to the cell. Offsets of its two layers along each axis (periodic wrap from nb;
-    // a neighbour outside the domain maps to the cell, so every index stays in bounds).
-    const int X0 = bx ? nb.xm : 0, X1 = bx ? 0 : nb.xp;
-    const int Y0 = by ? nb.ym : 0, Y1 = by ? 0 : nb.yp;
-    const int Z0 = bz ? nb.zm : 0, Z1 = bz ? 0 : nb.zp;
+    // to the cell. Byte offsets of its two layers along each axis (periodic wrap from
+    // nb; a neighbour outside the domain maps to the cell, so every index stays in bounds).
+    const int X0 = (bx ? nb.xm : 0) * 4, X1 = (bx ? 0 : nb.xp) * 4;
+    const int Y0 = (by ? nb.ym : 0) * 4, Y1 = (by ? 0 : nb.yp) * 4;
+    const int Z0 = (bz ? nb.zm : 0) * 4, Z1 = (bz ? 0 : nb.zp) * 4;
     // corner (i, j, k) is link link_from(ax + i, ay + j, az + k) = base - (9 i + 3 j + k),
     // base in [13, 26]: after the shift its bit sits at 13 - (9 i + 3 j + k)
     const int base = 26 - (bx ? 0 : 9) - (by ? 0 : 3) - (bz ? 0 : 1);
     const uint32_t sh = links >> (base - 13);
-    // a solid or outside point reads the cell itself (the index is selected, not the
-    // loaded value, so every load is unconditional)
+    // a solid or outside point reads the cell itself (the offset is masked to zero,
+    // not the loaded value selected, so every load is unconditional)
     auto corner = [&](int i, int j, int k, int off) -> float {
-        const bool sp = (sh >> (13 - 9 * i - 3 * j - k)) & 1u;
-        const float v = T[sp ? cell : cell + off];
+        const float v = bld_f32(rT, cb + (off & ~bit_mask(sh, 13 - 9 * i - 3 * j - k)));
         if (XFace) {
             const bool below = bx && i == 0, above = !bx && i == 1;  // x - 1 / x + 1 layer
             return (xlo && below) ? tlo : ((xhi && above) ? thi : v);
@@ -221,15 +223,15 @@ __device__ __forceinline__ float temperature_fast(const StepParams& P, const flo
     const float adv = fmaf(c1 - c0, fz, c0);
     // the six face neighbours (links 4, 22, 10, 16, 12, 14 pull from +x, -x, +y, -y, +z, -z)
     auto face = [&](int bit, int off, int dx) -> float {
-        const bool sp = (links >> bit) & 1u;
-        const float v = T[sp ? cell : cell + off];
+        const float v = bld_f32(rT, cb + ((off * 4) & ~bit_mask(links, bit)));
         if (XFace) return (xlo && dx < 0) ? tlo : ((xhi && dx > 0) ? thi : v);
         return v;
     };
     const float tsum = (face(4, nb.xp, 1) + face(22, nb.xm, -1)) + (face(10, nb.yp, 0) + face(16, nb.ym, 0)) +
                        (face(12, nb.zp, 0) + face(14, nb.zm, 0));
     const float lap = fmaf(-6.0f, tc, tsum);
-    const float source = (P.source_on && P.in_ch > 9) ? clampf(src[cell], -0.006f, 0.006f) : 0.0f;
+    const float source =
+        (P.source_on && P.in_ch > 9) ? clampf(bld_f32(make_rsrc(src, P.cells * 4), cb), -0.006f, 0.006f) : 0.0f;
     return clampf(adv + fmaf(P.alpha, lap, source), -1.0f, 1.0f);
 }
 
@@ -307,10 +309,13 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     // population loads, which keeps the two sets of values from being live together
     float temp = 0.0f;
     float ux0 = 0.0f, uy0 = 0.0f, uz0 = 0.0f;
+    // the carried velocity as a buffer: 3 planes of N values
+    const int ub = cell * (int)sizeof(typename Uv::T), us = (int)N * (int)sizeof(typename Uv::T);
+    const Rsrc ru = make_rsrc(u_state, 3 * us);
     if (P.thermal_on && !solid) {
-        ux0 = Uv::ld(u_state + cell);
-        uy0 = Uv::ld(u_state + N + cell);
-        uz0 = Uv::ld(u_state + 2 * N + cell);
+        ux0 = Uv::bld(ru, ub, 0);
+        uy0 = Uv::bld(ru, ub, us);
+        uz0 = Uv::bld(ru, ub, 2 * us);
         if (XF && (xlo || xhi))
             temp = temperature_fast<XF>(P, t_in, C.src, cell, x, y, z, make_nbr(P, x, y, z), links, ux0, uy0, uz0,
                                           xlo, xhi);
@@ -375,10 +380,10 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
             if (layer) shell_blend<St::kShifted>(P, C.shell_state, cell, layer, g, r);
         }
         if (P.thermal_on) {
-            t_out[cell] = temp;
-            Uv::st(u_state + cell, r.ux);
-            Uv::st(u_state + N + cell, r.uy);
-            Uv::st(u_state + 2 * N + cell, r.uz);
+            bst_f32(make_rsrc(t_out, P.cells * 4), cell * 4, temp);
+            Uv::bst(ru, ub, 0, r.ux);
+            Uv::bst(ru, ub, us, r.uy);
+            Uv::bst(ru, ub, 2 * us, r.uz);
         }
         if (Mode != 0 && P.face_store_faces) face_store(*P.self_dev, x, y, z, g, store_out);
     }

@@ -123,17 +123,41 @@ struct StS16 {
     }
 };
 
-// velocity carried to the next step's temperature advection
+// velocity carried to the next step's temperature advection; bld / bst address
+// it as a buffer (byte offsets: per-lane off, wave-uniform soff)
 struct UF32 {
     using T = float;
     static __device__ __forceinline__ float ld(const T* p) { return *p; }
     static __device__ __forceinline__ void st(T* p, float v) { *p = v; }
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+    }
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, soff, 0);
+    }
 };
 struct UF16 {
     using T = uint16_t;
     static __device__ __forceinline__ float ld(const T* p) { return (float)__builtin_bit_cast(_Float16, *p); }
     static __device__ __forceinline__ void st(T* p, float v) { *p = __builtin_bit_cast(uint16_t, (_Float16)v); }
+    static __device__ __forceinline__ float bld(Rsrc r, int off, int soff) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
+        return (float)__builtin_bit_cast(_Float16, b);
+    }
+    static __device__ __forceinline__ void bst(Rsrc r, int off, int soff, float v) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), r, off, soff, 0);
+    }
 };
+
+// one float of a buffer at byte offset off
+__device__ __forceinline__ float bld_f32(Rsrc r, int off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void bst_f32(Rsrc r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+}
+// all ones where bit b of v is set, else zero
+__device__ __forceinline__ int bit_mask(uint32_t v, int b) { return (int)(v << (31 - b)) >> 31; }
 
 __device__ constexpr float wq(int q) {
     return (lat_cx(q) * lat_cx(q) + lat_cy(q) * lat_cy(q) + lat_cz(q) * lat_cz(q)) == 0   ? 8.0f / 27.0f
