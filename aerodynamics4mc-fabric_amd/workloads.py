@@ -371,3 +371,41 @@ def adjoint_case(n: int = 64, heat_grid: int = 16, probes: int = 512, fill: floa
         "inflow_start": start.reshape(-1).astype(np.float32),
         "heat_start": np.zeros(heat_grid ** 3, np.float32),
     }
+
+
+# the batched workload: (edge, count) of its bricks
+BATCH_BRICKS = ((32, 40), (64, 12), (128, 4))
+
+
+def batch_brick_specs(seed: int = 2024) -> list:
+    """The 56 bricks of the batched workload (40 x 32^3 + 12 x 64^3 + 4 x 128^3), drawn
+    from one seeded generator: per brick a solid-box seed, a fill in [0.05, 0.20], an
+    inflow speed in [0.02, 0.08] and a direction uniform in the x-y plane."""
+    import math
+    rng = np.random.default_rng(seed)
+    specs = []
+    for edge, count in BATCH_BRICKS:
+        for _ in range(count):
+            s = int(rng.integers(1, 1 << 30))
+            fill = float(rng.uniform(0.05, 0.20))
+            speed = float(rng.uniform(0.02, 0.08))
+            ang = float(rng.uniform(0.0, 2.0 * math.pi))
+            specs.append({"edge": edge, "seed": s, "fill": fill, "speed": speed, "angle": ang})
+    return specs
+
+
+def batch_brick_case(spec: dict):
+    """(config, packet, temperature, solid) of one batched brick: velocity inlet with
+    the brick's inflow, SGS and Boussinesq buoyancy on, seeded solid boxes and a seeded
+    hot blob (amplitude 0.02, sigma edge / 16) on the floor."""
+    import math
+    edge, seed, speed, ang = spec["edge"], spec["seed"], spec["speed"], spec["angle"]
+    v = (speed * math.cos(ang), speed * math.sin(ang), 0.0)
+    cfg = channel_config(speed, 50.0, sgs=True, buoyancy=True)
+    cfg.initial_velocity[0], cfg.initial_velocity[1] = v[0], v[1]
+    cfg.x_min.velocity[0], cfg.x_min.velocity[1] = v[0], v[1]
+    solid = solid_boxes(edge, edge, edge, spec["fill"], seed=seed, edge=(2, max(3, edge // 8)))
+    t = hot_blob(edge, edge, edge, 0.02, edge / 16.0, seed=seed + 1, axis=2)
+    t[solid.astype(bool)] = 0.0
+    pk = make_packet(solid, u0=v, perturb=1e-3, seed_u=seed + 2, temperature=t)
+    return cfg, pk, t, solid

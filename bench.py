@@ -172,48 +172,33 @@ def run_meso(steps, warmup, no_cpu):
     return out
 
 
-BATCH_BRICKS = ((32, 40), (64, 12), (128, 4))  # (edge, count)
-
-
 def run_batched(steps, warmup, per_brick_steps=40):
-    """40 x 32^3 + 12 x 64^3 + 4 x 128^3 FP16 bricks, each with seeded solid boxes
-    (5-20 % fill), its own inflow (speed 0.02-0.08, direction uniform in the x-y
-    plane), Boussinesq buoyancy and a seeded hot blob; stepped together with the
-    int16 atlas of every brick published every 4 steps (stride 4), the cycle replayed
-    from a HIP graph. The same bricks stepped one launch sequence per brick (with the
-    per-brick atlas read back and quantised on the host) give the per-brick rate."""
-    import math
+    """40 x 32^3 + 12 x 64^3 + 4 x 128^3 FP16 bricks (workloads.batch_brick_specs), each
+    with seeded solid boxes (5-20 % fill), its own inflow (speed 0.02-0.08, direction
+    uniform in the x-y plane), Boussinesq buoyancy and a seeded hot blob; stepped
+    together with the int16 atlas of every brick published every 4 steps (stride 4),
+    the cycle replayed from a HIP graph. The same bricks stepped one launch sequence
+    per brick (with the per-brick atlas read back and quantised on the host) give the
+    per-brick rate."""
     import aero_lbm as A
     import workloads as W
     lib = A.load()
     A.check(lib.aero_lbm_init_rect(32, 32, 32, 11, 4), "init_rect")
     ov = W.reference_overrides(A.KERNEL_FAST)
     A.check(lib.aero_lbm_set_solver_overrides(ctypes.byref(ov)), "set_solver_overrides")
-    rng = np.random.default_rng(2024)
     keys, cells = [], 0
     key = 20000
-    for edge, count in BATCH_BRICKS:
-        for _ in range(count):
-            seed = int(rng.integers(1, 1 << 30))
-            fill = float(rng.uniform(0.05, 0.20))
-            speed = float(rng.uniform(0.02, 0.08))
-            ang = float(rng.uniform(0.0, 2.0 * math.pi))
-            v = (speed * math.cos(ang), speed * math.sin(ang), 0.0)
-            cfg = W.channel_config(speed, 50.0, sgs=True, buoyancy=True)
-            cfg.initial_velocity[0], cfg.initial_velocity[1] = v[0], v[1]
-            cfg.x_min.velocity[0], cfg.x_min.velocity[1] = v[0], v[1]
-            solid = W.solid_boxes(edge, edge, edge, fill, seed=seed, edge=(2, max(3, edge // 8)))
-            t = W.hot_blob(edge, edge, edge, 0.02, edge / 16.0, seed=seed + 1, axis=2)
-            t[solid.astype(bool)] = 0.0
-            pk = W.make_packet(solid, u0=v, perturb=1e-3, seed_u=seed + 2, temperature=t)
-            A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "config")
-            lib.aero_lbm_set_realtime_solver_mode(A.SOLVER_D3Q27_FP16_INPLACE)
-            ctx = A.Context(edge, edge, edge, key=key)
-            ctx.set_temperature(t)
-            ctx.step(pk)  # upload, initialise, first step (untimed)
-            keys.append(key)
-            cells += edge ** 3
-            key += 1
+    for spec in W.batch_brick_specs():
+        edge = spec["edge"]
+        cfg, pk, t, _ = W.batch_brick_case(spec)
+        A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfg)), "config")
+        lib.aero_lbm_set_realtime_solver_mode(A.SOLVER_D3Q27_FP16_INPLACE)
+        ctx = A.Context(edge, edge, edge, key=key)
+        ctx.set_temperature(t)
+        ctx.step(pk)  # upload, initialise, first step (untimed)
+        keys.append(key)
+        cells += edge ** 3
+        key += 1
     lib.aero_lbm_set_realtime_solver_mode(0)
     K = 4
     batch = A.Batch(keys, steps_per_publish=K, stride=4, scales=[20.0] * len(keys), autotune=True)

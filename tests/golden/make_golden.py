@@ -162,15 +162,86 @@ def config3_horizons(horizons=(100, 200, 300), n=128):
     return out
 
 
+def config2_meso(checkpoints=(1, 100, 500), dims=(256, 64, 256)):
+    """D3Q19 mesoscale 256 x 64 (vertical) x 256 over value-noise terrain (heights up
+    to 24 cells), geostrophic wind 0.04 lattice units rotated 30 degrees from +x with a
+    log-law profile, lattice viscosity 5e-4: SHA-256 of the FP32 oracle's 6-channel
+    state after each checkpoint step, probe values, and the FP64 oracle's probes at
+    the last checkpoint for the FP32 error report."""
+    cfg = W.meso_config(*dims)
+    F = W.meso_forcing(*dims)
+    cells = int(np.prod(dims))
+    idx = probes(cells)
+    out = {"name": "config2_meso", "dims": list(dims), "checkpoints": list(checkpoints),
+           "desc": "D3Q19 mesoscale 256x64x256 (y vertical), workloads.meso_forcing defaults (seed 3), "
+                   "workloads.meso_config (nu_lattice 5e-4, dx 64 m, dt 5 s), tau_min 0.58"}
+    for prec in (32, 64):
+        t0 = time.time()
+        o = pyoracle.MesoOracle(*dims, dx=cfg.dx_m, dt=cfg.dt_s, nu=cfg.molecular_nu_m2_s, pr=cfg.prandtl_air,
+                                prt=cfg.turbulent_prandtl, precision=prec)
+        done = 0
+        for c in checkpoints:
+            while done < c:
+                state = o.step(F)
+                done += 1
+            if prec == 64 and c != checkpoints[-1]:
+                continue
+            s6 = state.reshape(-1, 6)
+            key = f"fp{prec}_step{c}"
+            out[key] = {"probe_state": s6[idx].astype(float).tolist()}
+            if prec == 32:
+                out[key]["state_sha256"] = digest(state)
+            print(f"config2 fp{prec} step {c}: {time.time() - t0:.0f} s", flush=True)
+    out["probe_cells"] = idx
+    return out
+
+
+def config4_batch(steps=500, probes_per_brick=64):
+    """FP64 oracle of each of the 56 batched bricks (workloads.batch_brick_specs) after
+    `steps` steps (about 50 min on 8 cores; per-brick results are cached, so an
+    interrupted run resumes): velocity and temperature at seeded probe cells, for the
+    error report of the batched FP16 run."""
+    import aero_lbm as A
+    ov = W.reference_overrides(A.KERNEL_FAST)
+    cache_dir = os.environ.get("AERO_GOLDEN_CACHE", "/tmp/aero_golden_cache")
+    os.makedirs(cache_dir, exist_ok=True)
+    out = {"name": "config4_batch", "steps": steps,
+           "desc": "40 x 32^3 + 12 x 64^3 + 4 x 128^3 bricks of workloads.batch_brick_specs (seed 2024), each "
+                   "stepped alone by the FP64 oracle from its packet and hot blob; probe values after the "
+                   "given steps", "bricks": []}
+    t0 = time.time()
+    for i, spec in enumerate(W.batch_brick_specs()):
+        n = spec["edge"]
+        idx = probes(n ** 3, count=probes_per_brick, seed=1000 + i)
+        cache = os.path.join(cache_dir, f"config4_b{i}_s{steps}_p{probes_per_brick}.json")
+        if os.path.exists(cache):
+            with open(cache) as fh:
+                out["bricks"].append(json.load(fh))
+            continue
+        cfg, pk, t, solid = W.batch_brick_case(spec)
+        o = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=64)
+        o.set_temperature(t)
+        o.run(pk, steps)
+        rec = {"index": i, "edge": n, "speed": spec["speed"], "probe_cells": idx,
+               "probe_solid": [int(v) for v in solid.reshape(-1)[idx]],
+               "probe_flow": [[float("%.9g" % v) for v in row] for row in o.flow().reshape(-1, 4)[idx].tolist()],
+               "probe_temperature": [float("%.9g" % v) for v in o.temperature()[idx].tolist()]}
+        with open(cache, "w") as fh:
+            json.dump(rec, fh)
+        out["bricks"].append(rec)
+        print(f"config4 brick {i} ({n}^3): {time.time() - t0:.0f} s", flush=True)
+    return out
+
+
 JOBS = {"config1": config1, "config3_small": config3_small, "config3_full": config3_full,
-        "config3_horizons": config3_horizons}
+        "config3_horizons": config3_horizons, "config2_meso": config2_meso, "config4_batch": config4_batch}
 
 if __name__ == "__main__":
     names = sys.argv[1:] or ["config1", "config3_small"]
     for name in names:
         res = JOBS[name]()
         with open(os.path.join(HERE, f"{name}.json"), "w") as fh:
-            if name in ("config3_full", "config3_horizons"):
+            if name in ("config3_full", "config3_horizons", "config4_batch", "config2_meso"):
                 json.dump(res, fh, separators=(",", ":"))
             else:
                 json.dump(res, fh, indent=0)
