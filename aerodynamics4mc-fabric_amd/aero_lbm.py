@@ -181,6 +181,13 @@ PROTOTYPES = {
     "aero_lbm_step_rect_scaled": (c_int, [_F, c_int, c_int, c_int, c_longlong, c_float, _F]),
     "aero_lbm_step_rect_cached": (c_int, [c_int, c_int, c_int, c_longlong, _F]),
     "aero_lbm_step_rect_cached_scaled": (c_int, [c_int, c_int, c_int, c_longlong, c_float, _F]),
+    "aero_lbm_patch_d3q27_f16_static_cells_rect": (c_int, [c_int, c_int, c_int, c_longlong, c_int, _I, _U8, _U8]),
+    "aero_lbm_step_rect_with_sparse_overlays": (c_int, [_F, c_int, c_int, c_int, c_longlong, _I, _F, c_int, _F]),
+    "aero_lbm_shift_context": (c_int, [c_int, c_longlong, c_int, c_int, c_int]),
+    "aero_lbm_context_compact_initialized": (c_int, [c_longlong]),
+    "aero_lbm_exchange_halo": (c_int, [c_int, c_longlong, c_longlong, c_int, c_int, c_int]),
+    "aero_lbm_exchange_halo_layers_rect": (c_int, [c_int, c_int, c_int, c_int, c_longlong, c_longlong, c_int, c_int,
+                                                   c_int]),
     "aero_lbm_has_context": (c_int, [c_longlong]),
     "aero_lbm_context_realtime_cached_initialized": (c_int, [c_longlong]),
     "aero_lbm_get_temperature_state_rect": (c_int, [c_int, c_int, c_int, c_longlong, _F]),
@@ -252,6 +259,9 @@ PROTOTYPES = {
                                                                    _F, _F, _F]),
     "aero_lbm_simulation_copy_brick_world_packed_flow_atlas": (c_int, [c_longlong, c_longlong, c_int, c_int, c_int,
                                                                        c_int, c_int, _I16, c_int]),
+    "aero_lbm_simulation_set_packed_flow_atlas": (c_int, [c_longlong, c_longlong, _I16, c_int]),
+    "aero_lbm_simulation_poll_packed_flow_atlas": (c_int, [c_longlong, c_longlong, _I16, c_int]),
+    "aero_lbm_simulation_runtime_info": (c_char_p, []),
     "aero_lbm_simulation_last_error": (c_char_p, []),
     "aero_lbm_batch_create": (c_int, [POINTER(c_longlong), c_int, c_int, c_int, _F, c_int, POINTER(c_longlong)]),
     "aero_lbm_batch_run": (c_int, [c_longlong, c_int, c_int, _F]),

@@ -100,6 +100,7 @@ struct World {
     bool exact_hints = false;
     uint64_t epoch = 0;
     uint64_t cached_steps = 0, packet_steps = 0, dynamic_upload_resets = 0;
+    uint64_t patch_attempts = 0, patch_steps = 0, patch_failures = 0, patch_cells = 0;
     BrickMap bricks;
     CoordSet closure;
     std::vector<AeroLbmWorldDelta> pending;
@@ -118,6 +119,7 @@ struct Service {
     long long next_key = -1;
     int next_upload = 1;
     std::unordered_map<int, PendingUpload> uploads;
+    std::unordered_map<long long, std::vector<int16_t>> atlases;  // callers' packed flow atlases
 };
 
 std::mutex g_mu;
@@ -500,10 +502,49 @@ bool sync_dynamic(const World& w, Brick& b) {
     return true;
 }
 
-// ref: 1678-1704 (static patches of a cached context re-enter through the packet path)
-bool step_cached(World& w, Brick& b) {
-    if (!experimental(w) || !b.compact || b.context == 0 || b.geometry_dirty || b.pending_reinit || b.forcing_dirty)
+// fan direction code (1-6 = x-, x+, y-, y+, z-, z+) of a client fan surface kind
+// (ref: simulation_bridge.cpp:539-549)
+uint8_t fan_code_of_kind(uint8_t kind) {
+    return (kind >= kKindFanXNeg && kind <= kKindFanXNeg + 5) ? (uint8_t)(kind - kKindFanXNeg + 1) : 0;
+}
+
+// the cells patched by world deltas, applied to the cached in-place context in place
+// of a packet rebuild (ref: simulation_bridge.cpp:1617-1676)
+bool apply_static_patches(World& w, Brick& b) {
+    if (b.patched_cells.empty() || b.context == 0 || !has_static(w, b)) return false;
+    w.patch_attempts++;
+    const StaticData& s = *b.stat;
+    const int n = w.size;
+    std::vector<int> cells;
+    std::vector<uint8_t> solid, fan;
+    std::unordered_set<int> seen;
+    for (int cell : b.patched_cells) {
+        if (cell < 0 || (size_t)cell >= cube(n) || !seen.insert(cell).second) continue;
+        const uint8_t so = s.obstacle[(size_t)cell] ? 1 : 0;
+        cells.push_back(cell);
+        solid.push_back(so);
+        fan.push_back(so ? 0 : fan_code_of_kind(s.surface_kind[(size_t)cell]));
+    }
+    if (!cells.empty() && !aero_lbm_patch_d3q27_f16_static_cells_rect(n, n, n, b.context, (int)cells.size(),
+                                                                      cells.data(), solid.data(), fan.data())) {
+        w.patch_failures++;
         return false;
+    }
+    w.patch_steps += cells.empty() ? 0 : 1;
+    w.patch_cells += cells.size();
+    b.patched_cells.clear();
+    b.forcing_dirty = false;
+    b.geometry_dirty = false;
+    b.pending_reinit = false;
+    b.stale = true;
+    return true;
+}
+
+// ref: 1678-1704: a cached in-place context takes pending cell patches in place and
+// steps on; anything else re-enters through the packet path
+bool step_cached(World& w, Brick& b) {
+    if (!experimental(w) || !b.compact || b.context == 0 || b.geometry_dirty || b.pending_reinit) return false;
+    if (b.forcing_dirty && !apply_static_patches(w, b)) return false;
     const int n = w.size;
     bool ok = configure_compact(b.cv, n) && aero_lbm_step_rect_cached(n, n, n, b.context, nullptr) != 0;
     aero_lbm_benchmark_reset_config();
@@ -1379,6 +1420,92 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_packed_
                 dst += kAtlasCh;
             }
     return 1;
+}
+
+// ref: simulation_bridge.cpp:6711-6762
+extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_packed_flow_atlas(long long service_key, long long atlas_key,
+                                                                             const int16_t* values, int value_count) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (value_count <= 0 || !values) {
+        set_err("simulation_set_packed_flow_atlas: invalid atlas buffer");
+        return 0;
+    }
+    Service* svc = lookup(service_key);
+    if (!svc) {
+        set_err("simulation_set_packed_flow_atlas: missing service");
+        return 0;
+    }
+    svc->atlases[atlas_key].assign(values, values + value_count);
+    return 1;
+}
+
+extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_poll_packed_flow_atlas(long long service_key,
+                                                                              long long atlas_key, int16_t* out,
+                                                                              int value_count) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (value_count <= 0 || !out) {
+        set_err("simulation_poll_packed_flow_atlas: invalid output buffer");
+        return 0;
+    }
+    Service* svc = lookup(service_key);
+    if (!svc) {
+        set_err("simulation_poll_packed_flow_atlas: missing service");
+        return 0;
+    }
+    auto it = svc->atlases.find(atlas_key);
+    if (it == svc->atlases.end()) {
+        set_err("simulation_poll_packed_flow_atlas: missing atlas");
+        return 0;
+    }
+    if ((int)it->second.size() != value_count) {
+        set_err("simulation_poll_packed_flow_atlas: size mismatch");
+        return 0;
+    }
+    std::copy(it->second.begin(), it->second.end(), out);
+    return 1;
+}
+
+// ref: simulation_bridge.cpp:6866-6990 (the text's fields are this build's own)
+extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info() {
+    static thread_local std::string text;
+    std::lock_guard<std::mutex> lk(g_mu);
+    size_t worlds = 0, bricks = 0, active = 0, contexts = 0, compact = 0, atlases = 0;
+    uint64_t cached = 0, packet = 0, resets = 0, p_try = 0, p_ok = 0, p_fail = 0, p_cells = 0, epoch = 0;
+    int mode = -1;
+    bool mixed = false;
+    if (g_service) {
+        atlases = g_service->atlases.size();
+        worlds = g_service->worlds.size();
+        for (const auto& wv : g_service->worlds) {
+            const World& w = wv.second;
+            mixed |= mode >= 0 && mode != w.mode;
+            mode = w.mode;
+            bricks += w.bricks.size();
+            for (const auto& bv : w.bricks) {
+                active += bv.second.active ? 1 : 0;
+                contexts += bv.second.context != 0 ? 1 : 0;
+                compact += bv.second.compact ? 1 : 0;
+            }
+            cached += w.cached_steps;
+            packet += w.packet_steps;
+            resets += w.dynamic_upload_resets;
+            p_try += w.patch_attempts;
+            p_ok += w.patch_steps;
+            p_fail += w.patch_failures;
+            p_cells += w.patch_cells;
+            epoch = std::max(epoch, w.epoch);
+        }
+    }
+    text = "service=" + std::to_string(g_service ? 1 : 0) + " worlds=" + std::to_string(worlds) +
+           " solver_mode=" + (mixed ? std::string("mixed") : std::to_string(mode)) +
+           " bricks=" + std::to_string(bricks) + " active=" + std::to_string(active) +
+           " contexts=" + std::to_string(contexts) + " cached_contexts=" + std::to_string(compact) +
+           " atlases=" + std::to_string(atlases) + " cached_steps=" + std::to_string(cached) +
+           " packet_steps=" + std::to_string(packet) + " dynamic_upload_resets=" + std::to_string(resets) +
+           " patch_attempts=" + std::to_string(p_try) + " patch_steps=" + std::to_string(p_ok) +
+           " patch_failures=" + std::to_string(p_fail) + " patch_cells=" + std::to_string(p_cells) +
+           " epoch=" + std::to_string(epoch) + " lbm=" + (aero_lbm_runtime_info() ? aero_lbm_runtime_info() : "");
+    return text.c_str();
 }
 
 extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_last_error() {

@@ -727,6 +727,72 @@ __global__ void __launch_bounds__(256) publish_batch(const BatchBrick* __restric
 }
 
 // ---------------------------------------------------------------- init and link masks
+// Static-cell patch of an A-A context (flip[i]: kPatch* bits of cells[i]; marks holds
+// 1 at every restarted cell). The context's state is the reference's pre-collision
+// populations f, as the next step reads them: a restarted fluid cell starts from the
+// rest equilibrium, and every other fluid cell keeps the value it holds for the link
+// toward a patched neighbour. In layout 2 or 3 (after an odd step, or initial) a
+// cell's incoming values are its own slots, so only the restarted cell is written.
+// In layout 1 (after an even step) the cell's incoming value of link q is slot
+// opp(q) of its neighbour x - c_q, or its own slot q when that link bounces (the
+// link mask, rebuilt for the new flags): a restarted cell's incoming slots are
+// written there, and for a fluid neighbour x = y + c_q, not restarted, of a cell y
+// whose state changed, the value of x's link q moves between its bounce slot (own
+// slot q) and its streamed slot (slot opp(q) of y). Each slot is the incoming slot
+// of exactly one cell, so the threads write disjoint slots.
+__device__ __forceinline__ bool patch_neighbour(const StepParams& P, int x, int y, int z, int q, int* out) {
+    const int a = wrapc(x + lat_cx(q), P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
+    const int b = wrapc(y + lat_cy(q), P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);
+    const int c = wrapc(z + lat_cz(q), P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]);
+    if (a < 0 || b < 0 || c < 0 || a >= P.nx || b >= P.ny || c >= P.nz) return false;
+    *out = (a * P.ny + b) * P.nz + c;
+    return true;
+}
+
+template <class St>
+__global__ void __launch_bounds__(256) patch_reset_kernel(StepParams P, const uint32_t* __restrict__ mask,
+                                                          const uint8_t* __restrict__ flags,
+                                                          const uint8_t* __restrict__ marks, typename St::T* f,
+                                                          int layout, const int* __restrict__ cells,
+                                                          const uint8_t* __restrict__ flip, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int cell = cells[i];
+    const int kind = flip[i];
+    if (cell < 0 || cell >= P.cells) return;
+    const bool restart = (kind & kPatchReset) && !(kind & kPatchSolid);
+    const size_t Np = (size_t)P.plane;
+    const int yz = P.ny * P.nz;
+    const int x = cell / yz, y = (cell - x * yz) / P.nz, z = cell - x * yz - y * P.nz;
+    auto rest = [](int q) { return St::kShifted ? 0.0f : wq(q); };
+    if (layout != 1) {
+        if (restart)
+            for (int q = 0; q < 27; ++q) St::st(f + q * Np + cell, rest(q), q);
+        return;
+    }
+    if (kind & kPatchLinks) {
+        for (int q = 0; q < 27; ++q) {
+            if (q == 13) continue;
+            int nb;
+            if (!patch_neighbour(P, x, y, z, q, &nb)) continue;
+            if ((flags[nb] & kCellSolid) || marks[nb]) continue;  // fluid and not restarted
+            // nb pulls its link q from this cell: streamed slot opp(q) here, bounce slot q at nb
+            typename St::T* streamed = f + lat_opp(q) * Np + cell;
+            typename St::T* bounced = f + q * Np + nb;
+            if (kind & kPatchSolid) *bounced = *streamed;
+            else *streamed = *bounced;
+        }
+    }
+    if (!restart) return;
+    const uint32_t links = mask[cell] & kMaskLinks;
+    for (int q = 0; q < 27; ++q) {
+        int src;
+        const bool own = ((links >> q) & 1u) || !patch_neighbour(P, x, y, z, lat_opp(q), &src);
+        if (own) St::st(f + q * Np + cell, rest(q), q);
+        else St::st(f + lat_opp(q) * Np + src, rest(q), q);
+    }
+}
+
 template <class St, class Uv>
 __global__ void __launch_bounds__(256) init_kernel(StepParams P, CellFields C, typename St::T* __restrict__ f,
                                                    float* __restrict__ T, float* __restrict__ T_other,
@@ -1067,6 +1133,32 @@ void launch_batch_publish(FastStorage st, const BatchBrick* table, const BatchGr
         case FastStorage::F16: hipLaunchKernelGGL(fast::publish_batch<fast::StF16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
         case FastStorage::BF16: hipLaunchKernelGGL(fast::publish_batch<fast::StBF16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
         case FastStorage::S16: hipLaunchKernelGGL(fast::publish_batch<fast::StS16>, grid, dim3(256), 0, s, table, g.publish_owner); break;
+    }
+}
+
+void launch_fast_patch_reset(const StepParams& P, FastStorage st, const uint32_t* mask, const uint8_t* flags,
+                             const uint8_t* marks, void* f, int layout, const int* cells, const uint8_t* flip,
+                             int count, hipStream_t s) {
+    using namespace fast;
+    if (count <= 0) return;
+    const dim3 g((unsigned)((count + 255) / 256));
+    switch (st) {
+        case FastStorage::F32:
+            hipLaunchKernelGGL(patch_reset_kernel<StF32>, g, dim3(256), 0, s, P, mask, flags, marks, (float*)f,
+                               layout, cells, flip, count);
+            break;
+        case FastStorage::F16:
+            hipLaunchKernelGGL(patch_reset_kernel<StF16>, g, dim3(256), 0, s, P, mask, flags, marks, (uint16_t*)f,
+                               layout, cells, flip, count);
+            break;
+        case FastStorage::BF16:
+            hipLaunchKernelGGL(patch_reset_kernel<StBF16>, g, dim3(256), 0, s, P, mask, flags, marks, (uint16_t*)f,
+                               layout, cells, flip, count);
+            break;
+        case FastStorage::S16:
+            hipLaunchKernelGGL(patch_reset_kernel<StS16>, g, dim3(256), 0, s, P, mask, flags, marks, (uint16_t*)f,
+                               layout, cells, flip, count);
+            break;
     }
 }
 

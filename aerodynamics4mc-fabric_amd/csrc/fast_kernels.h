@@ -56,6 +56,16 @@ void launch_fast_readout(const StepParams& P, const CellFields& C, FastStorage s
                          const float* store, int layout, int stride, int out_ch, float scale, float* out,
                          hipStream_t s);
 
+// Static-cell patch of an A-A context after its flags and link mask were rebuilt.
+// flip[i] bits: the populations of cells[i] restart (kPatchReset), its state differs
+// from before the patch so its neighbours' links change (kPatchLinks), it ends solid
+// (kPatchSolid); marks = 1 at every restarted cell. layout as in launch_fast_readout
+// (1, 2 or 3).
+constexpr uint8_t kPatchReset = 1, kPatchLinks = 2, kPatchSolid = 4;
+void launch_fast_patch_reset(const StepParams& P, FastStorage st, const uint32_t* mask, const uint8_t* flags,
+                             const uint8_t* marks, void* f, int layout, const int* cells, const uint8_t* flip,
+                             int count, hipStream_t s);
+
 // ---- many bricks per launch ----
 // One brick of a batched step: its step parameters and buffers for that step.
 // Blocks of a batched grid find their brick through a per-block owner table; a

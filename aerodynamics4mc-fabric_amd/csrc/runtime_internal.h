@@ -80,6 +80,13 @@ struct Context {
     StepParams params_host = {};       // last uploaded parameters (source of the async copy)
     bool params_valid = false;
     int class_wrap[6] = {-1, -1, -1, -1, -1, -1};  // periodic flags the cell classes were built with
+    // static-cell patches of A-A contexts: device copies of the last patch (cells, solid
+    // values, flip kinds) and a per-cell marker of the flipped cells
+    int* patch_cells = nullptr;
+    uint8_t* patch_solid = nullptr;
+    uint8_t* patch_flip = nullptr;
+    int patch_cap = 0;
+    uint8_t* patch_marks = nullptr;
     std::vector<float> payload_host;
     std::vector<void*> allocations;
     uint64_t timing_upload_bytes = 0;

@@ -285,8 +285,33 @@ JNIEXPORT jboolean JNICALL NSB(nativeCopyBrickWorldPackedFlowAtlas)(JNIEnv* env,
     return jbool(ok);
 }
 
+// a caller's packed flow atlas under a key of the service (ref:
+// native/src/aero_lbm_simulation_bridge.cpp:8789-8840): the input goes back with
+// JNI_ABORT, the polled output with mode 0 whatever the result, as the reference
+JNIEXPORT jboolean JNICALL NSB(nativeSetPackedFlowAtlas)(JNIEnv* env, jclass, jlong service, jlong atlas,
+                                                        jshortArray values) {
+    if (!values) return JNI_FALSE;
+    const jsize n = len(env, values);
+    Shorts v(env, values);
+    if (!v) return JNI_FALSE;
+    return jbool(aero_lbm_simulation_set_packed_flow_atlas(service, atlas, reinterpret_cast<const int16_t*>(v.data),
+                                                          (int)n));
+}
+
+JNIEXPORT jboolean JNICALL NSB(nativePollPackedFlowAtlas)(JNIEnv* env, jclass, jlong service, jlong atlas,
+                                                         jshortArray out) {
+    if (!out) return JNI_FALSE;
+    const jsize n = len(env, out);
+    Shorts o(env, out);
+    if (!o) return JNI_FALSE;
+    o.write_back = true;
+    return jbool(aero_lbm_simulation_poll_packed_flow_atlas(service, atlas, reinterpret_cast<int16_t*>(o.data),
+                                                           (int)n));
+}
+
+// ref: simulation_bridge.cpp:8976-8979
 JNIEXPORT jstring JNICALL NSB(nativeRuntimeInfo)(JNIEnv* env, jclass) {
-    const char* s = aero_lbm_runtime_info();
+    const char* s = aero_lbm_simulation_runtime_info();
     return env->NewStringUTF(s ? s : "");
 }
 

@@ -227,8 +227,37 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect(const float* packet, int nx, int ny,
 AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect_scaled(const float* packet, int nx, int ny, int nz, long long context_key, float output_velocity_scale, float* output_flow);
 AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect_cached(int nx, int ny, int nz, long long context_key, float* output_flow);
 AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect_cached_scaled(int nx, int ny, int nz, long long context_key, float output_velocity_scale, float* output_flow);
+/* ref: capi.h:173-182. Static-cell patch of an in-place (A-A) context: solid flags of
+ * the listed cells (last value of a repeated cell wins), fan directions 0-6 (no fan
+ * force on the in-place path: refused while fan forcing is on), cells that turn
+ * fluid restart from the rest equilibrium; the link masks follow the new flags. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_patch_d3q27_f16_static_cells_rect(
+    int nx, int ny, int nz, long long context_key, int patch_count,
+    const int* cell_indices, const uint8_t* solid_values, const uint8_t* fan_dir_values);
+/* ref: capi.h:195-205. One packet step whose payload takes, for each listed cell, 6
+ * overlay values: vx, vy, vz, pressure, temperature, source. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_step_rect_with_sparse_overlays(
+    const float* packet, int nx, int ny, int nz, long long context_key,
+    const int* overlay_cells, const float* overlay_values, int overlay_count, float* output_flow);
+/* ref: capi.h:206. Moves every field of a context by -d (new cell x takes old x + d);
+ * cells entering from outside take the rest equilibrium. Exact FP32 contexts. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_shift_context(int grid_size, long long context_key, int dx, int dy, int dz);
 AERO_LBM_CAPI_EXPORT int aero_lbm_has_context(long long context_key);
+/* ref: capi.h:208. The reference's compact experimental solver is not part of this
+ * library: always 0. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_context_compact_initialized(long long context_key);
 AERO_LBM_CAPI_EXPORT int aero_lbm_context_realtime_cached_initialized(long long context_key);
+/* ref: capi.h:210-228. Halo exchange between two cubic contexts whose second lies at
+ * offset (-1..1 per axis) from the first: populations (and the temperature when it
+ * was updated last step) of the boundary boxes copied both ways; halo = grid / 4 for
+ * aero_lbm_exchange_halo. Exact FP32 contexts; in-place contexts are left alone, as
+ * the reference's exchange leaves its in-place buffers. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_exchange_halo(
+    int grid_size, long long first_context_key, long long second_context_key,
+    int offset_x, int offset_y, int offset_z);
+AERO_LBM_CAPI_EXPORT int aero_lbm_exchange_halo_layers_rect(
+    int nx, int ny, int nz, int halo_layers, long long first_context_key, long long second_context_key,
+    int offset_x, int offset_y, int offset_z);
 AERO_LBM_CAPI_EXPORT int aero_lbm_get_temperature_state_rect(int nx, int ny, int nz, long long context_key, float* out_temperature);
 AERO_LBM_CAPI_EXPORT int aero_lbm_sample_temperature_point_rect(
     int nx, int ny, int nz, long long context_key,
@@ -345,6 +374,14 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_dynamic_brick(
 AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_packed_flow_atlas(
     long long service_key, long long world_key, int brick_size, int brick_x, int brick_y, int brick_z,
     int sample_stride, int16_t* out_packed_flow, int packed_value_count);
+/* ref: capi.h:671-682. A caller's packed flow atlas kept under a key of the service
+ * and read back whole (poll: value_count must equal the stored count). */
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_packed_flow_atlas(
+    long long service_key, long long atlas_key, const int16_t* atlas_values, int value_count);
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_poll_packed_flow_atlas(
+    long long service_key, long long atlas_key, int16_t* out_atlas_values, int value_count);
+/* ref: capi.h:695. Counts of the service's worlds, bricks and contexts as text. */
+AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info(void);
 AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_last_error(void);
 
 /* ---- batched stepping (extension: no counterpart in the reference, whose brick

@@ -1063,6 +1063,133 @@ ORACLE_API void SFX(oracle_set_temperature)(void* h, const float* t) {
     }
 }
 
+/* jni.cpp:6156-6415 and 4725-4778 (patch_d3q27_f16_static_cells, the patch and reset
+ * kernels): the entries set the obstacle flags in order; every cell an entry changed
+ * has its populations (the streamed f the next step collides) reset after the last
+ * entry, to zero when it ends solid and to the rest equilibrium feq(1, 0) when it
+ * ends fluid. Fan direction codes drive a force of the reference's FP16 step only. */
+ORACLE_API void SFX(oracle_patch_cells)(void* h, int count, const int* cells, const unsigned char* solid) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    const size_t N = c->cells;
+    unsigned char* reset = (unsigned char*)calloc(N, 1);
+    for (int i = 0; i < count; ++i) {
+        const int cell = cells[i];
+        if (cell < 0 || (size_t)cell >= N) continue;
+        const unsigned char s = solid[i] ? 1 : 0;
+        if (c->obstacle[cell] != s) reset[cell] = 1;
+        c->obstacle[cell] = s;
+    }
+    for (size_t cell = 0; cell < N; ++cell) {
+        if (!reset[cell]) continue;
+        for (int q = 0; q < 27; ++q)
+            c->f[(size_t)q * N + cell] = c->obstacle[cell] ? K(0.0) : SFX(feq)(c, q, K(1.0), K(0.0), K(0.0), K(0.0));
+    }
+    free(reset);
+}
+
+/* jni.cpp:8067-8088 shift_scalar_field, 8090-8131 shift_population_field: new(x) =
+ * old(x + d), fill outside */
+static void SFX(shift_plane)(R* p, int nx, int ny, int nz, int dx, int dy, int dz, R fill) {
+    const size_t N = (size_t)nx * ny * nz;
+    R* tmp = (R*)malloc(N * sizeof(R));
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+                const int ox = x + dx, oy = y + dy, oz = z + dz;
+                const int in = ox >= 0 && oy >= 0 && oz >= 0 && ox < nx && oy < ny && oz < nz;
+                tmp[((size_t)x * ny + y) * nz + z] = in ? p[((size_t)ox * ny + oy) * nz + oz] : fill;
+            }
+    memcpy(p, tmp, N * sizeof(R));
+    free(tmp);
+}
+
+/* jni.cpp:8432-8476 shift_context_cpu_state */
+ORACLE_API int SFX(oracle_shift)(void* h, int dx, int dy, int dz) {
+    SFX(Ctx)* c = (SFX(Ctx)*)h;
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const size_t N = c->cells;
+    if (dx == 0 && dy == 0 && dz == 0) return 1;
+    if (abs(dx) >= nx || abs(dy) >= ny || abs(dz) >= nz) return 0;
+    for (int q = 0; q < 27; ++q) {
+        const R eq = SFX(feq)(c, q, K(1.0), K(0.0), K(0.0), K(0.0));
+        SFX(shift_plane)(c->f + (size_t)q * N, nx, ny, nz, dx, dy, dz, eq);
+        SFX(shift_plane)(c->f_post + (size_t)q * N, nx, ny, nz, dx, dy, dz, eq);
+    }
+    SFX(shift_plane)(c->rho, nx, ny, nz, dx, dy, dz, K(1.0));
+    R* zero_fields[] = {c->ux, c->uy, c->uz, c->ref_ux, c->ref_uy, c->ref_uz, c->ref_p, c->ref_T,
+                        c->fan_mask, c->fan_ux, c->fan_uy, c->fan_uz, c->src, c->T, c->T_next, c->T_scr};
+    for (size_t k = 0; k < sizeof(zero_fields) / sizeof(zero_fields[0]); ++k)
+        SFX(shift_plane)(zero_fields[k], nx, ny, nz, dx, dy, dz, K(0.0));
+    unsigned char* ob = (unsigned char*)malloc(N);
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y)
+            for (int z = 0; z < nz; ++z) {
+                const int ox = x + dx, oy = y + dy, oz = z + dz;
+                const int in = ox >= 0 && oy >= 0 && oz >= 0 && ox < nx && oy < ny && oz < nz;
+                ob[((size_t)x * ny + y) * nz + z] = in ? c->obstacle[((size_t)ox * ny + oy) * nz + oz] : 0;
+            }
+    memcpy(c->obstacle, ob, N);
+    free(ob);
+    c->force[0] = c->force[1] = c->force[2] = 0.0f;
+    return 1;
+}
+
+/* jni.cpp:2198-2203 temperature_was_updated_last_step */
+static int SFX(t_updated_last)(const SFX(Ctx) * c) {
+    const uint64_t stride = (uint64_t)(c->thermal_stride > 1 ? c->thermal_stride : 1);
+    if (stride <= 1 || c->step_counter == 0) return 1;
+    return ((c->step_counter - 1) % stride) == 0;
+}
+
+/* one axis of the halo boxes, jni.cpp:8201-8263 halo_exchange_slab_bounds */
+static int SFX(halo_axis)(int off, int n, int halo, int core, int* a_src, int* b_dst, int* b_src, int* a_dst,
+                          int* size) {
+    if (off == -1) { *a_src = halo; *b_dst = n - halo; *b_src = core; *a_dst = 0; *size = halo; return 1; }
+    if (off == 0) { *a_src = halo; *b_dst = halo; *b_src = halo; *a_dst = halo; *size = core; return 1; }
+    if (off == 1) { *a_src = core; *b_dst = 0; *b_src = halo; *a_dst = n - halo; *size = halo; return 1; }
+    return 0;
+}
+
+static void SFX(copy_box)(const R* src, R* dst, int n, int planes, size_t stride, const int s0[3], const int t0[3],
+                          const int size[3]) {
+    for (int p = 0; p < planes; ++p)
+        for (int i = 0; i < size[0]; ++i)
+            for (int j = 0; j < size[1]; ++j)
+                for (int k = 0; k < size[2]; ++k) {
+                    const size_t from = ((size_t)(s0[0] + i) * n + (s0[1] + j)) * n + (s0[2] + k);
+                    const size_t to = ((size_t)(t0[0] + i) * n + (t0[1] + j)) * n + (t0[2] + k);
+                    dst[(size_t)p * stride + to] = src[(size_t)p * stride + from];
+                }
+}
+
+/* jni.cpp:8266-8426 exchange_context_halo_cpu_layers: populations f of the boundary
+ * boxes both ways, the temperature too when either context updated it last step
+ * (then the next-temperature field takes the temperature) */
+ORACLE_API int SFX(oracle_exchange_halo)(void* ha, void* hb, int ox, int oy, int oz, int halo_layers) {
+    SFX(Ctx)* a = (SFX(Ctx)*)ha;
+    SFX(Ctx)* b = (SFX(Ctx)*)hb;
+    const int n = a->nx;
+    if (a->nx != b->nx || a->ny != b->ny || a->nz != b->nz || a->nx != a->ny || a->ny != a->nz) return 0;
+    int halo = halo_layers < n / 2 - 1 ? halo_layers : n / 2 - 1;
+    if (halo < 1) halo = 1;
+    const int core = n - 2 * halo;
+    if (core <= 0) return 0;
+    int as[3], bd[3], bs[3], ad[3], size[3];
+    const int off[3] = {ox, oy, oz};
+    for (int k = 0; k < 3; ++k)
+        if (!SFX(halo_axis)(off[k], n, halo, core, &as[k], &bd[k], &bs[k], &ad[k], &size[k])) return 0;
+    const size_t N = a->cells;
+    SFX(copy_box)(a->f, b->f, n, 27, N, as, bd, size);
+    SFX(copy_box)(b->f, a->f, n, 27, N, bs, ad, size);
+    if (SFX(t_updated_last)(a) || SFX(t_updated_last)(b)) {
+        SFX(copy_box)(a->T, b->T, n, 1, N, as, bd, size);
+        SFX(copy_box)(b->T, a->T, n, 1, N, bs, ad, size);
+        memcpy(a->T_next, a->T, N * sizeof(R));
+        memcpy(b->T_next, b->T, N * sizeof(R));
+    }
+    return 1;
+}
+
 ORACLE_API void SFX(oracle_get_temperature)(void* h, float* t) {
     SFX(Ctx)* c = (SFX(Ctx)*)h;
     for (size_t i = 0; i < c->cells; ++i) t[i] = (float)c->T[i];

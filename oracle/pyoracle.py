@@ -55,6 +55,11 @@ def load():
         getattr(lib, p + "oracle_feq").restype = real
         getattr(lib, p + "oracle_weights").argtypes = [R]
         getattr(lib, p + "oracle_set_shell").argtypes = [c_void_p, c_void_p, POINTER(c_float), c_int, c_float]
+        getattr(lib, p + "oracle_patch_cells").argtypes = [c_void_p, c_int, POINTER(c_int), POINTER(ctypes.c_uint8)]
+        getattr(lib, p + "oracle_shift").argtypes = [c_void_p, c_int, c_int, c_int]
+        getattr(lib, p + "oracle_shift").restype = c_int
+        getattr(lib, p + "oracle_exchange_halo").argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int]
+        getattr(lib, p + "oracle_exchange_halo").restype = c_int
     lib.oracle_sanitize_config.argtypes = [c_void_p]
     lib.oracle_set_threads.argtypes = [c_int]
     lib.oracle_set_threads(min(16, os.cpu_count() or 1))
@@ -103,6 +108,24 @@ class Oracle:
         out = np.zeros(self.cells, dtype=np.float32)
         self._f("oracle_get_temperature")(self.h, _fp(out))
         return out
+
+    def shift(self, dx, dy, dz):
+        """The reference's context shift (every field: new(x) = old(x + d))."""
+        if not self._f("oracle_shift")(self.h, dx, dy, dz):
+            raise ValueError("shift exceeds the grid")
+
+    def exchange_halo(self, other, offset, halo_layers):
+        """The reference's halo exchange with `other` lying at `offset` (-1..1 per axis)."""
+        if not self._f("oracle_exchange_halo")(self.h, other.h, *offset, halo_layers):
+            raise ValueError("invalid halo exchange")
+
+    def patch_cells(self, cells, solid):
+        """The reference's static-cell patch on the oracle state (obstacle flags set in
+        order; every changed cell restarts from zero if solid, the rest equilibrium if fluid)."""
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        solid = np.ascontiguousarray(solid, dtype=np.uint8)
+        self._f("oracle_patch_cells")(self.h, len(cells), cells.ctypes.data_as(POINTER(c_int)),
+                                      solid.ctypes.data_as(POINTER(ctypes.c_uint8)))
 
     def temperature_r(self):
         out = np.zeros(self.cells, dtype=self.real)

@@ -128,6 +128,8 @@ SIGNATURES = {
     NSB + "nativeStepBrickWorldRuntime": (c_uint8, [c_int64, c_int64, c_int32]),
     NSB + "nativeCopyBrickWorldPackedFlowAtlas": (c_uint8, [c_int64, c_int64] + [c_int32] * 5 + [J]),
     NSB + "nativeGetBrickWorldRuntimeStatus": (c_uint8, [c_int64, c_int64, J]),
+    NSB + "nativeSetPackedFlowAtlas": (c_uint8, [c_int64, c_int64, J]),
+    NSB + "nativePollPackedFlowAtlas": (c_uint8, [c_int64, c_int64, J]),
     NSB + "nativeRuntimeInfo": (J, []),
     NSB + "nativeLastError": (J, []),
     MNB + "nativeDeriveTransport": (J, [c_int32] * 3 + [ctypes.c_float] * 5),

@@ -132,6 +132,55 @@ def test_single_brick_matches_manual_driving(lib, mode):
         w.release()
 
 
+def test_cell_patch_deltas_patch_the_cached_context(lib):
+    """World deltas of type 8 (cell patches) on a cached in-place brick go through
+    aero_lbm_patch_d3q27_f16_static_cells_rect and the brick steps on cached
+    (simulation_bridge.cpp:1617-1704): the world equals a context driven by hand with
+    the same patch, bit for bit, and its runtime info counts the patch."""
+    n = 16
+    mode = A.SOLVER_D3Q27_FP16_INPLACE
+    ob = brick_static(n, 7)
+    rng = np.random.default_rng(2)
+    flow = np.tile(np.array([0.04, 0.0, 0.0, 0.0], np.float32), n ** 3)
+    flow[0::4] += rng.uniform(-1e-3, 1e-3, n ** 3).astype(np.float32)
+    air = np.zeros(n ** 3, np.float32)
+    w = A.BrickWorld(world=12, size=n, mode=mode)
+    try:
+        w.hints([(0, 0, 0)], exact=True)
+        w.upload_static((0, 0, 0), ob)
+        w.upload_dynamic((0, 0, 0), flow, air)
+        w.step(3)
+        f, a = constrained(ob, flow, air)
+        pk = B.build_packet(n, ob, np.zeros(n ** 3, np.uint8), np.zeros(n ** 3, np.float32), f, a, [True] * 6)
+        m = Manual(lib, n, 4002, mode, pk)
+        m.step(2)
+        # a fluid cell turns solid, a solid cell turns fluid
+        fluid = np.argwhere(ob.reshape(n, n, n) == 0)
+        solid = np.argwhere(ob.reshape(n, n, n) != 0)
+        to_solid = next(c for c in fluid if 4 < c[0] < n - 4)
+        to_fluid = solid[len(solid) // 2]
+        deltas = (A.WorldDelta * 2)()
+        for d, c, v in zip(deltas, (to_solid, to_fluid), (1, 0)):
+            d.type, d.x, d.y, d.z, d.data0, d.data1 = 8, int(c[0]), int(c[1]), int(c[2]), 12, v
+        A.check(lib.aero_lbm_simulation_submit_world_deltas(w.service, deltas, 2), "deltas")
+        log("world step after two cell patches")
+        w.step(3)
+        cells = np.array([(c[0] * n + c[1]) * n + c[2] for c in (to_solid, to_fluid)], np.int32)
+        A.check(lib.aero_lbm_patch_d3q27_f16_static_cells_rect(n, n, n, m.ctx.key, 2, cells.ctypes.data_as(A._I),
+                                                               np.array([1, 0], np.uint8).ctypes.data_as(A._U8),
+                                                               np.zeros(2, np.uint8).ctypes.data_as(A._U8)), "patch")
+        m.step(3)
+        got, air_got, _ = w.dynamic((0, 0, 0))
+        assert np.array_equal(got, m.ctx.flow())
+        assert np.array_equal(air_got, m.ctx.temperature())
+        assert np.all(got.reshape(-1, 4)[cells[0]] == 0.0)
+        info = lib.aero_lbm_simulation_runtime_info().decode()
+        assert "patch_steps=1" in info and "patch_cells=2" in info and "patch_failures=0" in info, info
+        m.ctx.release()
+    finally:
+        w.release()
+
+
 def test_boundary_reference_shell_and_seeding(lib):
     """Brick A carries flow and a boundary reference; brick B (x+ neighbour) has no
     state and is seeded from A's mean with A's adjacent layer as its x- ghost."""

@@ -127,7 +127,41 @@ def test_world_uploads_copies_status_atlas_and_release_modes(jni):
     assert N("nativeSubmitWorldDeltas", svc, ragged, floats) == 0
 
     info = N("nativeRuntimeInfo")
-    assert isinstance(jvm.value(info), str)
+    text = jvm.value(info)
+    assert text.startswith("service=1 worlds=1") and "bricks=1" in text, text
+    N("nativeReleaseService", svc)
+
+
+def test_packed_flow_atlas_store(jni):
+    """nativeSetPackedFlowAtlas / nativePollPackedFlowAtlas (simulation_bridge.cpp:
+    6711-6762, 8789-8840): the atlas comes back whole under its key; the input array
+    goes back with JNI_ABORT, the polled array with mode 0 whatever the result."""
+    jvm, N = jni
+    svc = N("nativeCreateService")
+    vals = np.arange(-40, 40, dtype=np.int16) * 97
+    src = jvm.array("short", vals)
+    assert N("nativeSetPackedFlowAtlas", svc, 5, src) == 1
+    assert jvm.modes(src) == [JNI_ABORT]
+    out = jvm.array("short", np.zeros(vals.size))
+    assert N("nativePollPackedFlowAtlas", svc, 5, out) == 1
+    assert jvm.modes(out) == [0] and np.array_equal(jvm.value(out), vals)
+    short = jvm.array("short", np.full(vals.size - 1, 3))
+    assert N("nativePollPackedFlowAtlas", svc, 5, short) == 0
+    assert jvm.modes(short) == [0] and np.all(jvm.value(short) == 3)
+    assert b"size mismatch" in A.load().aero_lbm_simulation_last_error()
+    assert N("nativePollPackedFlowAtlas", svc, 6, jvm.array("short", np.zeros(vals.size))) == 0
+    assert b"missing atlas" in A.load().aero_lbm_simulation_last_error()
+    assert N("nativeSetPackedFlowAtlas", svc + 1, 5, src) == 0
+    assert b"missing service" in A.load().aero_lbm_simulation_last_error()
+    # the C ABI view of the same store
+    lib = A.load()
+    got = np.zeros(vals.size, np.int16)
+    assert lib.aero_lbm_simulation_poll_packed_flow_atlas(svc, 5, got.ctypes.data_as(A._I16), got.size) == 1
+    assert np.array_equal(got, vals)
+    assert lib.aero_lbm_simulation_set_packed_flow_atlas(svc, 5, None, 4) == 0
+    assert b"invalid atlas buffer" in lib.aero_lbm_simulation_last_error()
+    assert "atlases=1" in lib.aero_lbm_simulation_runtime_info().decode()
+    assert lib.aero_lbm_context_compact_initialized(12345) == 0
     N("nativeReleaseService", svc)
 
 
