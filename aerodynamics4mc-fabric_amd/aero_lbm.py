@@ -237,6 +237,7 @@ PROTOTYPES = {
     "aero_lbm_simulation_submit_world_deltas": (c_int, [c_longlong, POINTER(WorldDelta), c_int]),
     "aero_lbm_simulation_ensure_brick_world_runtime": (c_int, [c_longlong, c_longlong, c_int, c_float, c_float]),
     "aero_lbm_simulation_set_brick_world_solver_mode": (c_int, [c_longlong, c_longlong, c_int]),
+    "aero_lbm_simulation_set_brick_world_batching": (c_int, [c_longlong, c_longlong, c_int]),
     "aero_lbm_simulation_set_brick_world_active_hints": (c_int, [c_longlong, c_longlong, c_int, _I, c_int]),
     "aero_lbm_simulation_set_brick_world_exact_active_hints": (c_int, [c_longlong, c_longlong, c_int, _I, c_int]),
     "aero_lbm_simulation_get_brick_world_runtime_status": (c_int, [c_longlong, c_longlong, _I, c_int]),
@@ -493,6 +494,11 @@ class BrickWorld:
 
     def step(self, steps: int = 1):
         self._check(self.lib.aero_lbm_simulation_step_brick_world_runtime(self.service, self.world, steps), "step")
+
+    def batching(self, enabled: bool):
+        """Cached in-place bricks step in shared launches (the default) or one by one."""
+        self._check(self.lib.aero_lbm_simulation_set_brick_world_batching(self.service, self.world, int(enabled)),
+                    "batching")
 
     def status(self):
         out = np.zeros(8, np.int32)

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -34,6 +35,7 @@ struct Member {
     int layout, store_cur, t_cur;  // parity at creation, restored after every cycle
     size_t atlas_offset;           // int16 values before this brick in the atlas buffer
     int atlas_values;
+    uint64_t gen;                  // the context's generation the tables were built from
 };
 
 struct Batch {
@@ -83,12 +85,35 @@ bool members_alive(const Batch& b) {
             set_error("batch: a member context was released or re-created; create the batch again");
             return false;
         }
-        if (m.ctx->aa_layout != m.layout || m.ctx->store_cur != m.store_cur || m.ctx->t_cur != m.t_cur) {
-            set_error("batch: a member context was stepped outside the batch; create the batch again");
+        if (m.ctx->generation != m.gen || m.ctx->aa_layout != m.layout || m.ctx->store_cur != m.store_cur ||
+            m.ctx->t_cur != m.t_cur) {
+            set_error("batch: a member context was stepped or changed outside the batch; create the batch again");
             return false;
         }
     }
     return true;
+}
+
+// Table entry of context c for a step that reads face store sc and temperature
+// buffer tc, with the context's cached parameters. Every byte is set (padding
+// zeroed), so two entries compare equal with memcmp exactly when they describe the
+// same step.
+void fill_entry(BatchBrick& e, Context& c, int sc, int tc) {
+    std::memset(&e, 0, sizeof(e));
+    std::memcpy(&e.P, &c.params_host, sizeof(StepParams));
+    const CellFields C = cell_fields(c);
+    std::memcpy(&e.C, &C, sizeof(CellFields));
+    e.mask = c.mask;
+    e.f = aa_buf(c);
+    e.t_in = c.T[tc];
+    e.t_out = c.T[1 - tc];
+    e.u_state = c.u_state;
+    e.store_in = c.face_store[sc];
+    e.store_out = c.face_store[1 - sc];
+    e.rare_list = c.rare_list;
+    e.rare_count = c.rare_count;
+    e.rare_cap = c.rare_cap;
+    e.rare_vals = c.rare_vals;
 }
 
 // tables and owner maps for the current member order
@@ -118,19 +143,7 @@ bool build_tables(Batch& b) {
             const int layout_before = (j & 1) ? flip(m.layout) : m.layout;
             const int sc = m.store_cur ^ (j & 1);
             const int tc = P.thermal_on ? (m.t_cur ^ (j & 1)) : m.t_cur;
-            e.P = P;
-            e.C = cell_fields(c);
-            e.mask = c.mask;
-            e.f = aa_buf(c);
-            e.t_in = c.T[tc];
-            e.t_out = c.T[1 - tc];
-            e.u_state = c.u_state;
-            e.store_in = c.face_store[sc];
-            e.store_out = c.face_store[1 - sc];
-            e.rare_list = c.rare_list;
-            e.rare_count = c.rare_count;
-            e.rare_cap = c.rare_cap;
-            e.rare_vals = c.rare_vals;
+            fill_entry(e, c, sc, tc);
             e.bulk_begin = bulk_begin;
             e.gather_begin = gather_begin;
             e.update_begin = update_begin;
@@ -307,7 +320,220 @@ bool autotune(Batch& b, size_t working_set) {
     return build_tables(b);
 }
 
+// ---------------------------------------------------------------- one step of resident contexts
+// The brick-world runtime steps its cached in-place bricks through here, one world
+// step at a time: the contexts of one storage and A-A parity form a group that
+// takes one rare-gather, one rare-update and one bulk launch. A group keeps the
+// device copy of its table and block owners and uploads them again only when an
+// entry changed (parameters, buffers, rare-cell list, membership).
+struct StepGroup {
+    std::vector<BatchBrick> host;  // the entries as last uploaded
+    std::vector<uint16_t> owners_host;
+    BatchBrick* table = nullptr;
+    uint16_t* owners = nullptr;
+    size_t table_cap = 0, owners_cap = 0;
+    BatchGrids grids{};
+    ~StepGroup() {
+        if (table) (void)hipFree(table);
+        if (owners) (void)hipFree(owners);
+    }
+};
+
+// key: storage kind, step mode (1 even, 2 odd) and shell instantiation
+std::unordered_map<int, std::unique_ptr<StepGroup>> g_step_groups;
+
+bool batchable(const Context& c) {
+    const StepParams& P = c.params_host;
+    return (storage_16bit(c.storage) || c.storage == Storage::F32AA) && c.initialized && c.has_payload &&
+           c.params_valid && (c.aa_layout == 1 || c.aa_layout == 2) && !P.runtime_mode && !P.thermal_bfecc &&
+           !P.cylinder && !P.fan_on;
+}
+
+bool step_group(const std::vector<Context*>& ctxs, FastStorage kind, int mode, bool shell) {
+    const int nm = (int)ctxs.size();
+    std::vector<BatchBrick> fresh((size_t)nm);
+    std::vector<uint16_t> bulk, gather, update;
+    size_t working_set = 0;
+    for (int i = 0; i < nm; ++i) {
+        Context& c = *ctxs[i];
+        BatchBrick& e = fresh[(size_t)i];
+        fill_entry(e, c, c.store_cur, c.t_cur);
+        e.bulk_begin = (int)bulk.size();
+        e.gather_begin = (int)gather.size();
+        e.update_begin = (int)update.size();
+        bulk.insert(bulk.end(), fast_tile_blocks(c.params_host), (uint16_t)i);
+        gather.insert(gather.end(), (c.rare_count + 255) / 256, (uint16_t)i);
+        update.insert(update.end(), (c.rare_count + 127) / 128, (uint16_t)i);
+        working_set += 27 * elem_bytes(c.storage) * (size_t)c.params_host.plane + c.cells * 16;
+    }
+    if (bulk.size() > 65535u * 64u || nm > 65535) {
+        set_error("batched world step: too many bricks");
+        return false;
+    }
+    const int key = ((int)kind * 4 + mode) * 2 + (shell ? 1 : 0);
+    std::unique_ptr<StepGroup>& slot = g_step_groups[key];
+    if (!slot) slot = std::make_unique<StepGroup>();
+    StepGroup& g = *slot;
+    hipStream_t st = rt().stream;
+    std::vector<uint16_t> owners;
+    owners.reserve(bulk.size() + gather.size() + update.size());
+    owners.insert(owners.end(), bulk.begin(), bulk.end());
+    owners.insert(owners.end(), gather.begin(), gather.end());
+    owners.insert(owners.end(), update.begin(), update.end());
+    const bool same = g.host.size() == fresh.size() && g.owners_host == owners &&
+                      std::memcmp(g.host.data(), fresh.data(), fresh.size() * sizeof(BatchBrick)) == 0;
+    if (!same) {
+        // the previous upload may still read the host copies: wait before replacing them
+        if (!hip_ok(hipStreamSynchronize(st), "sync")) return false;
+        g.host.swap(fresh);
+        g.owners_host.swap(owners);
+        if (g.host.size() > g.table_cap) {
+            if (g.table) (void)hipFree(g.table);
+            g.table = nullptr;
+            g.table_cap = 0;
+            if (!hip_ok(hipMalloc((void**)&g.table, g.host.size() * sizeof(BatchBrick)), "hipMalloc")) return false;
+            g.table_cap = g.host.size();
+        }
+        if (g.owners_host.size() > g.owners_cap) {
+            if (g.owners) (void)hipFree(g.owners);
+            g.owners = nullptr;
+            g.owners_cap = 0;
+            if (!hip_ok(hipMalloc((void**)&g.owners, g.owners_host.size() * sizeof(uint16_t)), "hipMalloc"))
+                return false;
+            g.owners_cap = g.owners_host.size();
+        }
+        if (!hip_ok(hipMemcpyAsync(g.table, g.host.data(), g.host.size() * sizeof(BatchBrick), hipMemcpyHostToDevice,
+                                   st), "upload step table") ||
+            !hip_ok(hipMemcpyAsync(g.owners, g.owners_host.data(), g.owners_host.size() * sizeof(uint16_t),
+                                   hipMemcpyHostToDevice, st), "upload step owners"))
+            return false;
+        g.grids.bulk_owner = g.owners;
+        g.grids.gather_owner = g.owners + bulk.size();
+        g.grids.update_owner = g.grids.gather_owner + gather.size();
+        g.grids.publish_owner = nullptr;
+        g.grids.bulk_blocks = (int)bulk.size();
+        g.grids.gather_blocks = (int)gather.size();
+        g.grids.update_blocks = (int)update.size();
+        g.grids.publish_blocks = 0;
+    }
+    launch_batch_step(kind, mode, fast_store_policy(working_set), shell, g.table, g.grids, st);
+    if (!hip_ok(hipGetLastError(), "batched world step")) return false;
+    // the state changes of an individual A-A step (storage_step)
+    for (Context* c : ctxs) {
+        c->store_cur ^= 1;
+        c->aa_layout = mode == 1 ? 1 : 2;
+        c->post_collision = true;
+        if (c->params_host.thermal_on) c->t_cur ^= 1;
+        c->steps += 1;
+    }
+    return true;
+}
+
 }  // namespace
+
+bool context_batchable(long long key) {
+    std::lock_guard<std::mutex> lk(rt().mu);
+    const Context* c = find_context(key);
+    return c && batchable(*c);
+}
+
+Context* prepare_cached_step(long long key, int nx, int ny, int nz) {
+    std::lock_guard<std::mutex> lk(rt().mu);
+    clear_error();
+    Context* c = get_context(key, nx, ny, nz, false);
+    if (!c || !c->has_payload) {
+        set_error("batched world step: context " + std::to_string(key) + " has no resident payload");
+        return nullptr;
+    }
+    // the host half of step_context with the resident payload
+    StepParams P;
+    if (!prepare_params(*c, P)) return nullptr;
+    update_thermal_activity(*c, P);
+    P.thermal_on = c->thermal_on ? 1 : 0;
+    step_switches(*c, P);
+    if (!sync_params(*c, P) || !storage_prepare_step(*c, P)) return nullptr;
+    if (!batchable(*c)) {
+        set_error("batched world step: context " + std::to_string(key) + " cannot join a batched step");
+        return nullptr;
+    }
+    return c;
+}
+
+bool batch_step_contexts(const std::vector<Context*>& ctxs) {
+    std::lock_guard<std::mutex> lk(rt().mu);
+    clear_error();
+    // groups in first-appearance order: (storage, mode, shell)
+    std::vector<std::pair<int, std::vector<Context*>>> groups;
+    for (Context* c : ctxs) {
+        if (!c || !batchable(*c)) {
+            set_error("batched world step: a context cannot join a batched step");
+            return false;
+        }
+        const int mode = c->aa_layout == 2 ? 1 : 2;  // after an odd step the next one is even
+        const int key = ((int)kind_of(c->storage) * 4 + mode) * 2 + (c->shell_on ? 1 : 0);
+        auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
+        if (it == groups.end()) {
+            groups.emplace_back(key, std::vector<Context*>{});
+            it = groups.end() - 1;
+        }
+        it->second.push_back(c);
+    }
+    for (auto& g : groups) {
+        Context& c0 = *g.second.front();
+        const int mode = c0.aa_layout == 2 ? 1 : 2;
+        if (!step_group(g.second, kind_of(c0.storage), mode, c0.shell_on)) return false;
+    }
+    return true;
+}
+
+bool publish_context_atlas(long long key, int stride, float vscale, int16_t* out, int value_count) {
+    std::lock_guard<std::mutex> lk(rt().mu);
+    clear_error();
+    Context* c = find_context(key);
+    if (!c || !c->initialized || !c->params_valid || !(storage_16bit(c->storage) || c->storage == Storage::F32AA) ||
+        stride <= 0 || !out) {
+        set_error("packed atlas publish: context missing or not in an in-place storage");
+        return false;
+    }
+    const int sx = (c->nx + stride - 1) / stride, sy = (c->ny + stride - 1) / stride, sz = (c->nz + stride - 1) / stride;
+    const size_t values = (size_t)sx * sy * sz * 4;
+    if ((size_t)value_count != values) {
+        set_error("packed atlas publish: value_count mismatch");
+        return false;
+    }
+    // one table entry whose "step" is the context's last one: store_out is the face
+    // store that step wrote and the layout is the one it left
+    BatchBrick e;
+    fill_entry(e, *c, c->store_cur ^ 1, c->t_cur);
+    e.publish_layout = c->aa_layout;
+    e.publish_stride = stride;
+    e.publish_scale = vscale;
+    const int blocks = (int)((values / 4 + 255) / 256);
+    hipStream_t st = rt().stream;
+    BatchBrick* table = nullptr;
+    uint16_t* owners = nullptr;
+    int16_t* dev = nullptr;
+    bool ok = hip_ok(hipMallocAsync((void**)&table, sizeof(BatchBrick), st), "hipMallocAsync") &&
+              hip_ok(hipMallocAsync((void**)&owners, (size_t)blocks * sizeof(uint16_t), st), "hipMallocAsync") &&
+              hip_ok(hipMallocAsync((void**)&dev, values * sizeof(int16_t), st), "hipMallocAsync");
+    if (ok) {
+        e.publish_out = dev;
+        ok = hip_ok(hipMemcpyAsync(table, &e, sizeof(e), hipMemcpyHostToDevice, st), "upload publish entry") &&
+             hip_ok(hipMemsetAsync(owners, 0, (size_t)blocks * sizeof(uint16_t), st), "owners");
+    }
+    if (ok) {
+        BatchGrids g{};
+        g.publish_owner = owners;
+        g.publish_blocks = blocks;
+        launch_batch_publish(kind_of(c->storage), table, g, st);
+        ok = hip_ok(hipGetLastError(), "publish") &&
+             hip_ok(hipMemcpyAsync(out, dev, values * sizeof(int16_t), hipMemcpyDeviceToHost, st), "atlas readback");
+    }
+    if (table) (void)hipFreeAsync(table, st);
+    if (owners) (void)hipFreeAsync(owners, st);
+    if (dev) (void)hipFreeAsync(dev, st);
+    return hip_ok(hipStreamSynchronize(st), "sync") && ok;
+}
 
 }  // namespace aero
 
@@ -366,7 +592,7 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_batch_create(const long long* keys,
         P.shell_layers = c->shell_on ? c->shell_layers : 0;
         P.shell_relax = c->shell_on ? c->shell_relax : 0.0f;
         if (!sync_params(*c, P)) return 0;
-        Member m{keys[i], c, c->aa_layout, c->store_cur, c->t_cur, 0, 0};
+        Member m{keys[i], c, c->aa_layout, c->store_cur, c->t_cur, 0, 0, c->generation};
         const int sx = (c->nx + publish_stride - 1) / publish_stride, sy = (c->ny + publish_stride - 1) / publish_stride,
                   sz = (c->nz + publish_stride - 1) / publish_stride;
         m.atlas_offset = b->atlas_values;

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/aero_lbm_capi.h"
+#include "runtime_internal.h"
 
 namespace aero {
 namespace bricks {
@@ -98,7 +99,9 @@ struct World {
     float dx = 1.0f, dt = 0.05f;
     int mode = AERO_LBM_REALTIME_SOLVER_CLASSIC_D3Q27;
     bool exact_hints = false;
+    bool batching = true;  // cached in-place bricks step in shared launches
     uint64_t epoch = 0;
+    uint64_t batched_steps = 0;  // world steps whose cached bricks took one batched launch set
     uint64_t cached_steps = 0, packet_steps = 0, dynamic_upload_resets = 0;
     uint64_t patch_attempts = 0, patch_steps = 0, patch_failures = 0, patch_cells = 0;
     BrickMap bricks;
@@ -555,6 +558,24 @@ bool step_cached(World& w, Brick& b) {
     return ok;
 }
 
+// The host half of step_cached for a brick whose step joins the world step's
+// batched launch: parameters resolved under the brick's compact configuration,
+// launch deferred. false: the brick steps on its own (step_actual).
+bool defer_cached(World& w, Brick& b, std::vector<Context*>& batch) {
+    if (!experimental(w) || !b.compact || b.context == 0 || b.geometry_dirty || b.pending_reinit || b.forcing_dirty)
+        return false;
+    if (!context_batchable(b.context)) return false;
+    SolverModeScope scope(w.mode);
+    const int n = w.size;
+    Context* c = configure_compact(b.cv, n) ? prepare_cached_step(b.context, n, n, n) : nullptr;
+    aero_lbm_benchmark_reset_config();
+    if (!c) return false;
+    batch.push_back(c);
+    b.stale = true;
+    w.cached_steps++;
+    return true;
+}
+
 long long next_context_key(Service& s) { return s.next_key--; }
 
 // one brick step: cached when the compact context is ready, else a packet step
@@ -841,11 +862,15 @@ bool step_world(Service& svc, long long key, World& w, int steps) {
             if (solver_active(w, e.second) && dyn_valid(w, e.second.dyn.get())) snaps.emplace(e.first, e.second.dyn);
         // the population halo between face-adjacent contexts (ref: 1056-1089) is left
         // out: the in-place contexts keep their populations in the 16-bit A-A
-        // buffer, which the reference's exchange (jni.cpp:7817-7818) does not touch
+        // buffer, which the reference's exchange (jni.cpp:7817-7818) does not touch.
+        // Cached in-place bricks are independent within a world step, so their steps
+        // are collected and launched together after the loop.
+        std::vector<Context*> batch;
         for (auto& e : w.bricks) {
             Brick& b = e.second;
             if (!solver_active(w, b) || !b.dyn) continue;
             if (compact_ready(w, b) && !needs_seed.count(e.first)) {
+                if (w.batching && defer_cached(w, b, batch)) continue;
                 if (!step_actual(svc, w, e.first, b, *b.dyn)) all_ok = false;
                 continue;
             }
@@ -879,6 +904,14 @@ bool step_world(Service& svc, long long key, World& w, int steps) {
                 b.dyn = std::move(trial.dyn);
             }
             b.pending_reinit = b.geometry_dirty = b.forcing_dirty = false;
+        }
+        if (!batch.empty()) {
+            if (batch_step_contexts(batch)) {
+                w.batched_steps++;
+            } else {
+                set_err(std::string("step_brick_world_runtime: batched step failed: ") + aero_lbm_last_error());
+                all_ok = false;
+            }
         }
         recompute_active(w);
         prune(w);
@@ -1067,7 +1100,7 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_brick_world_solver_m
     std::lock_guard<std::mutex> lk(g_mu);
     World* w = find_world(lookup(skey), wkey, "simulation_set_brick_world_solver_mode");
     if (!w) return 0;
-    const int m = (mode >= AERO_LBM_REALTIME_SOLVER_AUTO && mode <= AERO_LBM_REALTIME_SOLVER_D3Q27_FP32_INPLACE)
+    const int m = (mode >= AERO_LBM_REALTIME_SOLVER_AUTO && mode <= AERO_LBM_REALTIME_SOLVER_D3Q27_S16_INPLACE)
                       ? mode
                       : AERO_LBM_REALTIME_SOLVER_CLASSIC_D3Q27;
     if (m != w->mode) {
@@ -1078,6 +1111,17 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_brick_world_solver_m
         }
         w->mode = m;
     }
+    return 1;
+}
+
+// extension: batched stepping of a world's cached in-place bricks (on by default);
+// off, every brick steps through its own cached-step call as the reference does
+extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_brick_world_batching(long long skey, long long wkey,
+                                                                                int enabled) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    World* w = find_world(lookup(skey), wkey, "simulation_set_brick_world_batching");
+    if (!w) return 0;
+    w->batching = enabled != 0;
     return 1;
 }
 
@@ -1391,6 +1435,11 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_packed_
     if (it == w->bricks.end() || !it->second.dyn) return 0;
     Brick& b = it->second;
     const float vscale = w->dt > 0.0f ? w->dx / w->dt : 1.0f;
+    // an in-place context publishes on the device: int16 quantisation there, a quarter
+    // of the float atlas read back
+    if (b.context != 0 && experimental(*w) && aero::context_batchable(b.context) &&
+        aero::publish_context_atlas(b.context, stride, vscale, out, value_count))
+        return 1;
     if (b.context != 0) {
         std::vector<float> atlas((size_t)acells * kFlowCh);
         if (!aero_lbm_extract_flow_atlas_rect(size, size, size, b.context, stride, atlas.data(), (int)atlas.size())) {
@@ -1471,6 +1520,7 @@ extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info() {
     std::lock_guard<std::mutex> lk(g_mu);
     size_t worlds = 0, bricks = 0, active = 0, contexts = 0, compact = 0, atlases = 0;
     uint64_t cached = 0, packet = 0, resets = 0, p_try = 0, p_ok = 0, p_fail = 0, p_cells = 0, epoch = 0;
+    uint64_t batched = 0;
     int mode = -1;
     bool mixed = false;
     if (g_service) {
@@ -1487,6 +1537,7 @@ extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info() {
                 compact += bv.second.compact ? 1 : 0;
             }
             cached += w.cached_steps;
+            batched += w.batched_steps;
             packet += w.packet_steps;
             resets += w.dynamic_upload_resets;
             p_try += w.patch_attempts;
@@ -1501,6 +1552,7 @@ extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info() {
            " bricks=" + std::to_string(bricks) + " active=" + std::to_string(active) +
            " contexts=" + std::to_string(contexts) + " cached_contexts=" + std::to_string(compact) +
            " atlases=" + std::to_string(atlases) + " cached_steps=" + std::to_string(cached) +
+           " batched_world_steps=" + std::to_string(batched) +
            " packet_steps=" + std::to_string(packet) + " dynamic_upload_resets=" + std::to_string(resets) +
            " patch_attempts=" + std::to_string(p_try) + " patch_steps=" + std::to_string(p_ok) +
            " patch_failures=" + std::to_string(p_fail) + " patch_cells=" + std::to_string(p_cells) +

@@ -109,7 +109,11 @@ struct StS16 {
         return (uint16_t)k.x;
     }
     static __device__ __forceinline__ float unpack(uint32_t b, int q) {
-        return (float)(int)(int16_t)(uint16_t)b * step(q);
+        // the product is kept as its own rounded value (the empty asm hides it from
+        // FMA contraction), so every instantiation of the step sums the same values
+        float v = (float)(int)(int16_t)(uint16_t)b * step(q);
+        __asm__("" : "+v"(v));
+        return v;
     }
     static __device__ __forceinline__ float ld(const T* p, int q) { return unpack(__builtin_nontemporal_load(p), q); }
     static __device__ __forceinline__ void st(T* p, float v, int q) { __builtin_nontemporal_store(pack(v, q), p); }

@@ -423,6 +423,7 @@ bool sync_params(Context& c, StepParams& P) {
     P.self_dev = c.params_dev;
     if (!c.params_valid || std::memcmp(&c.params_host, &P, sizeof(P)) != 0) {
         if (!hip_ok(hipStreamSynchronize(rt().stream), "sync")) return false;
+        c.generation++;
         std::memcpy(&c.params_host, &P, sizeof(P));
         if (!hip_ok(hipMemcpyAsync(c.params_dev, &c.params_host, sizeof(P), hipMemcpyHostToDevice, rt().stream),
                     "upload step parameters"))
@@ -613,6 +614,7 @@ void step_switches(const Context& c, StepParams& P) {
 // One solver step of a context. packet == nullptr means "resident payload";
 // packet_on_device says the packet is in device memory.
 bool step_context(Context& c, const float* packet, float* out, int out_ch, float scale, bool packet_on_device) {
+    c.generation++;  // stepped outside any batch
     StepParams P;
     if (!prepare_params(c, P)) return false;
     if (packet) {
@@ -915,6 +917,7 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_set_temperature_state_rect(int nx, int ny, int
         tmp[i] = (m < 1.0f) ? m : 1.0f;
         if (tmp[i] != 0.0f) c->temperature_nonzero = true;
     }
+    c->generation++;
     return (hip_ok(hipMemcpyAsync(c->T[c->t_cur], tmp.data(), c->cells * 4, hipMemcpyHostToDevice, rt().stream),
                    "upload") &&
             hip_ok(hipStreamSynchronize(rt().stream), "sync"))

@@ -33,6 +33,11 @@ struct Context {
     bool payload_src_nonzero = false;
     bool payload_t_nonzero = false;
     uint64_t steps = 0;
+    // Bumped by every change that can outdate a batch's copy of this context's step
+    // parameters or buffer pointers: an individual step, a parameter upload, a link
+    // mask / rare-list rebuild, a temperature upload, a shell change. Batches record it
+    // and refuse to launch when it moved.
+    uint64_t generation = 0;
     int f_cur = 0;                 // A-B: index of the buffer holding the state
     int t_cur = 0;                 // temperature ping-pong index
     int aa_layout = 3;             // A-A: 1 after an even step, 2 after an odd step, 3 initial state
@@ -148,6 +153,8 @@ int step_rect_device_packet(const float* packet_dev, int nx, int ny, int nz, lon
 // context `key` exists with these dimensions and holds an ingested payload
 bool context_resident(long long key, int nx, int ny, int nz);
 void step_switches(const Context& c, StepParams& P);
+// the sticky thermal switch of a context under parameters P
+void update_thermal_activity(Context& c, const StepParams& P);
 
 // storage-specific operations (storage.hip)
 bool storage_alloc(Context& c);
@@ -156,5 +163,23 @@ bool storage_readout(Context& c, const StepParams& P, int stride, int out_ch, fl
 bool storage_sample(Context& c, const StepParams& P, int x, int y, int z, float* out_dev4);
 size_t context_device_bytes(const Context& c);
 bool storage_rebuild_masks(Context& c, const StepParams& P);
+// buffers a step with parameters P needs besides the populations (A-A face stores)
+bool storage_prepare_step(Context& c, const StepParams& P);
+
+// ---- batched stepping of resident in-place contexts (batch.hip) ----
+// The context can join a batched step: FP16 / BF16 / S16 / FP32 A-A storage, stepped
+// at least once (A-A layout 1 or 2), resident payload, benchmark-mode parameters.
+bool context_batchable(long long key);
+// The host half of a cached step of `key` under the current configuration: the
+// parameters are resolved and uploaded as aero_lbm_step_rect_cached would, without
+// the launch. Returns the context, or nullptr with the error set.
+Context* prepare_cached_step(long long key, int nx, int ny, int nz);
+// One step of every listed context (prepared by prepare_cached_step), in shared
+// launches: contexts at the same storage and parity share one rare-gather, one
+// rare-update and one bulk launch.
+bool batch_step_contexts(const std::vector<Context*>& ctxs);
+// int16 packed flow atlas of one in-place context, quantised on the device (velocity
+// x vscale over +-5.6 m/s, pressure over +-0.03), copied to out (4 values per sample)
+bool publish_context_atlas(long long key, int stride, float vscale, int16_t* out, int value_count);
 
 }  // namespace aero

@@ -88,6 +88,7 @@ static bool shell_supported(const Context& c) { return c.storage != Storage::F32
 bool context_set_shell(Context& c, const float* flow_state, int flow_channels, int face_mask, int shell_layers,
                        float relaxation) {
     const float relax = std::isfinite(relaxation) ? std::fmin(std::fmax(relaxation, 0.0f), 0.5f) : 0.0f;
+    c.generation++;
     if (face_mask == 0 || shell_layers <= 0 || relax <= 0.0f) {
         c.shell_on = false;
         return true;
@@ -163,5 +164,6 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_clear_d3q27_f16_boundary_shell_rect
         return 0;
     }
     c->shell_on = false;
+    c->generation++;
     return 1;
 }

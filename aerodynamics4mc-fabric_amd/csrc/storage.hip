@@ -92,6 +92,7 @@ void mask_signature(const StepParams& P, int* out) {
 
 bool storage_rebuild_masks(Context& c, const StepParams& P) {
     if (!c.mask) return true;
+    c.generation++;
     hipStream_t st = rt().stream;
     launch_fast_mask(P, c.flags, c.mask, st);
     launch_fast_rare_list(P, c.mask, c.rare_list, c.rare_scratch, c.rare_counter, st);
@@ -121,6 +122,7 @@ bool storage_rebuild_masks(Context& c, const StepParams& P) {
 static bool ensure_face_store(Context& c, const StepParams& P) {
     if ((size_t)P.face_store_floats <= c.face_store_floats) return true;
     if (!hip_ok(hipStreamSynchronize(rt().stream), "sync")) return false;
+    c.generation++;
     for (int i = 0; i < 2; ++i) {
         if (c.face_store[i]) {
             hipFree(c.face_store[i]);
@@ -134,6 +136,10 @@ static bool ensure_face_store(Context& c, const StepParams& P) {
     }
     c.face_store_floats = (size_t)P.face_store_floats;
     return true;
+}
+
+bool storage_prepare_step(Context& c, const StepParams& P) {
+    return c.storage == Storage::F32Exact || ensure_face_store(c, P);
 }
 
 bool storage_step(Context& c, const StepParams& P) {

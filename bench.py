@@ -98,31 +98,81 @@ def run_mode(lib, A, W, mode, n, steps, warmup, inputs):
             "finite": finite, "max_speed": float(np.abs(flow[:, :3]).max())}
 
 
-def cpu_baseline(n, inputs, sample_steps=8):
-    """The FP32 oracle (plain-C restatement of the reference's single-threaded CPU
-    step) on this host, one core, on a bounded sample of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    pyoracle.set_threads(1)  # the reference CPU step is single-threaded
-    cfg, ov, packet, temp, solid = inputs
-    o = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=32)
-    o.set_temperature(temp)
-    o.run(packet, 1)  # warm-up step (includes initialisation)
-    t0 = time.perf_counter()
-    for _ in range(sample_steps):
-        o.step(None)
-    dt = time.perf_counter() - t0
-    cpu = platform.processor() or platform.machine()
+def host_cpu_facts() -> dict:
+    """lscpu-style facts of this host: model name, logical CPUs, CPUs this process may
+    use, and whether SMT is on."""
+    model = platform.processor() or platform.machine()
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
+                model = line.split(":", 1)[1].strip()
                 break
     except OSError:
         pass
-    return {"value": n ** 3 * sample_steps / dt / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
-            "sample": f"{sample_steps} steps of the same {n}^3 workload after 1 warm-up step, FP32 oracle "
-                      f"(oracle/liboracle.so), single thread on {cpu}; {dt:.1f} s"}
+    smt = None
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        pass
+    allowed = sorted(os.sched_getaffinity(0))
+    return {"model": model, "logical_cpus": os.cpu_count(), "allowed_cpus": len(allowed), "smt": smt}
+
+
+def cpu_worker(n: int, warm: int, steps: int, cpu: int) -> None:
+    """One CPU-baseline process: pinned to `cpu`, the FP32 oracle with one thread on
+    its own context of the 128^3 workload; prints its rate as JSON. Touches no GPU."""
+    os.sched_setaffinity(0, {cpu})
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    import workloads as W
+    pyoracle.set_threads(1)  # the reference CPU step is single-threaded
+    cfg, ov, packet, temp, solid = W.config3(n, beta=0.1, cs=0.1)
+    o = pyoracle.Oracle(n, n, n, cfg, overrides=ov, precision=32)
+    o.set_temperature(temp)
+    o.run(packet, warm)  # warm-up steps (the first includes initialisation)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(None)
+    dt = time.perf_counter() - t0
+    print(json.dumps({"cpu": cpu, "steps": steps, "seconds": dt, "lups": n ** 3 * steps / dt}), flush=True)
+
+
+def run_cpu_workers(n, warm, steps, cpus):
+    """Independent worker processes, one per listed CPU, started together."""
+    import subprocess
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                               f"{n},{warm},{steps},{c}"], stdout=subprocess.PIPE, text=True) for c in cpus]
+    out = []
+    for p in procs:
+        text, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu worker failed with status {p.returncode}")
+        out.append(json.loads(text.strip().splitlines()[-1]))
+    return out
+
+
+def cpu_baseline(n, sample_steps=8, agg_steps=3, max_procs=16):
+    """The FP32 oracle (plain-C restatement of the reference's single-threaded CPU
+    step) on this host's cores, on bounded samples of the same 128^3 workload: one
+    process pinned to one core (the reference's own configuration), then the
+    aggregate of independent contexts, one process per core, over the cores this
+    process may use (at most max_procs)."""
+    facts = host_cpu_facts()
+    allowed = sorted(os.sched_getaffinity(0))
+    one = run_cpu_workers(n, 2, sample_steps, allowed[:1])[0]
+    k = min(max_procs, len(allowed))
+    t0 = time.perf_counter()
+    many = run_cpu_workers(n, 1, agg_steps, allowed[:k])
+    wall = time.perf_counter() - t0
+    agg = sum(w["lups"] for w in many)
+    return {"value": one["lups"] / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": f"{sample_steps} steps of the same {n}^3 workload after 2 warm-up steps, FP32 oracle "
+                      f"(oracle/liboracle.so), one thread pinned to CPU {allowed[0]} of a {facts['model']}; "
+                      f"{one['seconds']:.1f} s",
+            "all_cores": {"value": agg / 1e9, "unit": "GLUPS", "cores": k,
+                          "sample": f"{k} independent processes, one pinned per CPU, each {agg_steps} steps of "
+                                    f"its own {n}^3 context after 1 warm-up step; {wall:.1f} s wall"},
+            "host": facts}
 
 
 MESO_DIMS = (256, 64, 256)  # x, y (vertical), z
@@ -292,10 +342,15 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-bricks line")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-fit line")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) five times, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.cpu_worker:
+        n_, warm_, steps_, cpu_ = (int(v) for v in args.cpu_worker.split(","))
+        cpu_worker(n_, warm_, steps_, cpu_)
+        return
     if args.calibrate_copy:
         import torch
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -357,8 +412,8 @@ def main():
     if "fp32" in results and head != "fp32":
         line["fp16_over_fp32"] = round(results[head]["glups_wall"] / results["fp32"]["glups_wall"], 3)
     if not args.no_cpu:
-        print("bench: cpu baseline (oracle, 1 core)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(n, inputs)
+        print("bench: cpu baseline (oracle, 1 core, then one process per core)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(n)
     if not args.no_meso:
         print(f"bench: mesoscale d3q19 {MESO_DIMS} steps={args.meso_steps}", file=sys.stderr, flush=True)
         line["mesoscale"] = run_meso(args.meso_steps, min(args.warmup, 20), args.no_cpu)

@@ -395,6 +395,13 @@ AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_last_error(void);
  * is written and copied to host memory. The cycle is captured in a HIP graph.
  * autotune_launch: time brick orders and store cache policies, keep the fastest.
  * Stepping a member outside the batch invalidates the batch. */
+/* Brick worlds use the same launches: each world step collects the steps of the
+ * cached in-place bricks (the reference's step_brick_compact_cached per brick,
+ * native/src/aero_lbm_simulation_bridge.cpp:1678-1704) and runs them as one launch
+ * set per storage and parity; their packed atlases are quantised on the device.
+ * enabled = 0 steps each brick through its own cached-step call instead. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_set_brick_world_batching(long long service_key, long long world_key,
+                                                                      int enabled);
 AERO_LBM_CAPI_EXPORT int aero_lbm_batch_create(const long long* context_keys, int count, int steps_per_publish,
                                                int publish_stride, const float* publish_scales, int autotune_launch,
                                                long long* out_batch);

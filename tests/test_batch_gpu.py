@@ -134,3 +134,29 @@ def test_batch_rejects_mixed_parity_and_outside_steps(lib):
     b.release()
     for k in ka:
         lib.aero_lbm_release_context(k)
+
+
+def test_batch_rejects_member_stepped_twice_with_new_obstacles(lib):
+    """Two outside steps of a member restore its parity, and a new packet rebuilds its
+    link mask and rare-cell list; the batch must refuse to launch on its stale tables
+    (the context's generation moved) before touching any device memory."""
+    ka, cfgs = make_contexts(lib, A.SOLVER_D3Q27_FP16_INPLACE, 7400)
+    b = A.Batch(ka, steps_per_publish=2)
+    b.run(1)
+    n, seed, speed, ang, thermal = SPECS[2]
+    solid = W.solid_boxes(n, n, n, 0.2, seed=99, edge=(2, 5), keep_clear_x=2)
+    pk = W.make_packet(solid, u0=(speed, 0.0, 0.0), perturb=1e-3, seed_u=5)
+    A.check(lib.aero_lbm_benchmark_set_config(ctypes.byref(cfgs[2])), "config")
+    lib.aero_lbm_set_realtime_solver_mode(A.SOLVER_D3Q27_FP16_INPLACE)
+    A.check(lib.aero_lbm_step_rect(A.fptr(pk), n, n, n, ka[2], None), "packet step")
+    A.check(lib.aero_lbm_step_rect_cached(n, n, n, ka[2], None), "cached step")
+    lib.aero_lbm_set_realtime_solver_mode(0)
+    before = flows(lib, ka)
+    with pytest.raises(A.AeroLbmError, match="outside the batch"):
+        b.run(1)
+    after = flows(lib, ka)
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    b.release()
+    for k in ka:
+        lib.aero_lbm_release_context(k)
