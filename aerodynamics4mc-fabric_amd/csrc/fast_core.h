@@ -109,11 +109,7 @@ struct StS16 {
         return (uint16_t)k.x;
     }
     static __device__ __forceinline__ float unpack(uint32_t b, int q) {
-        // the product is kept as its own rounded value (the empty asm hides it from
-        // FMA contraction), so every instantiation of the step sums the same values
-        float v = (float)(int)(int16_t)(uint16_t)b * step(q);
-        __asm__("" : "+v"(v));
-        return v;
+        return (float)(int)(int16_t)(uint16_t)b * step(q);
     }
     static __device__ __forceinline__ float ld(const T* p, int q) { return unpack(__builtin_nontemporal_load(p), q); }
     static __device__ __forceinline__ void st(T* p, float v, int q) { __builtin_nontemporal_store(pack(v, q), p); }
@@ -497,6 +493,27 @@ __device__ __forceinline__ void add_guo(float g[27], float fa, float ux, float u
 #undef GM
 }
 
+// The Guo source S_q = w_q [3 (c_q - u).F + 9 (c_q.u)(c_q.F)] for a force (0, 0, fa)
+// has 12 non-zero raw moments (weight moments E(0) = 1, E(1) = E(3) = 0,
+// E(2) = E(4) = 1/3): (a,b,1) = e_a e_b fa, (1,b,1) = e_b u_x fa, (a,1,1) = e_a u_y fa,
+// (a,b,2) = 2 e_a e_b u_z fa for a, b in {0, 2} (e_0 = 1, e_2 = 1/3). Added in the
+// halved form (scale 2^-(non-zero indices)).
+__device__ __forceinline__ void add_guo_z(float g[27], float fa, float ux, float uy, float uz) {
+    const float xf = ux * fa, yf = uy * fa, zf = uz * fa;
+    g[IDX(0, 0, 1)] = fmaf(0.5f, fa, g[IDX(0, 0, 1)]);
+    g[IDX(2, 0, 1)] = fmaf(0.25f / 3.0f, fa, g[IDX(2, 0, 1)]);
+    g[IDX(0, 2, 1)] = fmaf(0.25f / 3.0f, fa, g[IDX(0, 2, 1)]);
+    g[IDX(2, 2, 1)] = fmaf(0.125f / 9.0f, fa, g[IDX(2, 2, 1)]);
+    g[IDX(1, 0, 1)] = fmaf(0.25f, xf, g[IDX(1, 0, 1)]);
+    g[IDX(1, 2, 1)] = fmaf(0.125f / 3.0f, xf, g[IDX(1, 2, 1)]);
+    g[IDX(0, 1, 1)] = fmaf(0.25f, yf, g[IDX(0, 1, 1)]);
+    g[IDX(2, 1, 1)] = fmaf(0.125f / 3.0f, yf, g[IDX(2, 1, 1)]);
+    g[IDX(0, 0, 2)] = fmaf(1.0f, zf, g[IDX(0, 0, 2)]);
+    g[IDX(2, 0, 2)] = fmaf(0.5f / 3.0f, zf, g[IDX(2, 0, 2)]);
+    g[IDX(0, 2, 2)] = fmaf(0.5f / 3.0f, zf, g[IDX(0, 2, 2)]);
+    g[IDX(2, 2, 2)] = fmaf(0.25f / 9.0f, zf, g[IDX(2, 2, 2)]);
+}
+
 struct CollideResult {
     float ux, uy, uz;
     bool ok;
@@ -648,11 +665,29 @@ __device__ __forceinline__ void add_guo_xy(float g[27], float fx, float fy, floa
 #undef GM
 }
 
+// hardware reciprocal, square root and reciprocal square root (about 1 ulp), for
+// operands the collision keeps in the normal range (density >= 1e-6, relaxation
+// times in [0.5, 0.95], speeds and stress norms away from the denormals)
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Central moments -> raw moments along one axis in the halved form (see unshift_half),
+// for the three patterns of known zeros of the post-collision central moments:
+// (k0, 0, k2), (0, k1, 0) and (k0, k1, k2).
+__device__ __forceinline__ void unshift_k0k2(float k0, float& k1, float& k2, float hu, float hu2) {
+    k1 = hu * k0;
+    k2 = fmaf(hu2, k0, k2);
+}
+__device__ __forceinline__ void unshift_k1(float k1, float& k2, float tu) { k2 = tu * k1; }
+
 // g: populations in stored form (f - w when Shifted), overwritten with the post-collision values.
 // The collision needs only the raw moments of order <= 2 (density, momentum and the
 // second moments, from which the second central moments follow); the higher central
-// moments are set by the closure. The post-collision central moments go back to
-// populations through raw moments kept in the halved form of unshift_half/inv_half.
+// moments are set by the closure. The post-collision central moments (14 non-zero of
+// 27) go back to populations through raw moments kept in the halved form of
+// unshift_half / inv_half; the back-transform skips the known zeros, and the Guo
+// source enters as its 12 non-zero raw moments.
 // GravZ: the caller guarantees the gravity axis is z whenever buoyancy is on (a
 // separate instantiation: the x/y branch of the force costs registers)
 template <bool Shifted, bool GravZ = false>
@@ -671,8 +706,8 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
     r.ok = finite(rho) && finite(jx) && finite(jy) && finite(jz);
     if (!r.ok) return r;
     const float rho_s = fmaxf(1e-6f, rho);
-    const float inv = 1.0f / rho_s;
-    float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    const float ir = rcp_hw(rho_s);
+    float ux = jx * ir, uy = jy * ir, uz = jz * ir;
     // Boussinesq force along the gravity axis; F/rho = clamp(beta T)
     float b = 0.0f;
     if (P.buoyancy_on) {
@@ -685,7 +720,7 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
     {
         const float s2 = ux * ux + uy * uy + uz * uz;
         if (s2 > P.kmax_speed * P.kmax_speed) {
-            const float sc = P.kmax_speed / sqrtf(s2);
+            const float sc = P.kmax_speed * rsq_hw(s2);
             ux *= sc;
             uy *= sc;
             uz *= sc;
@@ -710,14 +745,15 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
         const float e = rho_s * cs2;
         const float nxx = c200 - e, nyy = c020 - e, nzz = c002 - e;
         const float qn2 = nxx * nxx + nyy * nyy + nzz * nzz + 2.0f * (c110 * c110 + c101 * c101 + c011 * c011);
-        const float smag = sqrtf(fmaxf(0.0f, qn2)) / fmaxf(1e-6f, 2.0f * rho_s * nu0);
+        const float smag = sqrt_hw(fmaxf(0.0f, qn2)) * rcp_hw(fmaxf(1e-6f, 2.0f * rho_s * nu0));
         const float nut = fminf(P.sgs_c2 * smag, P.nut_cap_ratio * nu0);
         const float tau_min = 0.5f + 3.0f * 1.0e-8f;
         ts = clampf(0.5f + 3.0f * clampf(nu0 + nut, 1.0e-8f, (0.95f - 0.5f) / 3.0f), tau_min, 0.95f);
         tn = clampf(P.tau_normal_base + (ts - P.tau_shear_base) * 0.30f, tau_min, 0.95f);
     }
-    const float sd = clampf(1.0f / tn, 0.0f, 1.95f);
-    const float so = clampf(1.0f / ts, 0.0f, 1.95f);
+    const float rts = rcp_hw(ts);
+    const float sd = clampf(rcp_hw(tn), 0.0f, 1.95f);
+    const float so = clampf(rts, 0.0f, 1.95f);
     // relaxation + Gaussian closure (ref: jni.cpp:1384-1430)
     const float eq2 = rho_s * cs2;
     const float d200 = fmaf(sd, eq2 - c200, c200);
@@ -725,57 +761,59 @@ __device__ __forceinline__ CollideResult collide(const StepParams& P, float g[27
     const float d002 = fmaf(sd, eq2 - c002, c002);
     const float om = 1.0f - so;
     const float d110 = om * c110, d101 = om * c101, d011 = om * c011;
-    const float ir = 1.0f / rho_s;
     const float t110 = 2.0f * d110, t101 = 2.0f * d101, t011 = 2.0f * d011;
-    const float d220 = (d200 * d020 + t110 * d110) * ir;
-    const float d202 = (d200 * d002 + t101 * d101) * ir;
-    const float d022 = (d020 * d002 + t011 * d011) * ir;
-    const float d211 = (d200 * d011 + t110 * d101) * ir;
-    const float d121 = (d020 * d101 + t110 * d011) * ir;
-    const float d112 = (d002 * d110 + t101 * d011) * ir;
-    const float d222 = (d200 * d020 * d002 + t110 * d110 * d002 + t101 * d101 * d020 + t011 * d011 * d200 +
-                        8.0f * d110 * d101 * d011) * ir * ir;
-    // post-collision central moments in the halved form; the others are zero
-#pragma unroll
-    for (int k = 0; k < 27; ++k) g[k] = 0.0f;
-    g[IDX(0, 0, 0)] = rho_s;
-    g[IDX(2, 0, 0)] = 0.5f * d200;
-    g[IDX(0, 2, 0)] = 0.5f * d020;
-    g[IDX(0, 0, 2)] = 0.5f * d002;
-    g[IDX(1, 1, 0)] = 0.25f * d110;
-    g[IDX(1, 0, 1)] = 0.25f * d101;
-    g[IDX(0, 1, 1)] = 0.25f * d011;
-    g[IDX(2, 2, 0)] = 0.25f * d220;
-    g[IDX(2, 0, 2)] = 0.25f * d202;
-    g[IDX(0, 2, 2)] = 0.25f * d022;
-    g[IDX(2, 1, 1)] = 0.125f * d211;
-    g[IDX(1, 2, 1)] = 0.125f * d121;
-    g[IDX(1, 1, 2)] = 0.125f * d112;
-    g[IDX(2, 2, 2)] = 0.125f * d222;
-    // central -> raw (halved form)
+    // post-collision central moments in the halved form (scale 2^-(non-zero indices));
+    // the 1/rho of the closure carries the scale of the fourth-order ones
+    const float irq = 0.25f * ir, ire = 0.125f * ir;
+    const float r0 = rho_s;
+    const float A = 0.5f * d200, B = 0.5f * d020, C = 0.5f * d002;
+    const float Dxy = 0.25f * d110, Dxz = 0.25f * d101, Dyz = 0.25f * d011;
+    const float Exy = (d200 * d020 + t110 * d110) * irq;
+    const float Exz = (d200 * d002 + t101 * d101) * irq;
+    const float Eyz = (d020 * d002 + t011 * d011) * irq;
+    const float Fx = (d200 * d011 + t110 * d101) * ire;
+    const float Fy = (d020 * d101 + t110 * d011) * ire;
+    const float Fz = (d002 * d110 + t101 * d011) * ire;
+    const float G = (d200 * d020 * d002 + t110 * d110 * d002 + t101 * d101 * d020 + t011 * d011 * d200 +
+                     8.0f * d110 * d101 * d011) * ire * ir;
+    // central -> raw, z lines (a, b): (a,b,0), (a,b,1), (a,b,2); zero where marked
     {
         const float hz = 0.5f * uz, tz = 2.0f * uz, hz2 = 0.5f * uz * uz;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) unshift_half(g[i * 3 + 0], g[i * 3 + 1], g[i * 3 + 2], hz, tz, hz2);
-        const float hy = 0.5f * uy, ty = 2.0f * uy, hy2 = 0.5f * uy * uy;
-#pragma unroll
-        for (int kx = 0; kx < 3; ++kx)
-#pragma unroll
-            for (int kz = 0; kz < 3; ++kz)
-                unshift_half(g[IDX(kx, 0, kz)], g[IDX(kx, 1, kz)], g[IDX(kx, 2, kz)], hy, ty, hy2);
-        const float hx = 0.5f * ux, tx = 2.0f * ux, hx2 = 0.5f * ux * ux;
-#pragma unroll
-        for (int ky = 0; ky < 3; ++ky)
-#pragma unroll
-            for (int kz = 0; kz < 3; ++kz)
-                unshift_half(g[IDX(0, ky, kz)], g[IDX(1, ky, kz)], g[IDX(2, ky, kz)], hx, tx, hx2);
+        g[IDX(0, 0, 0)] = r0;   g[IDX(0, 0, 2)] = C;   unshift_k0k2(r0, g[IDX(0, 0, 1)], g[IDX(0, 0, 2)], hz, hz2);
+        g[IDX(1, 0, 0)] = 0.0f; g[IDX(1, 0, 1)] = Dxz; unshift_k1(Dxz, g[IDX(1, 0, 2)], tz);
+        g[IDX(2, 0, 0)] = A;    g[IDX(2, 0, 2)] = Exz; unshift_k0k2(A, g[IDX(2, 0, 1)], g[IDX(2, 0, 2)], hz, hz2);
+        g[IDX(0, 1, 0)] = 0.0f; g[IDX(0, 1, 1)] = Dyz; unshift_k1(Dyz, g[IDX(0, 1, 2)], tz);
+        g[IDX(1, 1, 0)] = Dxy;  g[IDX(1, 1, 2)] = Fz;  unshift_k0k2(Dxy, g[IDX(1, 1, 1)], g[IDX(1, 1, 2)], hz, hz2);
+        g[IDX(2, 1, 0)] = 0.0f; g[IDX(2, 1, 1)] = Fx;  unshift_k1(Fx, g[IDX(2, 1, 2)], tz);
+        g[IDX(0, 2, 0)] = B;    g[IDX(0, 2, 2)] = Eyz; unshift_k0k2(B, g[IDX(0, 2, 1)], g[IDX(0, 2, 2)], hz, hz2);
+        g[IDX(1, 2, 0)] = 0.0f; g[IDX(1, 2, 1)] = Fy;  unshift_k1(Fy, g[IDX(1, 2, 2)], tz);
+        g[IDX(2, 2, 0)] = Exy;  g[IDX(2, 2, 2)] = G;   unshift_k0k2(Exy, g[IDX(2, 2, 1)], g[IDX(2, 2, 2)], hz, hz2);
     }
-    // Guo source as raw moments (ref: jni.cpp:1270-1281, added after the collision)
+    // y lines (a, c); (1,0,0), (0,1,0), (2,1,0), (1,2,0) are still zero
+    {
+        const float hy = 0.5f * uy, ty = 2.0f * uy, hy2 = 0.5f * uy * uy;
+        unshift_k0k2(r0, g[IDX(0, 1, 0)], g[IDX(0, 2, 0)], hy, hy2);
+        unshift_k1(Dxy, g[IDX(1, 2, 0)], ty);
+        unshift_k0k2(A, g[IDX(2, 1, 0)], g[IDX(2, 2, 0)], hy, hy2);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 1; c < 3; ++c)
+                unshift_half(g[IDX(a, 0, c)], g[IDX(a, 1, c)], g[IDX(a, 2, c)], hy, ty, hy2);
+    }
+    // x lines (b, c); (1,0,0) is still zero
+    {
+        const float hx = 0.5f * ux, tx = 2.0f * ux, hx2 = 0.5f * ux * ux;
+        unshift_k0k2(r0, g[IDX(1, 0, 0)], g[IDX(2, 0, 0)], hx, hx2);
+#pragma unroll
+        for (int k = 1; k < 9; ++k) unshift_half(g[IDX(0, k / 3, k % 3)], g[IDX(1, k / 3, k % 3)], g[IDX(2, k / 3, k % 3)], hx, tx, hx2);
+    }
+    // Guo source (ref: jni.cpp:1270-1281, added after the collision) as raw moments
     if (P.buoyancy_on && b != 0.0f) {
-        const float pref = fmaxf(0.0f, 1.0f - 0.5f * clampf(1.0f / ts, 0.0f, 1.999f));
+        const float pref = fmaxf(0.0f, 1.0f - 0.5f * clampf(rts, 0.0f, 1.999f));
         const float fa = pref * rho_s * b;  // P * F along the axis
         // gravity along z (the common case) with the x and y force terms left out
-        if (GravZ || P.g_axis == 2) add_guo<2>(g, fa, ux, uy, uz);
+        if (GravZ || P.g_axis == 2) add_guo_z(g, fa, ux, uy, uz);
         else add_guo_xy(g, P.g_axis == 0 ? fa : 0.0f, P.g_axis == 1 ? fa : 0.0f, ux, uy, uz);
     }
     if (Shifted) {

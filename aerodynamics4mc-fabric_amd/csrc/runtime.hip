@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -447,10 +448,22 @@ bool Context::alloc(void** p, size_t bytes) {
     return true;
 }
 
+// The reference's AERO_LBM_D3Q27_FP16_INPLACE switch (native/src/aero_lbm_jni.cpp:5730-5764):
+// with the solver mode left at auto, a benchmark-mode context without a preset
+// takes the FP16 in-place storage. Any value other than 0 / false / off turns it on.
+static bool fp16_inplace_env() {
+    const char* v = std::getenv("AERO_LBM_D3Q27_FP16_INPLACE");
+    return v && std::strcmp(v, "0") && std::strcmp(v, "false") && std::strcmp(v, "FALSE") && std::strcmp(v, "off") &&
+           std::strcmp(v, "OFF");
+}
+
 static Storage storage_for_mode(int mode, int variant) {
     // runtime (non-benchmark) mode runs on the exact FP32 kernels
     if (!rt().bench.enabled && (mode == AERO_LBM_REALTIME_SOLVER_AUTO || mode == AERO_LBM_REALTIME_SOLVER_CLASSIC_D3Q27))
         return Storage::F32Exact;
+    if (mode == AERO_LBM_REALTIME_SOLVER_AUTO && rt().bench.preset == AERO_LBM_BENCHMARK_PRESET_NONE &&
+        fp16_inplace_env())
+        return Storage::F16;
     switch (mode) {
         case AERO_LBM_REALTIME_SOLVER_D3Q27_FP16_INPLACE_EXPERIMENTAL: return Storage::F16;
         case AERO_LBM_REALTIME_SOLVER_D3Q27_BF16_INPLACE: return Storage::BF16;

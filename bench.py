@@ -279,6 +279,53 @@ def run_batched(steps, warmup, per_brick_steps=40):
     return out
 
 
+def world_run(edge, grid, steps, batching, publish_every=4, stride=4):
+    """One brick world of grid^3 bricks of edge^3 cells in FP16 in-place mode (seeded
+    solid boxes and inflow per brick), stepped through the drop-in C ABI one world step
+    at a time with every brick's packed atlas copied every publish_every steps.
+    Returns (wall seconds of the timed steps, cells)."""
+    import aero_lbm as A
+    import workloads as W
+    rng = np.random.default_rng(7)
+    w = A.BrickWorld(world=21, size=edge, mode=A.SOLVER_D3Q27_FP16_INPLACE)
+    coords = [(x, y, z) for x in range(grid) for y in range(grid) for z in range(grid)]
+    try:
+        w.batching(batching)
+        w.hints(coords, exact=True)
+        for i, c in enumerate(coords):
+            ob = W.solid_boxes(edge, edge, edge, 0.08, seed=100 + i, edge=(2, 6), keep_clear_x=2)
+            u, a = rng.uniform(0.02, 0.06), rng.uniform(0, 2 * np.pi)
+            flow = np.tile(np.array([u * np.cos(a), u * np.sin(a), 0.0, 0.0], np.float32), edge ** 3)
+            w.upload_static(c, ob.reshape(-1))
+            w.upload_dynamic(c, flow)
+        w.step(2)  # packet steps: contexts start and become cached (untimed)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            w.step(1)
+            if (k + 1) % publish_every == 0:
+                for c in coords:
+                    w.atlas(c, stride)
+        wall = time.perf_counter() - t0
+        return wall, len(coords) * edge ** 3
+    finally:
+        w.release()
+
+
+def run_world(steps=200, edge=32, grid=4):
+    """The brick-world runtime as the mod drives it (aero_lbm_simulation_step_brick_world_runtime
+    plus a packed-atlas copy per brick every 4 steps), 4 x 4 x 4 bricks of 32^3 FP16:
+    cached bricks in one batched launch set per world step against the per-brick
+    cached-step calls the reference's runtime makes."""
+    wb, cells = world_run(edge, grid, steps, True)
+    wp, _ = world_run(edge, grid, max(8, steps // 4), False)
+    sp = max(8, steps // 4)
+    out = {"workload": f"brick_world_{grid ** 3}x{edge}^3_fp16_step_publish4", "bricks": grid ** 3, "cells": cells,
+           "steps": steps, "glups": cells * steps / wb / 1e9, "ms_per_step": wb * 1e3 / steps,
+           "per_brick_glups": cells * sp / wp / 1e9, "per_brick_ms_per_step": wp * 1e3 / sp}
+    out["speedup_vs_per_brick"] = out["glups"] / out["per_brick_glups"]
+    return out
+
+
 def run_adjoint(reps=3):
     """Inflow/heat fit of one 64^3 brick (FP32): 200-step forward with checkpoints
     every 20 steps, backward over the 64x64x3 inflow profile and the 16^3 heat
@@ -342,6 +389,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-bricks line")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-fit line")
+    ap.add_argument("--no-world", action="store_true", help="skip the brick-world runtime line")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) five times, "
@@ -424,6 +472,11 @@ def main():
         print(f"bench: batched {line['batched']['glups']:.2f} GLUPS vs per-brick "
               f"{line['batched']['per_brick_glups']:.2f} ({line['batched']['speedup_vs_per_brick']:.2f}x); "
               f"{line['batched']['launch']}", file=sys.stderr, flush=True)
+    if not args.no_world:
+        print("bench: brick world 64 x 32^3 fp16", file=sys.stderr, flush=True)
+        line["world"] = run_world()
+        print(f"bench: world {line['world']['glups']:.2f} GLUPS batched vs {line['world']['per_brick_glups']:.2f} "
+              f"per brick ({line['world']['speedup_vs_per_brick']:.2f}x)", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         print("bench: adjoint fit 64^3 x 200 steps", file=sys.stderr, flush=True)
         line["adjoint"] = run_adjoint()

@@ -66,6 +66,36 @@ def config1(steps=1000, n=32):
     return out
 
 
+def config1_horizons(horizons=(50, 100, 200, 400), n=32):
+    """Config 1 (32^3) through the FP32 and FP64 oracles, probes at short horizons: the
+    flow (nu 1e-3, tau ~0.503) amplifies every rounding difference, so after 1000
+    steps the FP32 oracle is ~2e-2 relative away from the FP64 one and only a
+    bit-exact path meets a 1e-5 relative bar there; at these horizons the two
+    oracles still agree, and a faster FP32 kernel is held to 1e-5 relative of the
+    FP32 oracle."""
+    cfg, pk, solid = W.config1(n)
+    out = {"name": "config1_horizons", "n": n, "horizons": list(horizons),
+           "desc": "config 1 workload; FP32 and FP64 oracle probes (density and velocity) at each horizon"}
+    idx = probes(n ** 3)
+    out["probe_cells"] = idx
+    for prec in (32, 64):
+        o = pyoracle.Oracle(n, n, n, cfg, precision=prec)
+        done = 0
+        for h in horizons:
+            if done == 0:
+                o.run(pk, h)
+            else:
+                for _ in range(h - done):
+                    o.step(None)
+            done = h
+            out[f"fp{prec}_{h}"] = o.flow().reshape(-1, 4)[idx].astype(float).tolist()
+    for h in horizons:
+        a, b = np.array(out[f"fp32_{h}"]), np.array(out[f"fp64_{h}"])
+        out[f"fp32_vs_fp64_rel_{h}"] = float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)))
+        print(f"config1 horizon {h}: FP32 oracle vs FP64 {out[f'fp32_vs_fp64_rel_{h}']:.3e}", flush=True)
+    return out
+
+
 def config3_small(steps=300, n=32):
     cfg, ov, pk, t, solid = W.config3(n, beta=0.1, cs=0.1)
     out = {"name": "config3_small", "n": n, "steps": steps,
@@ -233,7 +263,7 @@ def config4_batch(steps=500, probes_per_brick=64):
     return out
 
 
-JOBS = {"config1": config1, "config3_small": config3_small, "config3_full": config3_full,
+JOBS = {"config1": config1, "config1_horizons": config1_horizons, "config3_small": config3_small, "config3_full": config3_full,
         "config3_horizons": config3_horizons, "config2_meso": config2_meso, "config4_batch": config4_batch}
 
 if __name__ == "__main__":

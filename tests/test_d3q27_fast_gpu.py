@@ -286,3 +286,55 @@ def test_config3_1000_steps_report(lib):
         log(f"  {name}: max velocity error / U_in = {errs[name]:.3e} ({ms:.1f} ms for {steps - 1} steps)")
     log(f"  FP32 oracle vs FP64 oracle: {oracle32:.3e}")
     assert errs["fp32"] <= 2.0 * oracle32
+
+
+def flow_errors(got, ref, u_in):
+    """velocity difference over the inflow speed, density difference over the density"""
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
+    return (float(np.max(np.abs(got[:, :3] - ref[:, :3]))) / u_in,
+            float(np.max(np.abs(got[:, 3] - ref[:, 3]) / (1.0 + ref[:, 3]))))
+
+
+@pytest.mark.parametrize("mode", [A.SOLVER_CLASSIC_D3Q27, A.SOLVER_D3Q27_FP32_INPLACE])
+def test_fast_fp32_config1_vs_fp32_oracle(lib, mode):
+    """The fast FP32 kernels (A-B: the bench's FP32 line; A-A FP32) on config 1 (32^3,
+    15% solid boxes, SGS at the reference constant) against the FP32 oracle: density
+    and velocity within 1e-5 relative (velocity relative to the inflow speed) at
+    50, 100 and 200 steps (tests/golden/config1_horizons.json), where the fast kernels
+    differ from the FP32 oracle by as much as the FP32 oracle differs from the FP64
+    one (4-8e-6). This low-viscosity flow (tau ~0.503) amplifies rounding
+    differences: at 400 steps the fast A-B kernel is 3.5e-5 of the inflow speed from
+    the FP32 oracle, and by 1000 steps the FP32 oracle itself is 5.8e-3 from the FP64
+    one, so only the bit-exact kernel (test_exact_config1_full_1000_steps) meets 1e-5
+    there; the 400- and 1000-step distances are reported beside the oracle's."""
+    gold = json.load(open(os.path.join(HERE, "golden", "config1_horizons.json")))
+    n = gold["n"]
+    cfg, pk, solid = W.config1(n)
+    idx = gold["probe_cells"]
+    ctx = make_ctx(lib, (n, n, n), cfg, W.reference_overrides(A.KERNEL_FAST), mode, key=33)
+    ctx.step(pk)
+    done = 1
+    for h in gold["horizons"]:
+        for _ in range(h - done):
+            ctx.step(None)
+        done = h
+        got = ctx.flow().reshape(-1, 4)[idx]
+        eu, er = flow_errors(got, gold[f"fp32_{h}"], 0.05)
+        ou, orr = flow_errors(gold[f"fp32_{h}"], gold[f"fp64_{h}"], 0.05)
+        log(f"  mode {mode} step {h}: velocity {eu:.2e}, density {er:.2e} of the FP32 oracle "
+            f"(FP32 vs FP64 oracle: {ou:.2e}, {orr:.2e})")
+        if h <= 200:
+            assert ou <= 1e-5 and orr <= 1e-5
+            assert eu <= 1e-5 and er <= 1e-5, h
+        else:
+            assert eu <= 1e-3 and er <= 1e-5, h
+    full = json.load(open(os.path.join(HERE, "golden", "config1.json")))
+    for _ in range(full["steps"] - done):
+        ctx.step(None)
+    got = ctx.flow().reshape(-1, 4)[full["fp32"]["probe_cells"]]
+    eu, er = flow_errors(got, full["fp64"]["probe_flow"], 0.05)
+    ou, orr = flow_errors(full["fp32"]["probe_flow"], full["fp64"]["probe_flow"], 0.05)
+    log(f"  mode {mode} step 1000 vs FP64 oracle: velocity {eu:.2e}, density {er:.2e} "
+        f"(FP32 oracle: {ou:.2e}, {orr:.2e})")
+    assert np.all(np.isfinite(got))
+    ctx.release()
