@@ -486,6 +486,76 @@ bool batch_step_contexts(const std::vector<Context*>& ctxs) {
     return true;
 }
 
+bool publish_contexts_atlas(const std::vector<long long>& keys, int stride, float vscale, std::vector<int16_t>& out,
+                            std::vector<size_t>& offsets) {
+    std::lock_guard<std::mutex> lk(rt().mu);
+    clear_error();
+    if (stride <= 0 || keys.empty()) {
+        set_error("packed atlas publish: invalid arguments");
+        return false;
+    }
+    const int nm = (int)keys.size();
+    std::vector<BatchBrick> host((size_t)nm);
+    std::vector<uint16_t> owners;
+    offsets.assign((size_t)nm, 0);
+    size_t values = 0;
+    for (int i = 0; i < nm; ++i) {
+        Context* c = find_context(keys[(size_t)i]);
+        if (!c || !c->initialized || !c->params_valid || !(storage_16bit(c->storage) || c->storage == Storage::F32AA)) {
+            set_error("packed atlas publish: context missing or not in an in-place storage");
+            return false;
+        }
+        const int sx = (c->nx + stride - 1) / stride, sy = (c->ny + stride - 1) / stride,
+                  sz = (c->nz + stride - 1) / stride;
+        const size_t n = (size_t)sx * sy * sz;
+        BatchBrick& e = host[(size_t)i];
+        fill_entry(e, *c, c->store_cur ^ 1, c->t_cur);  // the face store and layout of the last step
+        e.publish_layout = c->aa_layout;
+        e.publish_stride = stride;
+        e.publish_scale = vscale;
+        e.publish_begin = (int)owners.size();
+        owners.insert(owners.end(), (n + 255) / 256, (uint16_t)i);
+        offsets[(size_t)i] = values;
+        values += 4 * n;
+        if (i > 0 && kind_of(c->storage) != kind_of(find_context(keys[0])->storage)) {
+            set_error("packed atlas publish: contexts of one call share a storage");
+            return false;
+        }
+    }
+    if (owners.size() > 0x7fffffffu || nm > 65535) {
+        set_error("packed atlas publish: too many bricks");
+        return false;
+    }
+    hipStream_t st = rt().stream;
+    BatchBrick* table = nullptr;
+    uint16_t* own = nullptr;
+    int16_t* dev = nullptr;
+    out.resize(values);
+    bool ok = hip_ok(hipMallocAsync((void**)&table, host.size() * sizeof(BatchBrick), st), "hipMallocAsync") &&
+              hip_ok(hipMallocAsync((void**)&own, owners.size() * sizeof(uint16_t), st), "hipMallocAsync") &&
+              hip_ok(hipMallocAsync((void**)&dev, values * sizeof(int16_t), st), "hipMallocAsync");
+    if (ok) {
+        for (int i = 0; i < nm; ++i) host[(size_t)i].publish_out = dev + offsets[(size_t)i];
+        ok = hip_ok(hipMemcpyAsync(table, host.data(), host.size() * sizeof(BatchBrick), hipMemcpyHostToDevice, st),
+                    "upload publish table") &&
+             hip_ok(hipMemcpyAsync(own, owners.data(), owners.size() * sizeof(uint16_t), hipMemcpyHostToDevice, st),
+                    "upload publish owners");
+    }
+    if (ok) {
+        BatchGrids g{};
+        g.publish_owner = own;
+        g.publish_blocks = (int)owners.size();
+        launch_batch_publish(kind_of(find_context(keys[0])->storage), table, g, st);
+        ok = hip_ok(hipGetLastError(), "publish") &&
+             hip_ok(hipMemcpyAsync(out.data(), dev, values * sizeof(int16_t), hipMemcpyDeviceToHost, st),
+                    "atlas readback");
+    }
+    if (table) (void)hipFreeAsync(table, st);
+    if (own) (void)hipFreeAsync(own, st);
+    if (dev) (void)hipFreeAsync(dev, st);
+    return hip_ok(hipStreamSynchronize(st), "sync") && ok;
+}
+
 bool publish_context_atlas(long long key, int stride, float vscale, int16_t* out, int value_count) {
     std::lock_guard<std::mutex> lk(rt().mu);
     clear_error();

@@ -94,6 +94,16 @@ struct Brick {
 using BrickMap = std::unordered_map<Coord, Brick, CoordHash>;
 using CoordSet = std::unordered_set<Coord, CoordHash>;
 
+// Packed atlases of a world's in-place bricks, published together on the device at
+// the first request after a world step and then handed out brick by brick.
+struct AtlasCache {
+    uint64_t epoch = ~0ull;
+    int stride = 0;
+    float vscale = 0.0f;
+    std::unordered_map<Coord, std::pair<long long, size_t>, CoordHash> index;  // context key, offset in data
+    std::vector<int16_t> data;
+};
+
 struct World {
     int size = 0;
     float dx = 1.0f, dt = 0.05f;
@@ -107,6 +117,7 @@ struct World {
     BrickMap bricks;
     CoordSet closure;
     std::vector<AeroLbmWorldDelta> pending;
+    AtlasCache atlas_cache;
 };
 struct PendingUpload {
     long long world = 0;
@@ -1436,10 +1447,42 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_simulation_copy_brick_world_packed_
     Brick& b = it->second;
     const float vscale = w->dt > 0.0f ? w->dx / w->dt : 1.0f;
     // an in-place context publishes on the device: int16 quantisation there, a quarter
-    // of the float atlas read back
-    if (b.context != 0 && experimental(*w) && aero::context_batchable(b.context) &&
-        aero::publish_context_atlas(b.context, stride, vscale, out, value_count))
-        return 1;
+    // of the float atlas read back. With batching, the first request after a world
+    // step publishes every in-place brick of the world in one launch and one readback.
+    if (b.context != 0 && experimental(*w) && aero::context_batchable(b.context)) {
+        AtlasCache& ac = w->atlas_cache;
+        const Coord at{bx, by, bz};
+        auto cached = [&]() -> const int16_t* {
+            if (ac.epoch != w->epoch || ac.stride != stride || ac.vscale != vscale) return nullptr;
+            auto hit = ac.index.find(at);
+            return (hit == ac.index.end() || hit->second.first != b.context) ? nullptr : ac.data.data() + hit->second.second;
+        };
+        const int16_t* src = cached();
+        if (!src && w->batching) {
+            std::vector<long long> keys;
+            std::vector<Coord> coords;
+            for (const auto& e : w->bricks)
+                if (e.second.context != 0 && aero::context_batchable(e.second.context)) {
+                    keys.push_back(e.second.context);
+                    coords.push_back(e.first);
+                }
+            std::vector<size_t> offsets;
+            ac.index.clear();
+            ac.epoch = ~0ull;
+            if (aero::publish_contexts_atlas(keys, stride, vscale, ac.data, offsets)) {
+                for (size_t i = 0; i < keys.size(); ++i) ac.index[coords[i]] = {keys[i], offsets[i]};
+                ac.epoch = w->epoch;
+                ac.stride = stride;
+                ac.vscale = vscale;
+                src = cached();
+            }
+        }
+        if (src) {
+            std::copy_n(src, (size_t)value_count, out);
+            return 1;
+        }
+        if (aero::publish_context_atlas(b.context, stride, vscale, out, value_count)) return 1;
+    }
     if (b.context != 0) {
         std::vector<float> atlas((size_t)acells * kFlowCh);
         if (!aero_lbm_extract_flow_atlas_rect(size, size, size, b.context, stride, atlas.data(), (int)atlas.size())) {

@@ -181,5 +181,9 @@ bool batch_step_contexts(const std::vector<Context*>& ctxs);
 // int16 packed flow atlas of one in-place context, quantised on the device (velocity
 // x vscale over +-5.6 m/s, pressure over +-0.03), copied to out (4 values per sample)
 bool publish_context_atlas(long long key, int stride, float vscale, int16_t* out, int value_count);
+// the same for many contexts in one launch and one readback: out holds each
+// context's atlas at offsets[i] (4 values per sample); all share one storage
+bool publish_contexts_atlas(const std::vector<long long>& keys, int stride, float vscale, std::vector<int16_t>& out,
+                            std::vector<size_t>& offsets);
 
 }  // namespace aero

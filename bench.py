@@ -309,14 +309,17 @@ def world_run(edge, grid, steps, batching, publish_every=4, stride=4):
             w.upload_static(c, ob.reshape(-1))
             w.upload_dynamic(c, flow)
         w.step(2)  # packet steps: contexts start and become cached (untimed)
+        t_pub = 0.0
         t0 = time.perf_counter()
         for k in range(steps):
             w.step(1)
             if (k + 1) % publish_every == 0:
+                t1 = time.perf_counter()
                 for c in coords:
                     w.atlas(c, stride)
+                t_pub += time.perf_counter() - t1
         wall = time.perf_counter() - t0
-        return wall, len(coords) * edge ** 3
+        return wall, len(coords) * edge ** 3, t_pub
     finally:
         w.release()
 
@@ -326,12 +329,14 @@ def run_world(steps=200, edge=32, grid=4):
     plus a packed-atlas copy per brick every 4 steps), 4 x 4 x 4 bricks of 32^3 FP16:
     cached bricks in one batched launch set per world step against the per-brick
     cached-step calls the reference's runtime makes."""
-    wb, cells = world_run(edge, grid, steps, True)
-    wp, _ = world_run(edge, grid, max(8, steps // 4), False)
+    wb, cells, pb = world_run(edge, grid, steps, True)
     sp = max(8, steps // 4)
+    wp, _, pp = world_run(edge, grid, sp, False)
     out = {"workload": f"brick_world_{grid ** 3}x{edge}^3_fp16_step_publish4", "bricks": grid ** 3, "cells": cells,
            "steps": steps, "glups": cells * steps / wb / 1e9, "ms_per_step": wb * 1e3 / steps,
-           "per_brick_glups": cells * sp / wp / 1e9, "per_brick_ms_per_step": wp * 1e3 / sp}
+           "publish_ms_per_step": pb * 1e3 / steps,
+           "per_brick_glups": cells * sp / wp / 1e9, "per_brick_ms_per_step": wp * 1e3 / sp,
+           "per_brick_publish_ms_per_step": pp * 1e3 / sp}
     out["speedup_vs_per_brick"] = out["glups"] / out["per_brick_glups"]
     return out
 
