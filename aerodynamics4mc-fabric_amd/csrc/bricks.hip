@@ -9,6 +9,7 @@
 // reference's bridge calls its solver; every per-cell field it hands to the device
 // travels once per (re)start, not per step.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -112,6 +113,7 @@ struct World {
     bool batching = true;  // cached in-place bricks step in shared launches
     uint64_t epoch = 0;
     uint64_t batched_steps = 0;  // world steps whose cached bricks took one batched launch set
+    double host_ms_total = 0, host_ms_prepare = 0, host_ms_batch = 0;  // host time inside world steps
     uint64_t cached_steps = 0, packet_steps = 0, dynamic_upload_resets = 0;
     uint64_t patch_attempts = 0, patch_steps = 0, patch_failures = 0, patch_cells = 0;
     BrickMap bricks;
@@ -848,6 +850,9 @@ void apply_pending(Service& svc, long long key, World& w) {
 // one or more world steps (ref: 1004-1171)
 bool step_world(Service& svc, long long key, World& w, int steps) {
     if (steps <= 0) return false;
+    using clk = std::chrono::steady_clock;
+    const auto t_world = clk::now();
+    auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
     if (!ensure_l2(svc, w.size)) return false;
     bool all_ok = true;
     apply_pending(svc, key, w);
@@ -881,7 +886,12 @@ bool step_world(Service& svc, long long key, World& w, int steps) {
             Brick& b = e.second;
             if (!solver_active(w, b) || !b.dyn) continue;
             if (compact_ready(w, b) && !needs_seed.count(e.first)) {
-                if (w.batching && defer_cached(w, b, batch)) continue;
+                if (w.batching) {
+                    const auto t0 = clk::now();
+                    const bool deferred = defer_cached(w, b, batch);
+                    w.host_ms_prepare += ms_since(t0);
+                    if (deferred) continue;
+                }
                 if (!step_actual(svc, w, e.first, b, *b.dyn)) all_ok = false;
                 continue;
             }
@@ -917,7 +927,10 @@ bool step_world(Service& svc, long long key, World& w, int steps) {
             b.pending_reinit = b.geometry_dirty = b.forcing_dirty = false;
         }
         if (!batch.empty()) {
-            if (batch_step_contexts(batch)) {
+            const auto t0 = clk::now();
+            const bool stepped = batch_step_contexts(batch);
+            w.host_ms_batch += ms_since(t0);
+            if (stepped) {
                 w.batched_steps++;
             } else {
                 set_err(std::string("step_brick_world_runtime: batched step failed: ") + aero_lbm_last_error());
@@ -927,6 +940,7 @@ bool step_world(Service& svc, long long key, World& w, int steps) {
         recompute_active(w);
         prune(w);
     }
+    w.host_ms_total += ms_since(t_world);
     return all_ok;
 }
 
@@ -1564,6 +1578,7 @@ extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info() {
     size_t worlds = 0, bricks = 0, active = 0, contexts = 0, compact = 0, atlases = 0;
     uint64_t cached = 0, packet = 0, resets = 0, p_try = 0, p_ok = 0, p_fail = 0, p_cells = 0, epoch = 0;
     uint64_t batched = 0;
+    double h_total = 0, h_prep = 0, h_batch = 0;
     int mode = -1;
     bool mixed = false;
     if (g_service) {
@@ -1581,6 +1596,9 @@ extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info() {
             }
             cached += w.cached_steps;
             batched += w.batched_steps;
+            h_total += w.host_ms_total;
+            h_prep += w.host_ms_prepare;
+            h_batch += w.host_ms_batch;
             packet += w.packet_steps;
             resets += w.dynamic_upload_resets;
             p_try += w.patch_attempts;
@@ -1595,7 +1613,8 @@ extern "C" AERO_LBM_CAPI_EXPORT const char* aero_lbm_simulation_runtime_info() {
            " bricks=" + std::to_string(bricks) + " active=" + std::to_string(active) +
            " contexts=" + std::to_string(contexts) + " cached_contexts=" + std::to_string(compact) +
            " atlases=" + std::to_string(atlases) + " cached_steps=" + std::to_string(cached) +
-           " batched_world_steps=" + std::to_string(batched) +
+           " batched_world_steps=" + std::to_string(batched) + " world_host_ms=" + std::to_string(h_total) +
+           " world_host_ms_prepare=" + std::to_string(h_prep) + " world_host_ms_batch=" + std::to_string(h_batch) +
            " packet_steps=" + std::to_string(packet) + " dynamic_upload_resets=" + std::to_string(resets) +
            " patch_attempts=" + std::to_string(p_try) + " patch_steps=" + std::to_string(p_ok) +
            " patch_failures=" + std::to_string(p_fail) + " patch_cells=" + std::to_string(p_cells) +

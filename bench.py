@@ -319,6 +319,8 @@ def world_run(edge, grid, steps, batching, publish_every=4, stride=4):
                     w.atlas(c, stride)
                 t_pub += time.perf_counter() - t1
         wall = time.perf_counter() - t0
+        info = A.load().aero_lbm_simulation_runtime_info().decode()
+        print(f"bench: world batching={batching}: {info}", file=sys.stderr, flush=True)
         return wall, len(coords) * edge ** 3, t_pub
     finally:
         w.release()
