@@ -847,6 +847,14 @@ void apply_pending(Service& svc, long long key, World& w) {
     }
 }
 
+// ref: 1411-1439 (the result is not checked, as there)
+bool exchange_face_halo(const World& w, const Brick& a, const Brick& b, int ox, int oy, int oz) {
+    if (w.size <= 2 * kGhostLayers || a.context == 0 || b.context == 0 || !dyn_valid(w, a.dyn.get()) ||
+        !dyn_valid(w, b.dyn.get()) || !nonzero_flow(*a.dyn) || !nonzero_flow(*b.dyn))
+        return false;
+    return aero_lbm_exchange_halo_layers_rect(w.size, w.size, w.size, kGhostLayers, a.context, b.context, ox, oy, oz) != 0;
+}
+
 // one or more world steps (ref: 1004-1171)
 bool step_world(Service& svc, long long key, World& w, int steps) {
     if (steps <= 0) return false;
@@ -876,9 +884,21 @@ bool step_world(Service& svc, long long key, World& w, int steps) {
         Snapshots snaps;
         for (const auto& e : w.bricks)
             if (solver_active(w, e.second) && dyn_valid(w, e.second.dyn.get())) snaps.emplace(e.first, e.second.dyn);
-        // the population halo between face-adjacent contexts (ref: 1056-1089) is left
-        // out: the in-place contexts keep their populations in the 16-bit A-A
-        // buffer, which the reference's exchange (jni.cpp:7817-7818) does not touch.
+        // the population halo between face-adjacent contexts (ref: 1056-1089): the
+        // classic mode's exact FP32 contexts exchange one ghost layer with each
+        // positive-face neighbour; the in-place contexts keep their populations in the
+        // A-A buffer, which the reference's exchange (jni.cpp:7817-7818) does not
+        // touch, so in-place worlds skip the calls
+        if (!experimental(w))
+            for (auto& e : w.bricks) {
+                Brick& b = e.second;
+                if (!solver_active(w, b) || b.context == 0) continue;
+                for (int f = 1; f < 6; f += 2) {
+                    auto it = w.bricks.find(Coord{e.first.x + kNbr[f][0], e.first.y + kNbr[f][1], e.first.z + kNbr[f][2]});
+                    if (it == w.bricks.end() || !solver_active(w, it->second) || it->second.context == 0) continue;
+                    exchange_face_halo(w, b, it->second, kNbr[f][0], kNbr[f][1], kNbr[f][2]);
+                }
+            }
         // Cached in-place bricks are independent within a world step, so their steps
         // are collected and launched together after the loop.
         std::vector<Context*> batch;

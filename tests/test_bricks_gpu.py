@@ -341,6 +341,62 @@ def test_classic_world_runtime_mode_matches_oracle_chain(oracle_mod, lib):
         w.release()
 
 
+def test_classic_world_two_bricks_exchange_halo_like_oracle_chain(oracle_mod, lib):
+    """Two face-adjacent bricks along x in the classic solver mode. The first world
+    step bootstraps each brick's ghost face from its neighbour's uploaded state
+    (simulation_bridge.cpp:1354-1409); before every later step the two exact FP32
+    contexts exchange one ghost layer (aero_lbm_exchange_halo_layers_rect, 1056-1089,
+    1411-1439). The same chain on two FP32 oracle contexts (the oracle's halo exchange,
+    pinned in test_oracle_context_ops.py) gives bit-identical dynamic states."""
+    n = 16
+    obs = {(0, 0, 0): brick_static(n, 31), (1, 0, 0): brick_static(n, 32)}
+    rng = np.random.default_rng(6)
+    zeros8, zeros16, zf = np.zeros(n ** 3, np.uint8), np.zeros(n ** 3, np.uint16), np.zeros(n ** 3, np.float32)
+
+    def constrain(ob, f, a):
+        f, a, s = f.copy(), a.copy(), zf.copy()
+        B.apply_constraints(ob, zeros16, zeros8, zf, f, a, s)
+        return f, a
+
+    up = {}
+    for c, u in (((0, 0, 0), 0.03), ((1, 0, 0), 0.045)):
+        fl = np.tile(np.array([u, 0.0, 0.005, 0.0], np.float32), n ** 3)
+        fl[0::4] += rng.uniform(-2e-3, 2e-3, n ** 3).astype(np.float32)
+        up[c] = (fl, (0.02 * rng.uniform(-1.0, 1.0, n ** 3)).astype(np.float32))
+    lib.aero_lbm_benchmark_reset_config()  # classic bricks step in runtime mode
+    w = A.BrickWorld(world=15, size=n, mode=A.SOLVER_CLASSIC_D3Q27)
+    try:
+        w.hints(list(obs), exact=True)
+        for c in obs:
+            w.upload_static(c, obs[c])
+            w.upload_dynamic(c, *up[c])
+        cfg = A.default_config()
+        orc = {c: oracle_mod.Oracle(n, n, n, cfg, precision=32) for c in obs}
+        st = {c: constrain(obs[c], *up[c]) for c in obs}  # the snapshots the bootstrap copies from
+        a_, b_ = (0, 0, 0), (1, 0, 0)
+        boot = {}
+        for c, nb, positive in ((a_, b_, True), (b_, a_, False)):
+            f, a = st[c][0].copy(), st[c][1].copy()
+            B.copy_ghost(n, st[nb][0], st[nb][1], zf, 0, positive, f, a, zf.copy())
+            boot[c] = constrain(obs[c], f, a)
+        st = boot
+        for step in range(1, 7):
+            log(f"classic two-brick world step {step}")
+            w.step(1)
+            if step > 1:
+                orc[a_].exchange_halo(orc[b_], (1, 0, 0), 1)
+            for c in obs:
+                pk = B.build_packet(n, obs[c], zeros8, zf, st[c][0], st[c][1], [True] * 6)
+                orc[c].step(pk.reshape(-1))
+            st = {c: constrain(obs[c], orc[c].flow(), orc[c].temperature()) for c in obs}
+            for c in obs:
+                got_f, got_a, _ = w.dynamic(c)
+                assert np.array_equal(got_f, st[c][0]), (step, c, float(np.max(np.abs(got_f - st[c][0]))))
+                assert np.array_equal(got_a, st[c][1]), (step, c)
+    finally:
+        w.release()
+
+
 def run_world(bricks, n, mode, steps, batching):
     """Steps a world of the given bricks {coord: (obstacle, flow)}; returns the
     dynamic state and stride-1/stride-4 atlases of every brick and the runtime info."""
