@@ -220,6 +220,14 @@ PROTOTYPES = {
     "aero_lbm_get_solver_overrides": (c_int, [POINTER(SolverOverrides)]),
     "aero_lbm_run_cached_steps_timed": (c_int, [c_int, c_int, c_int, c_longlong, c_int, _F]),
     "aero_lbm_get_conservation_rect": (c_int, [c_int, c_int, c_int, c_longlong, POINTER(c_double)]),
+    "aero_lbm_analysis_max_compressed_size_3d": (c_longlong, [c_int, c_int, c_int]),
+    "aero_lbm_analysis_compress_float_grid_3d": (c_longlong, [_F, c_int, c_int, c_int, c_double, _U8, c_longlong]),
+    "aero_lbm_analysis_decompress_float_grid_3d": (c_int, [_U8, c_longlong, c_int, c_int, c_int, _F]),
+    "aero_lbm_analysis_compress_float_grids_3d_device": (c_int, [c_int, POINTER(c_void_p), _I, POINTER(c_double), c_void_p,
+                                                                 POINTER(c_longlong), POINTER(c_longlong),
+                                                                 POINTER(c_longlong)]),
+    "aero_lbm_analysis_compress_grids_timed": (c_int, [c_int, POINTER(c_void_p), _I, POINTER(c_double), c_void_p,
+                                                       POINTER(c_longlong), POINTER(c_longlong), c_int, _F]),
     "aero_lbm_mesoscale_default_config": (None, [POINTER(MesoscaleConfig)]),
     "aero_lbm_mesoscale_derive_transport": (c_int, [POINTER(MesoscaleConfig), POINTER(MesoscaleTransport)]),
     "aero_lbm_mesoscale_create_context": (c_int, [POINTER(MesoscaleConfig), POINTER(c_longlong)]),
@@ -295,6 +303,13 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise AeroLbmError(f"libaero_lbm.so not built at {path}; run __graft_entry__.build()")
+    # One HIP runtime per process: when torch is installed, load the one it ships
+    # before the library binds libamdhip64, so torch tensors and the library can share
+    # the device (torch fails to initialise under a runtime loaded before it).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in PROTOTYPES.items():
         fn = getattr(lib, name, None)
@@ -614,3 +629,63 @@ class Adjoint:
 
     def release(self):
         self.lib.aero_lbm_adjoint_release(self.h)
+
+
+# ---- analysis codec (zfp streams of 3D float grids; native/src/aero_lbm_analysis_codec.cpp)
+
+def compress_float_grid_3d(values: np.ndarray, tolerance: float, lib=None) -> bytes | None:
+    """values: float32 grid of shape (nz, ny, nx) (x fastest). The stream's bytes, or
+    None on error (reason in last_error()), as the reference's compressFloatGrid3d."""
+    lib = lib or load()
+    v = np.ascontiguousarray(values, np.float32)
+    nz, ny, nx = v.shape
+    cap = lib.aero_lbm_analysis_max_compressed_size_3d(nx, ny, nz)
+    if cap <= 0:
+        return None
+    out = np.zeros(cap, np.uint8)
+    n = lib.aero_lbm_analysis_compress_float_grid_3d(fptr(v), nx, ny, nz, float(tolerance), _ptr(out, _U8), cap)
+    return out[:n].tobytes() if n > 0 else None
+
+
+def decompress_float_grid_3d(data: bytes, nx: int, ny: int, nz: int, lib=None) -> np.ndarray | None:
+    """The grid of shape (nz, ny, nx) a stream decodes to, or None on error."""
+    lib = lib or load()
+    buf = np.frombuffer(data, np.uint8).copy()
+    out = np.zeros((nz, ny, nx), np.float32)
+    ok = lib.aero_lbm_analysis_decompress_float_grid_3d(_ptr(buf, _U8), len(buf), nx, ny, nz, fptr(out))
+    return out if ok else None
+
+
+def _device_grid_args(grids, tolerances):
+    n = len(grids)
+    vals = (c_void_p * n)(*[g.data_ptr() for g in grids])
+    dims = np.array([[g.shape[2], g.shape[1], g.shape[0]] for g in grids], np.int32).reshape(-1)
+    tols = (c_double * n)(*[float(t) for t in tolerances])
+    return n, vals, dims, tols
+
+
+def compress_device_grids(grids, tolerances, lib=None, repeats: int = 0):
+    """Device float32 tensors of shape (nz, ny, nx), coded in one set of launches.
+    Returns the list of streams (bytes); with repeats > 0 also the device time per
+    repeat in ms (the launches run that many times back to back)."""
+    import torch
+    lib = lib or load()
+    n, vals, dims, tols = _device_grid_args(grids, tolerances)
+    caps = [lib.aero_lbm_analysis_max_compressed_size_3d(int(dims[3 * i]), int(dims[3 * i + 1]), int(dims[3 * i + 2]))
+            for i in range(n)]
+    offs = np.cumsum([0] + caps[:-1]).astype(np.int64)
+    out = torch.zeros(int(sum(caps)), dtype=torch.uint8, device=grids[0].device)
+    c_offs = (c_longlong * n)(*offs.tolist())
+    c_caps = (c_longlong * n)(*caps)
+    sizes = (c_longlong * n)()
+    torch.cuda.synchronize()
+    check(lib.aero_lbm_analysis_compress_float_grids_3d_device(n, vals, _ptr(dims, _I), tols, c_void_p(out.data_ptr()),
+                                                               c_offs, c_caps, sizes), "compress_float_grids_3d_device")
+    host = out.cpu().numpy()
+    streams = [host[offs[i]:offs[i] + sizes[i]].tobytes() for i in range(n)]
+    if repeats <= 0:
+        return streams
+    ms = c_float(0)
+    check(lib.aero_lbm_analysis_compress_grids_timed(n, vals, _ptr(dims, _I), tols, c_void_p(out.data_ptr()), c_offs,
+                                                     c_caps, repeats, ctypes.byref(ms)), "compress_grids_timed")
+    return streams, ms.value

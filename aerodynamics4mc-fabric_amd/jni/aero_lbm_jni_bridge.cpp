@@ -309,6 +309,47 @@ JNIEXPORT jboolean JNICALL NSB(nativePollPackedFlowAtlas)(JNIEnv* env, jclass, j
                                                            (int)n));
 }
 
+// ---- analysis codec (ref: simulation_bridge.cpp:8880-8927 and 8929-8974): a zfp
+// stream of a 3D float grid as a new byte[], null on any failure; decompression into
+// a float[] of exactly nx * ny * nz values, written back only on success
+JNIEXPORT jbyteArray JNICALL NSB(nativeCompressFloatGrid3d)(JNIEnv* env, jclass, jfloatArray values, jint nx, jint ny,
+                                                           jint nz, jdouble tolerance) {
+    const long long cells = (long long)nx * ny * nz;
+    if (!values || nx <= 0 || ny <= 0 || nz <= 0 || len(env, values) != cells) return nullptr;
+    const long long cap = aero_lbm_analysis_max_compressed_size_3d(nx, ny, nz);
+    if (cap <= 0) return nullptr;
+    std::vector<uint8_t> bytes((std::size_t)cap);
+    long long n = 0;
+    {
+        Floats v(env, values);
+        if (!v) return nullptr;
+        n = aero_lbm_analysis_compress_float_grid_3d(v.data, nx, ny, nz, (double)tolerance, bytes.data(), cap);
+    }
+    if (n <= 0 || n > 0x7fffffffll) return nullptr;
+    jbyteArray out = env->NewByteArray((jsize)n);
+    if (!out) return nullptr;
+    env->SetByteArrayRegion(out, 0, (jsize)n, reinterpret_cast<const jbyte*>(bytes.data()));
+    return out;
+}
+
+JNIEXPORT jboolean JNICALL NSB(nativeDecompressFloatGrid3d)(JNIEnv* env, jclass, jbyteArray compressed, jint nx,
+                                                           jint ny, jint nz, jfloatArray out_values) {
+    if (!compressed || !out_values) return JNI_FALSE;
+    const long long cells = (long long)nx * ny * nz;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || len(env, out_values) != cells) return JNI_FALSE;
+    const jsize size = env->GetArrayLength(compressed);
+    std::vector<uint8_t> bytes((std::size_t)size);
+    if (size > 0) {
+        env->GetByteArrayRegion(compressed, 0, size, reinterpret_cast<jbyte*>(bytes.data()));
+        if (env->ExceptionCheck()) return JNI_FALSE;
+    }
+    Floats o(env, out_values);
+    if (!o) return JNI_FALSE;
+    const int ok = aero_lbm_analysis_decompress_float_grid_3d(bytes.data(), (long long)size, nx, ny, nz, o.data);
+    o.write_back = ok != 0;
+    return jbool(ok);
+}
+
 // ref: simulation_bridge.cpp:8976-8979
 JNIEXPORT jstring JNICALL NSB(nativeRuntimeInfo)(JNIEnv* env, jclass) {
     const char* s = aero_lbm_simulation_runtime_info();

@@ -7,6 +7,8 @@
 #ifndef AERO_LBM_JNI_H
 #define AERO_LBM_JNI_H
 
+#include <stddef.h>
+
 #include <stdarg.h>
 #include <stdint.h>
 
@@ -88,7 +90,9 @@ struct JNINativeInterface_ {
     jstring(JNICALL* NewStringUTF)(JNIEnv* env, const char* utf);                   /* 167 */
     void* slots_168_170[3];                                                          /* 168 GetStringUTFLength */
     jsize(JNICALL* GetArrayLength)(JNIEnv* env, jarray array);                       /* 171 */
-    void* slots_172_178[7];                                                          /* 172 NewObjectArray */
+    void* slots_172_175[4];                                                          /* 172 NewObjectArray */
+    jbyteArray(JNICALL* NewByteArray)(JNIEnv* env, jsize len);                       /* 176 */
+    void* slots_177_178[2];                                                          /* 177 NewCharArray */
     jintArray(JNICALL* NewIntArray)(JNIEnv* env, jsize len);                         /* 179 */
     void* slot_180;                                                                  /* 180 NewLongArray */
     jfloatArray(JNICALL* NewFloatArray)(JNIEnv* env, jsize len);                     /* 181 */
@@ -106,12 +110,29 @@ struct JNINativeInterface_ {
     void(JNICALL* ReleaseIntArrayElements)(JNIEnv* env, jintArray a, jint* e, jint mode);       /* 195 */
     void* slot_196;                                                                             /* 196 Long */
     void(JNICALL* ReleaseFloatArrayElements)(JNIEnv* env, jfloatArray a, jfloat* e, jint mode); /* 197 */
-    void* slots_198_210[13];                                                         /* 198 ReleaseDouble... */
+    void* slots_198_199[2];                                                          /* 198 ReleaseDouble... */
+    void(JNICALL* GetByteArrayRegion)(JNIEnv* env, jbyteArray a, jsize start, jsize len, jbyte* buf);       /* 200 */
+    void* slots_201_207[7];                                                                                 /* 201 */
+    void(JNICALL* SetByteArrayRegion)(JNIEnv* env, jbyteArray a, jsize start, jsize len, const jbyte* buf); /* 208 */
+    void* slots_209_210[2];                                                                                 /* 209 */
     void(JNICALL* SetIntArrayRegion)(JNIEnv* env, jintArray a, jsize start, jsize len, const jint* buf);     /* 211 */
     void* slot_212;                                                                                         /* 212 */
     void(JNICALL* SetFloatArrayRegion)(JNIEnv* env, jfloatArray a, jsize start, jsize len, const jfloat* buf); /* 213 */
-    void* slots_214_234[21]; /* 214 SetDoubleArrayRegion ... 234 IsVirtualThread */
+    void* slots_214_227[14]; /* 214 SetDoubleArrayRegion ... 227 DeleteWeakGlobalRef */
+    jboolean(JNICALL* ExceptionCheck)(JNIEnv* env); /* 228 */
+    void* slots_229_234[6];  /* 229 NewDirectByteBuffer ... 234 IsVirtualThread */
 };
+
+#ifdef __cplusplus
+/* slot positions the bridge relies on */
+static_assert(offsetof(JNINativeInterface_, NewStringUTF) == 167 * sizeof(void*), "JNI table layout");
+static_assert(offsetof(JNINativeInterface_, NewByteArray) == 176 * sizeof(void*), "JNI table layout");
+static_assert(offsetof(JNINativeInterface_, GetByteArrayRegion) == 200 * sizeof(void*), "JNI table layout");
+static_assert(offsetof(JNINativeInterface_, SetByteArrayRegion) == 208 * sizeof(void*), "JNI table layout");
+static_assert(offsetof(JNINativeInterface_, SetFloatArrayRegion) == 213 * sizeof(void*), "JNI table layout");
+static_assert(offsetof(JNINativeInterface_, ExceptionCheck) == 228 * sizeof(void*), "JNI table layout");
+static_assert(sizeof(JNINativeInterface_) == 235 * sizeof(void*), "JNI table layout");
+#endif
 
 #ifdef __cplusplus
 /* C++ view of JNIEnv: the table pointer plus inline forwarding members, as the
@@ -122,6 +143,12 @@ struct JNIEnv_ {
     jsize GetArrayLength(jarray a) { return functions->GetArrayLength(this, a); }
     jintArray NewIntArray(jsize n) { return functions->NewIntArray(this, n); }
     jfloatArray NewFloatArray(jsize n) { return functions->NewFloatArray(this, n); }
+    jbyteArray NewByteArray(jsize n) { return functions->NewByteArray(this, n); }
+    void GetByteArrayRegion(jbyteArray a, jsize s, jsize n, jbyte* b) { functions->GetByteArrayRegion(this, a, s, n, b); }
+    void SetByteArrayRegion(jbyteArray a, jsize s, jsize n, const jbyte* b) {
+        functions->SetByteArrayRegion(this, a, s, n, b);
+    }
+    jboolean ExceptionCheck() { return functions->ExceptionCheck(this); }
     jbyte* GetByteArrayElements(jbyteArray a, jboolean* c) { return functions->GetByteArrayElements(this, a, c); }
     jshort* GetShortArrayElements(jshortArray a, jboolean* c) { return functions->GetShortArrayElements(this, a, c); }
     jint* GetIntArrayElements(jintArray a, jboolean* c) { return functions->GetIntArrayElements(this, a, c); }

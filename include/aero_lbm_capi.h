@@ -471,6 +471,40 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_run_steps_timed(
     long long context_key, const AeroLbmMesoscaleConfig* config, const float* forcing, int steps, float* out_ms);
 AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_get_state(long long context_key, float* out_state);
 
+/* ---- analysis codec (ref: native/src/aero_lbm_analysis_codec.hpp:11-29, the
+ * functions behind nativeCompressFloatGrid3d / nativeDecompressFloatGrid3d,
+ * native/src/aero_lbm_simulation_bridge.cpp:8880-8976). Streams are zfp 1.0.1's
+ * (fixed-accuracy mode with the tolerance given, full header), byte for byte the
+ * reference's; a grid is values[x + nx * (y + ny * z)]. Errors in
+ * aero_lbm_last_error(): the reference's strings invalid_input, invalid_tolerance,
+ * zfp_write_header_failed (an extent above 65536), zfp_read_header_failed,
+ * zfp_header_mismatch, zfp_decompress_failed (a stream shorter than its blocks).
+ *
+ * max_compressed_size_3d: bytes a stream of that shape can need (zfp's
+ *   zfp_stream_maximum_size); 0 for an invalid shape.
+ * compress_float_grid_3d: host values in, host bytes out (capacity bytes at out);
+ *   returns the stream's size in bytes, 0 on error. The encoder runs on the GPU.
+ * decompress_float_grid_3d: host stream in, host grid out; 1 on success. Runs on
+ *   the host (a zfp stream has no block index; blocks decode in sequence). */
+AERO_LBM_CAPI_EXPORT long long aero_lbm_analysis_max_compressed_size_3d(int nx, int ny, int nz);
+AERO_LBM_CAPI_EXPORT long long aero_lbm_analysis_compress_float_grid_3d(
+    const float* values, int nx, int ny, int nz, double tolerance, uint8_t* out, long long capacity);
+AERO_LBM_CAPI_EXPORT int aero_lbm_analysis_decompress_float_grid_3d(
+    const uint8_t* compressed, long long compressed_size, int nx, int ny, int nz, float* out_values);
+/* extension (no counterpart in the reference): `count` grids (at most 64) already in
+ * device memory, coded in one set of launches. values[i]: device pointer of grid i;
+ * dims[3i..3i+2]: its nx, ny, nz; out: device buffer (8-byte aligned) that receives
+ * stream i at byte out_offsets[i] (a multiple of 8) with out_capacities[i] bytes
+ * room; out_sizes[i]: the stream's size. Host arrays except values[i] and out. */
+AERO_LBM_CAPI_EXPORT int aero_lbm_analysis_compress_float_grids_3d_device(
+    int count, const float* const* values, const int* dims, const double* tolerances, uint8_t* out,
+    const long long* out_offsets, const long long* out_capacities, long long* out_sizes);
+/* extension: the same launches `repeats` times back to back on the library's stream;
+ * *out_ms = device time per repeat (HIP events), for measurement */
+AERO_LBM_CAPI_EXPORT int aero_lbm_analysis_compress_grids_timed(
+    int count, const float* const* values, const int* dims, const double* tolerances, uint8_t* out,
+    const long long* out_offsets, const long long* out_capacities, int repeats, float* out_ms);
+
 /* ---- benchmark configuration (ref: native/include/aero_lbm_capi.h:698-703) ---- */
 AERO_LBM_CAPI_EXPORT void aero_lbm_benchmark_default_config(AeroLbmBenchmarkConfig* out_config);
 AERO_LBM_CAPI_EXPORT int aero_lbm_benchmark_default_preset_config(int preset, AeroLbmBenchmarkConfig* out_config);

@@ -2,10 +2,11 @@
 Python (test infrastructure).
 
 It builds a JNIEnv: a pointer to a 235-slot interface function table whose slots at
-the specification's indices (NewStringUTF 167, GetArrayLength 171, NewIntArray 179,
-NewFloatArray 181, Get<Byte|Short|Int|Float>ArrayElements 184/186/187/189,
-Release<...>ArrayElements 192/194/195/197, Set<Int|Float>ArrayRegion 211/213) are
-Python callbacks. Java arrays are numpy arrays behind opaque handles; Get*Elements
+the specification's indices (NewStringUTF 167, GetArrayLength 171, NewByteArray 176,
+NewIntArray 179, NewFloatArray 181, Get<Byte|Short|Int|Float>ArrayElements
+184/186/187/189, Release<...>ArrayElements 192/194/195/197, GetByteArrayRegion 200,
+Set<Byte|Int|Float>ArrayRegion 208/211/213, ExceptionCheck 228) are Python
+callbacks. Java arrays are numpy arrays behind opaque handles; Get*Elements
 hands out a copy, and Release*Elements copies it back only for modes 0 and
 JNI_COMMIT, as a copying JVM does, so a test sees exactly which arrays a native
 method wrote back. Every release is recorded with its mode."""
@@ -67,6 +68,18 @@ class FakeJVM:
         def get_array_length(env, a):
             return int(self.objects[a].size)
 
+        @self._slot(176, c_void_p, c_void_p, c_int32)
+        def new_byte_array(env, n):
+            return self._new(np.zeros(n, np.int8))
+
+        @self._slot(200, None, c_void_p, c_void_p, c_int32, c_int32, c_void_p)
+        def get_byte_region(env, a, start, n, buf):
+            ctypes.memmove(buf, self.objects[a][start:start + n].ctypes.data, n)
+
+        @self._slot(228, c_uint8, c_void_p)
+        def exception_check(env):
+            return 0
+
         @self._slot(179, c_void_p, c_void_p, c_int32)
         def new_int_array(env, n):
             return self._new(np.zeros(n, np.int32))
@@ -84,6 +97,7 @@ class FakeJVM:
                 self.objects[a][start:start + n] = src
             return fn
 
+        self._slot(208, None, c_void_p, c_void_p, c_int32, c_int32, c_void_p)(set_region(np.int8))
         self._slot(211, None, c_void_p, c_void_p, c_int32, c_int32, c_void_p)(set_region(np.int32))
         self._slot(213, None, c_void_p, c_void_p, c_int32, c_int32, c_void_p)(set_region(np.float32))
 
@@ -130,6 +144,8 @@ SIGNATURES = {
     NSB + "nativeGetBrickWorldRuntimeStatus": (c_uint8, [c_int64, c_int64, J]),
     NSB + "nativeSetPackedFlowAtlas": (c_uint8, [c_int64, c_int64, J]),
     NSB + "nativePollPackedFlowAtlas": (c_uint8, [c_int64, c_int64, J]),
+    NSB + "nativeCompressFloatGrid3d": (J, [J, c_int32, c_int32, c_int32, ctypes.c_double]),
+    NSB + "nativeDecompressFloatGrid3d": (c_uint8, [J, c_int32, c_int32, c_int32, J]),
     NSB + "nativeRuntimeInfo": (J, []),
     NSB + "nativeLastError": (J, []),
     MNB + "nativeDeriveTransport": (J, [c_int32] * 3 + [ctypes.c_float] * 5),
