@@ -16,10 +16,11 @@
 //   4. embedded coding of each plane (zfp/src/template/encode.c:178-207): the first n
 //      bits verbatim, then group tests, each run "1, zeros, 1" written as one field
 //      whose length comes from a count of trailing zeros, into the block's slot of a
-//      scratch buffer; the workgroup scans its 256 block lengths.
-// A second kernel scans the per-workgroup totals of each grid, a third assembles
-// every 64-bit word of the output from the slots of the blocks that overlap it (no
-// atomics and no zero fill: each word is written once).
+//      scratch buffer; the wavefront scans its 64 block lengths.
+// A second kernel sums each workgroup's earlier tiles for its bit offset, and every
+// block then writes the output words whose first bit it holds, taking the rest of a
+// word from the blocks that follow (no atomics and no zero fill: each word is
+// written once).
 //
 // Decoder (host): zfp's stream carries no block index, so block boundaries are only
 // known by decoding every block before them; the decoder is a serial C++ pass
@@ -42,9 +43,9 @@
 namespace aero {
 namespace codec {
 
-constexpr int kTile = 256;        // blocks per encoder workgroup
 constexpr int kMaxGrids = 64;     // grids per batched call
 constexpr int kMinExp = -1074;
+constexpr int kGroup = 4;         // tiles per assembly workgroup
 
 __device__ __forceinline__ int find_grid(const GridDesc* grids, int ngrids, int tile) {
     int g = 0;
@@ -52,141 +53,76 @@ __device__ __forceinline__ int find_grid(const GridDesc* grids, int ngrids, int 
     return g;
 }
 
-// exclusive scan over a 256-thread workgroup; returns the prefix, total in *total
-__device__ __forceinline__ uint32_t wg_scan_256(uint32_t v, uint32_t* lds4, uint32_t* total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = v;
+// one wavefront codes the 64 blocks of a tile into their scratch slots (word i of
+// lane j's slot at scratch[(tile * kSlotWords + i) * 64 + j], so a wavefront's stores
+// of the same word index are contiguous), then scans their lengths
+__global__ void __launch_bounds__(kTile) encode_kernel(const GridDesc* __restrict__ grids, int ngrids,
+                                                       unsigned long long* __restrict__ scratch,
+                                                       uint32_t* __restrict__ len, uint32_t* __restrict__ lpre,
+                                                       uint32_t* __restrict__ tsum) {
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const GridDesc& G = grids[find_grid(grids, ngrids, tile)];
+    const int b = (tile - G.tile_begin) * kTile + lane;
+    uint32_t bits = 0;
+    if (b < G.nblocks) bits = encode_block(G, b, scratch + (size_t)tile * kSlotWords * kTile + lane, kTile);
+    uint32_t inc = bits;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t t = __shfl_up(inc, o, 64);
         if (lane >= o) inc += t;
     }
-    if (lane == 63) lds4[wave] = inc;
-    __syncthreads();
-    uint32_t before = 0;
-#pragma unroll
-    for (int w = 0; w < 4; w++)
-        if (w < wave) before += lds4[w];
-    *total = lds4[0] + lds4[1] + lds4[2] + lds4[3];
-    return before + inc - v;
-}
-
-__global__ void __launch_bounds__(kTile) encode_kernel(const GridDesc* __restrict__ grids, int ngrids,
-                                                       unsigned long long* __restrict__ scratch,
-                                                       uint32_t* __restrict__ len, uint32_t* __restrict__ lpre,
-                                                       uint32_t* __restrict__ tsum) {
-    __shared__ uint32_t lds4[4];
-    const int tile = blockIdx.x;
-    const GridDesc& G = grids[find_grid(grids, ngrids, tile)];
-    const int b = (tile - G.tile_begin) * kTile + (int)threadIdx.x;
-    const size_t slot = (size_t)tile * kTile + threadIdx.x;
-    uint32_t bits = 0;
-    if (b < G.nblocks) bits = encode_block(G, b, scratch + slot * kSlotWords);
-    uint32_t total;
-    const uint32_t pre = wg_scan_256(bits, lds4, &total);
+    const size_t slot = (size_t)tile * kTile + lane;
     len[slot] = bits;
-    lpre[slot] = pre;
-    if (threadIdx.x == 0) tsum[tile] = total;
+    lpre[slot] = inc - bits;
+    if (lane == 63) tsum[tile] = inc;
 }
 
-// per grid (one workgroup each): exclusive prefix of its tiles' bit counts, and the
-// stream's total length in bits
-__global__ void __launch_bounds__(kTile) scan_kernel(const GridDesc* __restrict__ grids,
-                                                     const uint32_t* __restrict__ tsum,
-                                                     unsigned long long* __restrict__ tpre,
-                                                     unsigned long long* __restrict__ gbits) {
-    __shared__ unsigned long long lds4[4];
-    const GridDesc& G = grids[blockIdx.x];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    unsigned long long carry = 0;
-    for (int base = 0; base < G.ntiles; base += kTile) {
-        const int t = base + (int)threadIdx.x;
-        const unsigned long long v = t < G.ntiles ? tsum[G.tile_begin + t] : 0ull;
-        unsigned long long inc = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned long long u = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += u;
-        }
-        if (lane == 63) lds4[wave] = inc;
-        __syncthreads();
-        unsigned long long before = 0;
-#pragma unroll
-        for (int w = 0; w < 4; w++)
-            if (w < wave) before += lds4[w];
-        const unsigned long long chunk = lds4[0] + lds4[1] + lds4[2] + lds4[3];
-        if (t < G.ntiles) tpre[G.tile_begin + t] = carry + before + inc - v;
-        carry += chunk;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) gbits[blockIdx.x] = (unsigned long long)G.header_bits + carry;
-}
-
-// every output word of grid blockIdx.y, from the header and the scratch slots of
-// the blocks it overlaps
-__global__ void __launch_bounds__(256) assemble_kernel(const GridDesc* __restrict__ grids,
-                                                       const unsigned long long* __restrict__ scratch,
-                                                       const uint32_t* __restrict__ len,
-                                                       const uint32_t* __restrict__ lpre,
-                                                       const unsigned long long* __restrict__ tpre,
-                                                       const unsigned long long* __restrict__ gbits,
-                                                       unsigned long long* __restrict__ out) {
+// workgroup = kGroup tiles of one grid (blockIdx.y): the bit offset of its first tile
+// is the sum of the grid's earlier tiles; then every block writes the output words
+// whose first bit it holds (block 0 also the header's), each word once
+__global__ void __launch_bounds__(64 * kGroup) assemble_kernel(const GridDesc* __restrict__ grids,
+                                                               const unsigned long long* __restrict__ scratch,
+                                                               const uint32_t* __restrict__ len,
+                                                               const uint32_t* __restrict__ lpre,
+                                                               const uint32_t* __restrict__ tsum,
+                                                               unsigned long long* __restrict__ gbits,
+                                                               unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[kGroup];
+    __shared__ unsigned long long wave_base[kGroup];
     const GridDesc& G = grids[blockIdx.y];
-    const unsigned long long total = gbits[blockIdx.y];
-    unsigned long long nwords = (total + 63ull) >> 6;
-    if (nwords > G.cap_words) nwords = G.cap_words;  // the host reports the shortfall
-    const unsigned long long H = (unsigned long long)G.header_bits;
-    const unsigned long long data_bits = total - H;
-    for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
-         w += (unsigned long long)gridDim.x * blockDim.x) {
-        unsigned long long p = w << 6;
-        unsigned long long word = 0;
-        uint32_t filled = 0;
-        if (p < H) {
-            const uint32_t take = (uint32_t)min(64ull, H - p);
-            word = read_bits(G.header, p, take);
-            filled = take;
-            p += take;
+    const int t0 = (int)blockIdx.x * kGroup;
+    if (t0 >= G.ntiles) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long s = 0;
+    for (int i = threadIdx.x; i < t0; i += blockDim.x) s += tsum[G.tile_begin + i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) part[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long base = part[0] + part[1] + part[2] + part[3];
+        for (int w = 0; w < kGroup; w++) {
+            wave_base[w] = base;
+            if (t0 + w < G.ntiles) base += tsum[G.tile_begin + t0 + w];
         }
-        if (filled < 64 && p < total) {
-            unsigned long long qpos = p - H;
-            // tile holding bit qpos: the last tile whose prefix is <= qpos
-            int lo = 0, hi = G.ntiles - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (tpre[G.tile_begin + mid] <= qpos) lo = mid; else hi = mid - 1;
-            }
-            int t = lo;
-            unsigned long long tbase = tpre[G.tile_begin + t];
-            int valid = min(kTile, G.nblocks - t * kTile);
-            size_t slot0 = (size_t)(G.tile_begin + t) * kTile;
-            int a = 0, c = valid - 1;
-            const uint32_t inner = (uint32_t)(qpos - tbase);
-            while (a < c) {
-                const int mid = (a + c + 1) >> 1;
-                if (lpre[slot0 + mid] <= inner) a = mid; else c = mid - 1;
-            }
-            int j = a;
-            while (filled < 64 && qpos < data_bits) {
-                const size_t slot = slot0 + j;
-                const unsigned long long start = tbase + lpre[slot];
-                const uint32_t within = (uint32_t)(qpos - start);
-                const uint32_t take = min(64u - filled, len[slot] - within);
-                const unsigned long long bits = read_bits(scratch + slot * kSlotWords, within, take);
-                word |= filled ? bits << filled : bits;
-                filled += take;
-                qpos += take;
-                if (++j == valid) {  // next tile of the grid
-                    if (++t >= G.ntiles) break;
-                    tbase = tpre[G.tile_begin + t];
-                    valid = min(kTile, G.nblocks - t * kTile);
-                    slot0 = (size_t)(G.tile_begin + t) * kTile;
-                    j = 0;
-                }
-            }
-        }
-        out[G.out_word + w] = word;
+        if (t0 + kGroup >= G.ntiles) gbits[blockIdx.y] = (unsigned long long)G.header_bits + base;
     }
+    __syncthreads();
+    const int t = t0 + wave;
+    if (t >= G.ntiles) return;
+    const int b = t * kTile + lane;
+    if (b >= G.nblocks) return;
+    const size_t slot = (size_t)(G.tile_begin + t) * kTile + lane;
+    const unsigned long long H = (unsigned long long)G.header_bits;
+    const unsigned long long start = H + wave_base[wave] + lpre[slot];
+    const unsigned long long end = start + len[slot];
+    if (b == 0) {
+        for (unsigned long long w = 0; (w << 6) < H && w < G.cap_words; w++)
+            out[G.out_word + w] = gather_word(G, scratch, len, w << 6, 0, 0, H);
+    }
+    for (unsigned long long w = (start + 63) >> 6; (w << 6) < end && w < G.cap_words; w++)
+        out[G.out_word + w] = gather_word(G, scratch, len, w << 6, t, lane, start);
 }
 
 // ---- host side
@@ -200,8 +136,6 @@ struct Workspace {
     size_t lpre_cap = 0;
     uint32_t* tsum = nullptr;  // per tile
     size_t tsum_cap = 0;
-    unsigned long long* tpre = nullptr;  // per tile
-    size_t tpre_cap = 0;
     unsigned long long* gbits = nullptr;  // per grid
     size_t gbits_cap = 0;
     GridDesc* grids = nullptr;
@@ -350,20 +284,17 @@ bool run(const std::vector<GridRequest>& req, unsigned long long* out, std::vect
         !grow(W.len, W.len_cap, slots, "hipMalloc(codec lengths)") ||
         !grow(W.lpre, W.lpre_cap, slots, "hipMalloc(codec prefixes)") ||
         !grow(W.tsum, W.tsum_cap, (size_t)tiles, "hipMalloc(codec tile sums)") ||
-        !grow(W.tpre, W.tpre_cap, (size_t)tiles, "hipMalloc(codec tile prefixes)") ||
         !grow(W.gbits, W.gbits_cap, (size_t)kMaxGrids, "hipMalloc(codec totals)") ||
         !grow(W.grids, W.grids_cap, (size_t)kMaxGrids, "hipMalloc(codec grids)"))
         return false;
     hipStream_t st = rt().stream;
     if (!hip_ok(hipMemcpyAsync(W.grids, g.data(), n * sizeof(GridDesc), hipMemcpyHostToDevice, st), "hipMemcpyAsync(codec grids)"))
         return false;
-    unsigned long long max_words = 0;
-    for (const GridDesc& d : g) max_words = std::max(max_words, (unsigned long long)d.cap_words);
-    const unsigned blocks_x = (unsigned)std::min<unsigned long long>((max_words + 255) / 256, 4096ull);
+    int max_groups = 1;
+    for (const GridDesc& d : g) max_groups = std::max(max_groups, (d.ntiles + kGroup - 1) / kGroup);
     auto launch = [&]() {
         encode_kernel<<<tiles, kTile, 0, st>>>(W.grids, n, W.scratch, W.len, W.lpre, W.tsum);
-        scan_kernel<<<n, kTile, 0, st>>>(W.grids, W.tsum, W.tpre, W.gbits);
-        assemble_kernel<<<dim3(std::max(blocks_x, 1u), n), 256, 0, st>>>(W.grids, W.scratch, W.len, W.lpre, W.tpre, W.gbits, out);
+        assemble_kernel<<<dim3(max_groups, n), 64 * kGroup, 0, st>>>(W.grids, W.scratch, W.len, W.lpre, W.tsum, W.gbits, out);
     };
     if (timed_repeats > 0) {
         if (!W.ev0 && (!hip_ok(hipEventCreate(&W.ev0), "hipEventCreate") || !hip_ok(hipEventCreate(&W.ev1), "hipEventCreate")))

@@ -15,6 +15,7 @@ namespace aero {
 namespace codec {
 
 constexpr int kSlotWords = 36;    // scratch words per block: a block codes to at most 2120 bits
+constexpr int kTile = 64;         // blocks per encoder workgroup (one wavefront)
 constexpr int kMaxBlockBits = 2120;  // 1 + 8 + 63 + 64 * 32 (zfp/src/zfp.c:744-784 for floats)
 
 struct GridDesc {
@@ -69,9 +70,39 @@ AERO_HD inline void transpose32(uint32_t* a) {
     }
 }
 
-// appends bit fields, least significant first, to a block's scratch slot
+// rows 16..31 of the same transpose only (the 16 most significant bit planes), in
+// about half the operations: the first stage moves every row's high half into rows
+// 16..31, the remaining stages stay inside them
+AERO_HD inline void transpose32_high(uint32_t* a) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) a[r + 16] ^= ((a[r] >> 16) ^ a[r + 16]) & 0x0000ffffu;
+#pragma unroll
+    for (int j = 8, s = 1; j; j >>= 1, s++) {
+        const uint32_t m = s == 1 ? 0x00ff00ffu : s == 2 ? 0x0f0f0f0fu : s == 3 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+        for (int r = 16; r < 32; r++) {
+            if (r & j) continue;
+            const uint32_t t = ((a[r] >> j) ^ a[r + j]) & m;
+            a[r + j] ^= t;
+            a[r] ^= t << j;
+        }
+    }
+}
+
+// true if the condition holds for any lane of the wavefront (the lane itself on the host)
+AERO_HD inline bool any_lane(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __any(v);
+#else
+    return v;
+#endif
+}
+
+// appends bit fields, least significant first, to a block's scratch slot, whose
+// words lie `stride` words apart
 struct BitSink {
     unsigned long long* dst;
+    unsigned stride;
     unsigned long long acc;
     uint32_t fill;   // bits held in acc, < 64
     uint32_t words;  // words stored
@@ -80,12 +111,11 @@ struct BitSink {
         acc |= v << fill;
         const uint32_t f = fill + n;
         if (f >= 64) {
-            dst[words++] = acc;
+            dst[(size_t)words * stride] = acc;
+            words++;
             acc = fill ? v >> (64 - fill) : 0ull;
-            fill = f - 64;
-        } else {
-            fill = f;
         }
+        fill = f & 63u;
     }
 };
 
@@ -96,8 +126,9 @@ AERO_HD inline int cvt_trunc(float f) {
     return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : INT_MIN;
 }
 
-// codes block b of grid G into dst; returns its length in bits
-AERO_HD inline uint32_t encode_block(const GridDesc& G, int b, unsigned long long* dst) {
+// codes block b of grid G into the slot at dst (word i at dst[i * stride]); returns
+// its length in bits
+AERO_HD inline uint32_t encode_block(const GridDesc& G, int b, unsigned long long* dst, unsigned stride) {
     const int bxi = b % G.bx;
     const int rest = b / G.bx;
     const int byi = rest % G.by;
@@ -145,13 +176,24 @@ AERO_HD inline uint32_t encode_block(const GridDesc& G, int b, unsigned long lon
             }
         }
     }
-    // largest magnitude (a NaN never compares greater, zfp/src/template/encodef.c:31-40)
+    // largest magnitude as float bits (a NaN never compares greater,
+    // zfp/src/template/encodef.c:31-40); blocks without NaN or infinity skip the
+    // NaN filter here and the range checks of the conversion below
     uint32_t mx = 0;
 #pragma unroll
     for (int i = 0; i < 64; i++) {
-        uint32_t a = __builtin_bit_cast(uint32_t, v[i]) & 0x7fffffffu;
-        a = a > 0x7f800000u ? 0u : a;
+        const uint32_t a = __builtin_bit_cast(uint32_t, v[i]) & 0x7fffffffu;
         mx = mx > a ? mx : a;
+    }
+    const bool special = any_lane(mx >= 0x7f800000u);
+    if (special) {
+        mx = 0;
+#pragma unroll
+        for (int i = 0; i < 64; i++) {
+            uint32_t a = __builtin_bit_cast(uint32_t, v[i]) & 0x7fffffffu;
+            a = a > 0x7f800000u ? 0u : a;
+            mx = mx > a ? mx : a;
+        }
     }
     // its frexp exponent, clamped for subnormals; glibc leaves 0 for an infinity
     // (zfp/src/template/encodef.c:11-28)
@@ -162,13 +204,18 @@ AERO_HD inline uint32_t encode_block(const GridDesc& G, int b, unsigned long lon
         dst[0] = 0ull;  // one zero bit: every value is zero at this tolerance
         return 1u;
     }
-    BitSink sink{dst, 0ull, 0u, 0u};
+    BitSink sink{dst, stride, 0ull, 0u, 0u};
     sink.put(2ull * (unsigned long long)(emax + 127) + 1ull, 9);
     const int se = 30 - emax;  // zfp/src/template/encodef.c:43-58: s = 2^(30 - emax) in float
     const float s = se <= 127 ? __builtin_bit_cast(float, (uint32_t)(se + 127) << 23) : __builtin_bit_cast(float, 0x7f800000u);
     int q[64];
+    if (special || any_lane(se > 127)) {
 #pragma unroll
-    for (int i = 0; i < 64; i++) q[i] = cvt_trunc(s * v[i]);
+        for (int i = 0; i < 64; i++) q[i] = cvt_trunc(s * v[i]);
+    } else {  // finite values and a finite scale: |s * v| < 2^30
+#pragma unroll
+        for (int i = 0; i < 64; i++) q[i] = (int)(s * v[i]);
+    }
     // zfp/src/template/encode3.c:35-51: along x, then y, then z
 #pragma unroll
     for (int z = 0; z < 4; z++)
@@ -190,10 +237,15 @@ AERO_HD inline uint32_t encode_block(const GridDesc& G, int b, unsigned long lon
         lo[i] = ((uint32_t)q[perm3(i)] + 0xaaaaaaaau) ^ 0xaaaaaaaau;
         hi[i] = ((uint32_t)q[perm3(i + 32)] + 0xaaaaaaaau) ^ 0xaaaaaaaau;
     }
-    transpose32(lo);
-    transpose32(hi);
     // zfp/src/template/encode.c:178-207, planes 31 down to 32 - prec
     const int kmin = prec < 32 ? 32 - prec : 0;
+    if (any_lane(kmin < 16)) {
+        transpose32(lo);
+        transpose32(hi);
+    } else {
+        transpose32_high(lo);
+        transpose32_high(hi);
+    }
     uint32_t n = 0;  // coefficients already significant
 #pragma unroll
     for (int k = 31; k >= 0; k--) {
@@ -201,26 +253,24 @@ AERO_HD inline uint32_t encode_block(const GridDesc& G, int b, unsigned long lon
             const unsigned long long x = (unsigned long long)lo[k] | ((unsigned long long)hi[k] << 32);
             if (n) sink.put(n == 64 ? x : (x & ((1ull << n) - 1ull)), n);
             if (n < 64) {
+                // group tests: "0" ends the plane; otherwise "1", the zeros before the
+                // next one bit and the one itself, whose bit is implied when it is the
+                // last coefficient
                 unsigned long long r = x >> n;
-                while (true) {
-                    if (r == 0ull) {
-                        sink.put(0ull, 1);  // negative group test ends the plane
-                        break;
-                    }
-                    const uint32_t t = (uint32_t)__builtin_ctzll(r);
-                    if (n + t == 63u) {  // the last coefficient's one bit is implied
-                        sink.put(1ull, t + 1);
-                        n = 64;
-                        break;
-                    }
-                    sink.put(1ull | (2ull << t), t + 2);  // group test, t zeros, the one
-                    n += t + 1;
-                    r >>= (t + 1);
+                bool more = true;
+                while (more) {
+                    const uint32_t t = r ? (uint32_t)__builtin_ctzll(r) : 0u;
+                    const bool last = r && n + t == 63u;
+                    more = r && !last;
+                    const unsigned long long field = more ? (1ull | (2ull << t)) : (r ? 1ull : 0ull);
+                    sink.put(field, r ? (last ? t + 1 : t + 2) : 1u);
+                    n = r ? (last ? 64u : n + t + 1) : n;
+                    r = more ? r >> (t + 1) : 0ull;
                 }
             }
         }
     }
-    if (sink.fill) dst[sink.words] = sink.acc;
+    if (sink.fill) dst[(size_t)sink.words * stride] = sink.acc;
     return sink.words * 64u + sink.fill;
 }
 
@@ -231,6 +281,51 @@ AERO_HD inline unsigned long long read_bits(const unsigned long long* w, unsigne
     unsigned long long v = w[i] >> sh;
     if (sh && sh + n > 64) v |= w[i + 1] << (64 - sh);
     return n >= 64 ? v : (v & ((1ull << n) - 1ull));
+}
+
+// n <= 64 bits of slot (tile, lane) from bit p
+AERO_HD inline unsigned long long slot_bits(const unsigned long long* scratch, size_t tile, int lane,
+                                                        uint32_t p, uint32_t n) {
+    const unsigned long long* w = scratch + tile * kSlotWords * kTile + lane;
+    const uint32_t i = p >> 6, sh = p & 63u;
+    unsigned long long v = w[(size_t)i * kTile] >> sh;
+    if (sh && sh + n > 64) v |= w[(size_t)(i + 1) * kTile] << (64 - sh);
+    return n >= 64 ? v : (v & ((1ull << n) - 1ull));
+}
+
+// the 64 stream bits from bit p, which lies in the header (p < H) or in block (t, j)
+// of the grid that starts at bit bstart; zero past the last block
+AERO_HD inline unsigned long long gather_word(const GridDesc& G, const unsigned long long* __restrict__ scratch,
+                                          const uint32_t* __restrict__ len, unsigned long long p, int t, int j,
+                                          unsigned long long bstart) {
+    const unsigned long long H = (unsigned long long)G.header_bits;
+    unsigned long long word = 0;
+    uint32_t filled = 0;
+    if (p < H) {
+        const uint32_t take = (uint32_t)(H - p < 64ull ? H - p : 64ull);
+        word = read_bits(G.header, p, take);
+        filled = take;
+        p += take;
+    }
+    while (filled < 64 && t * kTile + j < G.nblocks) {
+        const size_t tile = (size_t)(G.tile_begin + t);
+        const uint32_t L = len[tile * kTile + j];
+        const unsigned long long within = p - bstart;
+        if (within < L) {
+            const uint32_t take = 64u - filled < L - (uint32_t)within ? 64u - filled : L - (uint32_t)within;
+            const unsigned long long bits = slot_bits(scratch, tile, j, (uint32_t)within, take);
+            word |= filled ? bits << filled : bits;
+            filled += take;
+            p += take;
+        } else {  // on to the next block
+            bstart += L;
+            if (++j == kTile) {
+                j = 0;
+                t++;
+            }
+        }
+    }
+    return word;
 }
 
 }  // namespace codec

@@ -393,6 +393,80 @@ def run_adjoint(reps=3):
                          "reference": "fp64 forward, central differences"}}
 
 
+CODEC_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "codec_zfp_traffic.json")
+
+
+def codec_profile(key):
+    """A field of the committed rocprofv3 summary of the codec's kernels, or None."""
+    try:
+        with open(CODEC_TRAFFIC_FILE) as fh:
+            return json.load(fh)["grid_256"][key]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def run_codec(reps=50):
+    """Analysis codec: zfp 1.0.1 fixed-accuracy streams of 3D float grids, byte for byte
+    the reference's nativeCompressFloatGrid3d. Two workloads: one 256^3 residual
+    velocity grid (a swirl of amplitude 0.05 plus Gaussian noise of 0.003, tolerance
+    0.02) for throughput, and the analysis payload the server codes per region (four
+    64^3 residual grids: vx, vy, vz at tolerance 0.02, pressure at 5e-4). Inputs are
+    resident in device memory; device time per call from HIP events over `reps`
+    back-to-back calls of the batched entry point; the host-memory entry point (the
+    JNI path, PCIe both ways) timed beside it. CPU baseline: the reference codec built
+    from its sources (oracle/_ref) when present, else the oracle, one thread."""
+    import torch
+    import aero_lbm as A
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import codec_cases as CC
+    lib = A.load()
+
+    def timed(grids, tols):
+        dev = [torch.from_numpy(g).cuda() for g in grids]
+        streams, ms = A.compress_device_grids(dev, tols, lib, repeats=reps)
+        t0, k = time.perf_counter(), 0
+        while time.perf_counter() - t0 < 0.5 or k < 3:
+            for g, t in zip(grids, tols):
+                A.compress_float_grid_3d(g, t, lib)
+            k += 1
+        return streams, ms, (time.perf_counter() - t0) / k * 1e3
+
+    big = [CC.grid("velocity", (256, 256, 256), 3)]
+    streams, ms, host_ms = timed(big, [CC.VELOCITY_TOL])
+    values = big[0].size
+    comp = len(streams[0])
+    alg = values * 4 + comp  # read every value once, write the stream once
+    traffic = codec_profile("bytes_per_call")
+    out = {"workload": "zfp_fixed_accuracy_256^3_velocity_residual_tol0.02", "values": values,
+           "compressed_bytes": comp, "bits_per_value": round(8 * comp / values, 3),
+           "device_us_per_call": round(ms * 1e3, 2), "input_gbps": values * 4 / (ms * 1e-3) / 1e9,
+           "host_call_ms": round(host_ms, 3),
+           "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": alg / (ms * 1e-3) / HBM_PEAK, "traffic": traffic,
+                        "algorithmic_bytes": alg,
+                        "note": "encode + assembly launches per call; the encoder is integer-VALU bound"}}
+    payload = [CC.grid(k, (64, 64, 64), 40 + i) for i, k in enumerate(["velocity", "velocity", "velocity", "pressure"])]
+    ptols = [CC.VELOCITY_TOL] * 3 + [CC.PRESSURE_TOL]
+    pstreams, pms, phost = timed(payload, ptols)
+    out["payload_4x64^3"] = {"device_us_per_call": round(pms * 1e3, 2), "host_call_ms": round(phost, 3),
+                             "compressed_bytes": sum(len(x) for x in pstreams)}
+    ref = CC.reference()
+    cpu, kind = (ref, "reference") if ref is not None else (CC.oracle(), "port")
+    t0, k = time.perf_counter(), 0
+    while time.perf_counter() - t0 < 2.0 or k < 2:
+        for g, t in zip(payload, ptols):
+            cpu.compress(g, t)
+        k += 1
+    cpu_ms = (time.perf_counter() - t0) / k * 1e3
+    out["payload_4x64^3"]["cpu_ms"] = round(cpu_ms, 3)
+    out["cpu_baseline"] = {"value": sum(g.size for g in payload) * 4 / (cpu_ms * 1e-3) / 1e9, "unit": "GB/s of input",
+                           "cores": 1, "kind": kind,
+                           "sample": f"{k} codings of the 4 x 64^3 payload on the host, "
+                                     f"{'reference codec with vendored zfp' if kind == 'reference' else 'oracle'}, "
+                                     f"{cpu_ms:.2f} ms each"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +481,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-fit line")
     ap.add_argument("--no-world", action="store_true", help="skip the brick-world runtime line")
+    ap.add_argument("--no-codec", action="store_true", help="skip the analysis-codec line")
+    ap.add_argument("--codec-only", action="store_true", help="only the analysis-codec line")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (read 1 GiB + write 1 GiB) five times, "
@@ -433,6 +509,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.codec_only:
+        print(json.dumps({"codec": run_codec()}), flush=True)
+        return
     import aero_lbm as A
     import workloads as W
     lib = A.load()
@@ -506,6 +585,12 @@ def main():
         print(f"bench: adjoint forward {line['adjoint']['forward_ms']:.1f} ms, backward "
               f"{line['adjoint']['backward_ms']:.1f} ms, fd max rel {line['adjoint']['fd_check']['max_rel']:.2e}",
               file=sys.stderr, flush=True)
+    if not args.no_codec:
+        line["codec"] = run_codec()
+        print(f"bench: codec {line['codec']['device_us_per_call']:.1f} us per 256^3 grid "
+              f"({line['codec']['input_gbps']:.0f} GB/s of input), payload "
+              f"{line['codec']['payload_4x64^3']['device_us_per_call']:.1f} us on the device vs "
+              f"{line['codec']['payload_4x64^3']['cpu_ms']:.2f} ms on the host CPU", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 
