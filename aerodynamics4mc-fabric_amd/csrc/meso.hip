@@ -124,6 +124,26 @@ __global__ void __launch_bounds__(256) prep_kernel(Params P, const float* __rest
     S[kBz * N + i] = clamp_speed(bz / P.vden);
 }
 
+// per-cell link mask: bit q (1..18) when direction q's source cell lies in the
+// domain and is solid, bit 31 when the cell itself is solid; rebuilt with the
+// coefficient planes, so a pull tests one bit instead of loading a flag byte
+constexpr uint32_t kSelfSolid = 1u << 31;
+
+__global__ void __launch_bounds__(256) mask_kernel(Params P, const uint8_t* __restrict__ solid,
+                                                   uint32_t* __restrict__ links) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.cells) return;
+    const int z = i % P.nz, y = (i / P.nz) % P.ny, x = i / (P.nz * P.ny);
+    uint32_t m = solid[i] ? kSelfSolid : 0u;
+#pragma unroll
+    for (int q = 1; q < kQ; ++q) {
+        const int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
+        if (sx >= 0 && sx < P.nx && sy >= 0 && sy < P.ny && sz >= 0 && sz < P.nz && solid[idx(P, sx, sy, sz)])
+            m |= 1u << q;
+    }
+    links[i] = m;
+}
+
 // ---------------------------------------------------------------- seeding (mesoscale.cpp:298-333)
 __global__ void __launch_bounds__(256) seed_kernel(Params P, const float* __restrict__ F, float* __restrict__ f,
                                                    float* __restrict__ g, float* __restrict__ ambient,
@@ -160,15 +180,15 @@ struct Pulled {
 };
 
 __device__ __forceinline__ Pulled pull(const Params& P, const float* __restrict__ pf, const float* __restrict__ pg,
-                                       const float* __restrict__ S, const uint8_t* __restrict__ solid, int i, int x,
-                                       int y, int z, int q, float ea, float bgx, float bgy, float bgz, float btemp) {
+                                       const float* __restrict__ S, uint32_t links, int i, int x, int y, int z, int q,
+                                       float ea, float bgx, float bgy, float bgz, float btemp) {
     const size_t N = (size_t)P.cells;
     const int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
     Pulled v;
     bool bounced = false;
     if (sx >= 0 && sx < P.nx && sy >= 0 && sy < P.ny && sz >= 0 && sz < P.nz) {
         const int src = idx(P, sx, sy, sz);
-        if (solid[src]) {
+        if ((links >> q) & 1u) {
             v.f = pf[opp(q) * N + i];
             v.g = tfeq(q, S[kSurfT * N + src], 0.0f, 0.0f, 0.0f);
             bounced = true;
@@ -223,13 +243,14 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
                                                       float* __restrict__ pg_out, const float* __restrict__ ambient,
                                                       const float* __restrict__ surf_in, const float* __restrict__ deep_in,
                                                       float* __restrict__ surf_out, float* __restrict__ deep_out,
-                                                      const float* __restrict__ S, const uint8_t* __restrict__ solid,
+                                                      const float* __restrict__ S, const uint32_t* __restrict__ links,
                                                       int local_first) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.cells) return;
     const size_t N = (size_t)P.cells;
     const int z = i % P.nz, y = (i / P.nz) % P.ny, x = i / (P.nz * P.ny);
-    if (solid[i]) {
+    const uint32_t lk = links[i];
+    if (lk & kSelfSolid) {
         // solid targets (mesoscale.cpp:448-457); their populations are never read
         surf_out[i] = S[kSurfT * N + i];
         deep_out[i] = S[kDeepT * N + i];
@@ -247,7 +268,7 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
         const float ea = edge_alpha(P, x, y, z);
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const Pulled v = pull(P, pf_in, pg_in, S, solid, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
+            const Pulled v = pull(P, pf_in, pg_in, S, lk, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
             f[q] = v.f;
             g[q] = v.g;
         }
@@ -331,23 +352,33 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
 __global__ void __launch_bounds__(256) macro_kernel(Params P, const float* __restrict__ pf, const float* __restrict__ pg,
                                                     float* __restrict__ ambient, const float* __restrict__ surface,
                                                     const float* __restrict__ deep, const float* __restrict__ S,
-                                                    const uint8_t* __restrict__ solid, float* __restrict__ out) {
+                                                    const uint32_t* __restrict__ links, float* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.cells) return;
     const size_t N = (size_t)P.cells;
     float amb, wx = 0.0f, wy = 0.0f, wz = 0.0f;
-    if (solid[i]) {
+    const uint32_t lk = links[i];
+    if (lk & kSelfSolid) {
         amb = S[kSurfT * N + i];
     } else {
         const int z = i % P.nz, y = (i / P.nz) % P.ny, x = i / (P.nz * P.ny);
         const float ea = edge_alpha(P, x, y, z);
-        const float bgx = S[kBx * N + i], bgy = S[kBy * N + i], bgz = S[kBz * N + i], amb_t = S[kAmbT * N + i];
+        // the background wind and temperature enter a pull only on the open faces
+        // and in the blending band, both within one cell of the side and top faces
+        const bool band = min(min(min(x, z), min(P.nx - 1 - x, P.nz - 1 - z)), P.ny - 1 - y) <= 1;
+        float bgx = 0.0f, bgy = 0.0f, bgz = 0.0f, amb_t = 0.0f;
+        if (band) {
+            bgx = S[kBx * N + i];
+            bgy = S[kBy * N + i];
+            bgz = S[kBz * N + i];
+            amb_t = S[kAmbT * N + i];
+        }
         float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
         amb = 0.0f;
         if (out) {
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {
-                const Pulled v = pull(P, pf, pg, S, solid, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
+                const Pulled v = pull(P, pf, pg, S, lk, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
                 rho += v.f;
                 jx += v.f * (float)cx(q);
                 jy += v.f * (float)cy(q);
@@ -361,7 +392,7 @@ __global__ void __launch_bounds__(256) macro_kernel(Params P, const float* __res
             wz = clampf(uz * P.vscale, -P.max_speed, P.max_speed);
         } else {
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) amb += pull(P, pf, pg, S, solid, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t).g;
+            for (int q = 0; q < kQ; ++q) amb += pull(P, pf, pg, S, lk, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t).g;
         }
     }
     ambient[i] = amb;
@@ -384,11 +415,12 @@ __global__ void __launch_bounds__(256) macro_kernel(Params P, const float* __res
 // the next collide then reads the cell's own values.
 __global__ void __launch_bounds__(256) stream_kernel(Params P, const float* __restrict__ pf, const float* __restrict__ pg,
                                                      float* __restrict__ f_out, float* __restrict__ g_out,
-                                                     const float* __restrict__ S, const uint8_t* __restrict__ solid) {
+                                                     const float* __restrict__ S, const uint32_t* __restrict__ links) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.cells) return;
     const size_t N = (size_t)P.cells;
-    if (solid[i]) {
+    const uint32_t lk = links[i];
+    if (lk & kSelfSolid) {
         const float wt = S[kSurfT * N + i];
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
@@ -402,7 +434,7 @@ __global__ void __launch_bounds__(256) stream_kernel(Params P, const float* __re
     const float bgx = S[kBx * N + i], bgy = S[kBy * N + i], bgz = S[kBz * N + i], amb_t = S[kAmbT * N + i];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
-        const Pulled v = pull(P, pf, pg, S, solid, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
+        const Pulled v = pull(P, pf, pg, S, lk, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
         f_out[q * N + i] = v.f;
         g_out[q * N + i] = v.g;
     }
@@ -425,6 +457,7 @@ struct Context {
     float* deep[2] = {nullptr, nullptr};
     float* stat = nullptr;
     uint8_t* solid = nullptr;
+    uint32_t* links = nullptr;
     float* forcing_dev = nullptr;
     float* out_dev = nullptr;
     std::vector<void*> allocations;
@@ -448,7 +481,7 @@ struct Context {
                  alloc((void**)&surface[k], N * 4) && alloc((void**)&deep[k], N * 4);
         }
         ok = ok && alloc((void**)&ambient, N * 4) && alloc((void**)&stat, kStats * N * 4) &&
-             alloc((void**)&solid, N) && alloc((void**)&forcing_dev, kForcing * N * 4) &&
+             alloc((void**)&solid, N) && alloc((void**)&links, N * 4) && alloc((void**)&forcing_dev, kForcing * N * 4) &&
              alloc((void**)&out_dev, kState * N * 4);
         forcing_host.clear();
         prep_vscale = NAN;
@@ -534,7 +567,7 @@ static bool prepare(Context& c, const Params& P, const float* forcing) {
     if (replan && c.initialized && !c.local_first) {
         // stream the pending post-collision state with the coefficients it belongs to
         hipLaunchKernelGGL(stream_kernel, grid1(c.cells), dim3(256), 0, st, P, c.pf[c.cur], c.pg[c.cur],
-                           c.pf[1 - c.cur], c.pg[1 - c.cur], c.stat, c.solid);
+                           c.pf[1 - c.cur], c.pg[1 - c.cur], c.stat, c.links);
         c.cur ^= 1;
         c.local_first = true;
     }
@@ -548,6 +581,7 @@ static bool prepare(Context& c, const Params& P, const float* forcing) {
     }
     if (replan) {
         hipLaunchKernelGGL(prep_kernel, grid1(c.cells), dim3(256), 0, st, P, c.forcing_dev, c.stat, c.solid);
+        hipLaunchKernelGGL(mask_kernel, grid1(c.cells), dim3(256), 0, st, P, c.solid, c.links);
         c.prep_vscale = P.vscale;
     }
     if (!c.initialized) {
@@ -564,12 +598,12 @@ static void launch_step(Context& c, const Params& P, bool with_output) {
     const int n = c.cur, s = c.scur;
     hipLaunchKernelGGL(collide_kernel, grid1(c.cells), dim3(256), 0, st, P, c.pf[n], c.pg[n], c.pf[1 - n],
                        c.pg[1 - n], c.ambient, c.surface[s], c.deep[s], c.surface[1 - s], c.deep[1 - s], c.stat,
-                       c.solid, c.local_first ? 1 : 0);
+                       c.links, c.local_first ? 1 : 0);
     c.cur ^= 1;
     c.scur ^= 1;
     c.local_first = false;
     hipLaunchKernelGGL(macro_kernel, grid1(c.cells), dim3(256), 0, st, P, c.pf[c.cur], c.pg[c.cur], c.ambient,
-                       c.surface[c.scur], c.deep[c.scur], c.stat, c.solid, with_output ? c.out_dev : nullptr);
+                       c.surface[c.scur], c.deep[c.scur], c.stat, c.links, with_output ? c.out_dev : nullptr);
 }
 
 static Context* bind(long long key, const AeroLbmMesoscaleConfig* config) {
@@ -737,7 +771,7 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_get_state(long long key, float* out_
     hipStream_t st = rt().stream;
     const meso::Params P = meso::make_params(c);
     hipLaunchKernelGGL(meso::macro_kernel, meso::grid1(c.cells), dim3(256), 0, st, P, c.pf[c.cur], c.pg[c.cur],
-                       c.ambient, c.surface[c.scur], c.deep[c.scur], c.stat, c.solid, c.out_dev);
+                       c.ambient, c.surface[c.scur], c.deep[c.scur], c.stat, c.links, c.out_dev);
     return (hip_ok(hipGetLastError(), "mesoscale state") &&
             hip_ok(hipMemcpyAsync(out_state, c.out_dev, c.cells * meso::kState * 4, hipMemcpyDeviceToHost, st),
                    "mesoscale readback") &&
