@@ -213,6 +213,29 @@ __device__ __forceinline__ Pulled pull(const Params& P, const float* __restrict_
     return v;
 }
 
+// the same rules for a cell at least two cells from the side and top faces and
+// above the floor: every source lies in the domain and no blending applies, so a
+// pull is the neighbour's value or, from a solid neighbour, the bounce
+__device__ __forceinline__ Pulled pull_interior(const float* __restrict__ pf, const float* __restrict__ pg,
+                                                const float* __restrict__ S, uint32_t links, size_t N, int i, int q,
+                                                int sxy, int sy1) {
+    const int src = i - (cx(q) * sxy + cy(q) * sy1 + cz(q));
+    Pulled v;
+    if ((links >> q) & 1u) {
+        v.f = pf[opp(q) * N + i];
+        v.g = tfeq(q, S[kSurfT * N + src], 0.0f, 0.0f, 0.0f);
+    } else {
+        v.f = pf[q * N + src];
+        v.g = pg[q * N + src];
+    }
+    return v;
+}
+
+__device__ __forceinline__ bool interior(const Params& P, int x, int y, int z) {
+    const int edge = min(min(x, z), min(P.nx - 1 - x, P.nz - 1 - z));
+    return min(edge, P.ny - 1 - y) >= 2 && y >= 1;
+}
+
 __device__ __forceinline__ float edge_alpha(const Params& P, int x, int y, int z) {
     const int edge = min(min(x, z), min(P.nx - 1 - x, P.nz - 1 - z));
     const int top = P.ny - 1 - y;
@@ -263,6 +286,14 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
         for (int q = 0; q < kQ; ++q) {
             f[q] = pf_in[q * N + i];
             g[q] = pg_in[q * N + i];
+        }
+    } else if (__all(interior(P, x, y, z))) {
+        const int sy1 = P.nz, sxy = P.ny * P.nz;
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const Pulled v = pull_interior(pf_in, pg_in, S, lk, N, i, q, sxy, sy1);
+            f[q] = v.f;
+            g[q] = v.g;
         }
     } else {
         const float ea = edge_alpha(P, x, y, z);
@@ -375,7 +406,11 @@ __global__ void __launch_bounds__(256) macro_kernel(Params P, const float* __res
         }
         float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
         amb = 0.0f;
-        if (out) {
+        if (!out && __all(interior(P, x, y, z))) {
+            const int sy1 = P.nz, sxy = P.ny * P.nz;
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) amb += pull_interior(pf, pg, S, lk, N, i, q, sxy, sy1).g;
+        } else if (out) {
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {
                 const Pulled v = pull(P, pf, pg, S, lk, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
