@@ -3,10 +3,9 @@
 // segment from its checkpoint and walks it backwards. Per step, three kernels:
 //   collide adjoint  one thread per cell: the 27 incoming values and the new
 //                    temperature are recomputed; the collision depends on them
-//                    through 10 raw moments and T, so the post-collision map is
-//                    evaluated on 11 dual-number seeds and its transposed Jacobian,
-//                    then the transposed moment map, applied to the adjoint of the
-//                    cell's outputs;
+//                    through 10 raw moments and T, so the hand-written reverse
+//                    mode of the post-collision map, then the transposed moment
+//                    map, carry the adjoint of the cell's outputs back to them;
 //   stream adjoint   one thread per (cell, direction): gathers the adjoints of
 //                    every read of the previous post-collision value (pull,
 //                    bounce-back, convective outflow);
@@ -145,20 +144,12 @@ __global__ void __launch_bounds__(128) collide_adjoint_kernel(Params P, const ui
     for (int k = 0; k < 3; ++k) ubar[k] = a1[(size_t)(28 + k) * N + c];
     float tb = a1[(size_t)27 * N + c];  // T^{n+1} is also read by the next step's update
     // the collision depends on g only through its kLow raw moments of order <= 2:
-    // one dual evaluation of the post-collision map per moment (and one for T),
-    // then the transposed moment map back onto the 27 populations
-    float m[kLow], mbar[kLow];
+    // the reverse mode of the post-collision map gives their adjoints (and T's),
+    // then the transposed moment map carries them back onto the 27 populations
+    float m[kLow], mbar[kLow], tcol;
     low_moments(g, m);
-    for (int i = 0; i <= kLow; ++i) {
-        Dual<float> md[kLow], fd[27], ud[3];
-        for (int k = 0; k < kLow; ++k) md[k] = Dual<float>(m[k], k == i ? 1.0f : 0.0f);
-        collide_low(P, md, Dual<float>(Tn, i == kLow ? 1.0f : 0.0f), fd, ud);
-        float acc = 0.0f;
-        for (int q = 0; q < 27; ++q) acc += fd[q].d * fbar[q];
-        for (int k = 0; k < 3; ++k) acc += ud[k].d * ubar[k];
-        if (i < kLow) mbar[i] = acc;
-        else tb += acc;
-    }
+    collide_low_adjoint<float>(P, m, Tn, fbar, ubar, mbar, &tcol);
+    tb += tcol;
     for (int q = 0; q < 27; ++q) {
         const float cx = (float)cxq(q), cy = (float)cyq(q), cz = (float)czq(q);
         const float v = mbar[0] + cx * mbar[1] + cy * mbar[2] + cz * mbar[3] + cx * cx * mbar[4] +

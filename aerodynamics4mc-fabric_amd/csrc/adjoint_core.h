@@ -225,6 +225,245 @@ template <class R> AD_HD void collide_low(const Params& P, const R m[kLow], R T,
     }
 }
 
+// Reverse mode of collide_low by hand, for R = float or double: given the adjoints
+// fbar of the 27 post-collision populations and ubar of the collision velocity,
+// the adjoints of the 10 low raw moments (mbar) and of T (*tbar). The forward
+// values are recomputed here; every branch (density floor, buoyancy clamp, speed
+// cap) takes the side collide_low takes, so this is the transposed Jacobian of
+// exactly that map.
+template <class R>
+AD_HD void collide_low_adjoint(const Params& P, const R m[kLow], R T, const R fbar[27], const R ubar_in[3],
+                               R mbar[kLow], R* tbar) {
+    // ---- forward values
+    const R rho = m[0];
+    const bool rho_live = R(1e-6f) < rho;  // rmax(1e-6, rho) returns rho
+    const R rho_s = rho_live ? rho : R(1e-6f);
+    const R u0[3] = {m[1] / rho_s, m[2] / rho_s, m[3] / rho_s};
+    const R bt = R(P.beta) * T;
+    const R lo_ = R(-0.02f), hi_ = R(0.02f);
+    const R b_in = lo_ < bt ? bt : lo_;
+    const bool b_live = (lo_ < bt) && (b_in < hi_);
+    R b = b_in < hi_ ? b_in : hi_;
+    const R sb = P.g_sign > 0 ? R(-1) : R(1);
+    b = sb * b;
+    R u1[3] = {u0[0], u0[1], u0[2]};
+    u1[P.g_axis] = u1[P.g_axis] + R(0.5) * b;
+    const R s2 = u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2];
+    const bool capped = val(s2) > P.kmax * P.kmax;  // the comparison collide_low makes
+    const R sc = capped ? R(P.kmax) / sqrt_(s2) : R(1);
+    const R u[3] = {capped ? u1[0] * sc : u1[0], capped ? u1[1] * sc : u1[1], capped ? u1[2] * sc : u1[2]};
+    const R ux = u[0], uy = u[1], uz = u[2];
+    const R k100 = m[1] - ux * m[0], k010 = m[2] - uy * m[0];
+    const R c200 = m[4] - R(2) * ux * m[1] + ux * ux * m[0];
+    const R c020 = m[5] - R(2) * uy * m[2] + uy * uy * m[0];
+    const R c002 = m[6] - R(2) * uz * m[3] + uz * uz * m[0];
+    const R c110 = (m[7] - ux * m[2]) - uy * k100;
+    const R c101 = (m[8] - ux * m[3]) - uz * k100;
+    const R c011 = (m[9] - uy * m[3]) - uz * k010;
+    const R cs2 = R(1.0 / 3.0);
+    const double om_off = 1.0 / P.tau_shear;
+    const R sd = R(fmin(fmax(1.0 / P.tau_normal, 0.0), 1.95)), so = R(fmin(fmax(om_off, 0.0), 1.95));
+    const R eq2 = rho_s * cs2;
+    const R d200 = c200 + sd * (eq2 - c200);
+    const R d020 = c020 + sd * (eq2 - c020);
+    const R d002 = c002 + sd * (eq2 - c002);
+    const R om = R(1) - so;
+    const R d110 = om * c110, d101 = om * c101, d011 = om * c011;
+    const R ir = R(1) / rho_s;
+    // central moments after the collision, then the three unshift passes (kept for
+    // the velocity derivative of each pass)
+    R D0[27];
+    for (int k = 0; k < 27; ++k) D0[k] = R(0);
+    D0[AD_IDX(0, 0, 0)] = rho_s;
+    D0[AD_IDX(2, 0, 0)] = d200;
+    D0[AD_IDX(0, 2, 0)] = d020;
+    D0[AD_IDX(0, 0, 2)] = d002;
+    D0[AD_IDX(1, 1, 0)] = d110;
+    D0[AD_IDX(1, 0, 1)] = d101;
+    D0[AD_IDX(0, 1, 1)] = d011;
+    D0[AD_IDX(2, 2, 0)] = (d200 * d020 + R(2) * d110 * d110) * ir;
+    D0[AD_IDX(2, 0, 2)] = (d200 * d002 + R(2) * d101 * d101) * ir;
+    D0[AD_IDX(0, 2, 2)] = (d020 * d002 + R(2) * d011 * d011) * ir;
+    D0[AD_IDX(2, 1, 1)] = (d200 * d011 + R(2) * d110 * d101) * ir;
+    D0[AD_IDX(1, 2, 1)] = (d020 * d101 + R(2) * d110 * d011) * ir;
+    D0[AD_IDX(1, 1, 2)] = (d002 * d110 + R(2) * d101 * d011) * ir;
+    const R p222 = d200 * d020 * d002 + R(2) * d110 * d110 * d002 + R(2) * d101 * d101 * d020 +
+                   R(2) * d011 * d011 * d200 + R(8) * d110 * d101 * d011;
+    D0[AD_IDX(2, 2, 2)] = p222 * ir * ir;
+    R D1[27], D2[27];
+    for (int k = 0; k < 27; ++k) D1[k] = D0[k];
+    lines(D1, 2, [uz](R& a, R& bb, R& c) { unshift_line(a, bb, c, uz); });
+    for (int k = 0; k < 27; ++k) D2[k] = D1[k];
+    lines(D2, 1, [uy](R& a, R& bb, R& c) { unshift_line(a, bb, c, uy); });
+    // ---- reverse
+    R fb[27];
+    for (int q = 0; q < 27; ++q) fb[q] = fbar[q];
+    R ub[3] = {ubar_in[0], ubar_in[1], ubar_in[2]};
+    R rsb = R(0), bb = R(0);
+    if (P.beta != 0.0f) {  // Guo source
+        const R pref = rmax(R(0), R(1) - R(0.5) * R(fmin(fmax(om_off, 0.0), 1.999)));
+        const R fa = pref * rho_s * b;
+        R fab = R(0);
+        for (int q = 0; q < 27; ++q) {
+            const R c[3] = {R(float(cxq(q))), R(float(cyq(q))), R(float(czq(q)))};
+            const R cu = c[0] * ux + c[1] * uy + c[2] * uz;
+            const R wf = wq<R>(q) * fb[q];
+            fab = fab + wf * (R(3) * (c[P.g_axis] - u[P.g_axis]) + R(9) * cu * c[P.g_axis]);
+            ub[P.g_axis] = ub[P.g_axis] - R(3) * wf * fa;
+            const R cub = R(9) * wf * fa * c[P.g_axis];
+            for (int k = 0; k < 3; ++k) ub[k] = ub[k] + cub * c[k];
+        }
+        rsb = rsb + fab * pref * b;
+        bb = bb + fab * pref * rho_s;
+    }
+    // inverse line transforms (forward: x, y, z), transposed in reverse order
+    auto inv_t = [](R& o0, R& o1, R& o2) {
+        const R a = o0, b1 = o1, c = o2;
+        o0 = b1;
+        o1 = R(0.5) * (c - a);
+        o2 = R(0.5) * (a + c) - b1;
+    };
+    lines(fb, 2, inv_t);
+    lines(fb, 1, inv_t);
+    lines(fb, 0, inv_t);
+    // unshift passes (forward: z on D0, y on D1, x on D2), transposed in reverse order
+    R uxb = R(0), uyb = R(0), uzb = R(0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int a0 = AD_IDX(0, i, j);
+            const R b0 = fb[a0], b1 = fb[a0 + 9], b2 = fb[a0 + 18];
+            uxb = uxb + b1 * D2[a0] + b2 * (R(2) * D2[a0 + 9] + R(2) * ux * D2[a0]);
+            fb[a0] = b0 + ux * b1 + ux * ux * b2;
+            fb[a0 + 9] = b1 + R(2) * ux * b2;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int a0 = AD_IDX(i, 0, j);
+            const R b0 = fb[a0], b1 = fb[a0 + 3], b2 = fb[a0 + 6];
+            uyb = uyb + b1 * D1[a0] + b2 * (R(2) * D1[a0 + 3] + R(2) * uy * D1[a0]);
+            fb[a0] = b0 + uy * b1 + uy * uy * b2;
+            fb[a0 + 3] = b1 + R(2) * uy * b2;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int a0 = AD_IDX(i, j, 0);
+            const R b0 = fb[a0], b1 = fb[a0 + 1], b2 = fb[a0 + 2];
+            uzb = uzb + b1 * D0[a0] + b2 * (R(2) * D0[a0 + 1] + R(2) * uz * D0[a0]);
+            fb[a0] = b0 + uz * b1 + uz * uz * b2;
+            fb[a0 + 1] = b1 + R(2) * uz * b2;
+        }
+    // closure (orders 4 and 6)
+    R d200b = fb[AD_IDX(2, 0, 0)], d020b = fb[AD_IDX(0, 2, 0)], d002b = fb[AD_IDX(0, 0, 2)];
+    R d110b = fb[AD_IDX(1, 1, 0)], d101b = fb[AD_IDX(1, 0, 1)], d011b = fb[AD_IDX(0, 1, 1)];
+    R irb = R(0);
+    rsb = rsb + fb[AD_IDX(0, 0, 0)];
+    {
+        const R x = fb[AD_IDX(2, 2, 0)];
+        d200b = d200b + x * ir * d020; d020b = d020b + x * ir * d200; d110b = d110b + x * ir * R(4) * d110;
+        irb = irb + x * (d200 * d020 + R(2) * d110 * d110);
+    }
+    {
+        const R x = fb[AD_IDX(2, 0, 2)];
+        d200b = d200b + x * ir * d002; d002b = d002b + x * ir * d200; d101b = d101b + x * ir * R(4) * d101;
+        irb = irb + x * (d200 * d002 + R(2) * d101 * d101);
+    }
+    {
+        const R x = fb[AD_IDX(0, 2, 2)];
+        d020b = d020b + x * ir * d002; d002b = d002b + x * ir * d020; d011b = d011b + x * ir * R(4) * d011;
+        irb = irb + x * (d020 * d002 + R(2) * d011 * d011);
+    }
+    {
+        const R x = fb[AD_IDX(2, 1, 1)];
+        d200b = d200b + x * ir * d011; d011b = d011b + x * ir * d200;
+        d110b = d110b + x * ir * R(2) * d101; d101b = d101b + x * ir * R(2) * d110;
+        irb = irb + x * (d200 * d011 + R(2) * d110 * d101);
+    }
+    {
+        const R x = fb[AD_IDX(1, 2, 1)];
+        d020b = d020b + x * ir * d101; d101b = d101b + x * ir * d020;
+        d110b = d110b + x * ir * R(2) * d011; d011b = d011b + x * ir * R(2) * d110;
+        irb = irb + x * (d020 * d101 + R(2) * d110 * d011);
+    }
+    {
+        const R x = fb[AD_IDX(1, 1, 2)];
+        d002b = d002b + x * ir * d110; d110b = d110b + x * ir * d002;
+        d101b = d101b + x * ir * R(2) * d011; d011b = d011b + x * ir * R(2) * d101;
+        irb = irb + x * (d002 * d110 + R(2) * d101 * d011);
+    }
+    {
+        const R x = fb[AD_IDX(2, 2, 2)];
+        const R pb = x * ir * ir;
+        irb = irb + x * R(2) * p222 * ir;
+        d200b = d200b + pb * (d020 * d002 + R(2) * d011 * d011);
+        d020b = d020b + pb * (d200 * d002 + R(2) * d101 * d101);
+        d002b = d002b + pb * (d200 * d020 + R(2) * d110 * d110);
+        d110b = d110b + pb * (R(4) * d110 * d002 + R(8) * d101 * d011);
+        d101b = d101b + pb * (R(4) * d101 * d020 + R(8) * d110 * d011);
+        d011b = d011b + pb * (R(4) * d011 * d200 + R(8) * d110 * d101);
+    }
+    rsb = rsb - irb * ir * ir;
+    // relaxation
+    const R c200b = d200b * (R(1) - sd), c020b = d020b * (R(1) - sd), c002b = d002b * (R(1) - sd);
+    rsb = rsb + (d200b + d020b + d002b) * sd * cs2;
+    const R c110b = om * d110b, c101b = om * d101b, c011b = om * d011b;
+    // central moments about u
+    R mb[kLow];
+    for (int k = 0; k < kLow; ++k) mb[k] = R(0);
+    mb[4] = c200b;
+    mb[1] = mb[1] - R(2) * ux * c200b;
+    mb[0] = mb[0] + ux * ux * c200b;
+    uxb = uxb + c200b * (R(2) * ux * m[0] - R(2) * m[1]);
+    mb[5] = c020b;
+    mb[2] = mb[2] - R(2) * uy * c020b;
+    mb[0] = mb[0] + uy * uy * c020b;
+    uyb = uyb + c020b * (R(2) * uy * m[0] - R(2) * m[2]);
+    mb[6] = c002b;
+    mb[3] = mb[3] - R(2) * uz * c002b;
+    mb[0] = mb[0] + uz * uz * c002b;
+    uzb = uzb + c002b * (R(2) * uz * m[0] - R(2) * m[3]);
+    R k100b = R(0), k010b = R(0);
+    mb[7] = c110b;
+    mb[2] = mb[2] - ux * c110b;
+    uxb = uxb - m[2] * c110b;
+    uyb = uyb - k100 * c110b;
+    k100b = k100b - uy * c110b;
+    mb[8] = c101b;
+    mb[3] = mb[3] - ux * c101b;
+    uxb = uxb - m[3] * c101b;
+    uzb = uzb - k100 * c101b;
+    k100b = k100b - uz * c101b;
+    mb[9] = c011b;
+    mb[3] = mb[3] - uy * c011b;
+    uyb = uyb - m[3] * c011b;
+    uzb = uzb - k010 * c011b;
+    k010b = k010b - uz * c011b;
+    mb[1] = mb[1] + k100b;
+    uxb = uxb - m[0] * k100b;
+    mb[0] = mb[0] - ux * k100b;
+    mb[2] = mb[2] + k010b;
+    uyb = uyb - m[0] * k010b;
+    mb[0] = mb[0] - uy * k010b;
+    ub[0] = ub[0] + uxb;
+    ub[1] = ub[1] + uyb;
+    ub[2] = ub[2] + uzb;
+    // speed cap
+    R u1b[3] = {ub[0], ub[1], ub[2]};
+    if (capped) {
+        const R scb = ub[0] * u1[0] + ub[1] * u1[1] + ub[2] * u1[2];
+        const R s2b = scb * (R(-0.5) * sc / s2);
+        for (int k = 0; k < 3; ++k) u1b[k] = ub[k] * sc + s2b * R(2) * u1[k];
+    }
+    // buoyancy and the division by the density
+    bb = bb + R(0.5) * u1b[P.g_axis];
+    for (int k = 0; k < 3; ++k) {
+        mb[k + 1] = mb[k + 1] + u1b[k] / rho_s;
+        rsb = rsb - u1b[k] * u0[k] / rho_s;
+    }
+    *tbar = b_live ? bb * sb * R(P.beta) : R(0);
+    mb[0] = mb[0] + (rho_live ? rsb : R(0));
+    for (int k = 0; k < kLow; ++k) mbar[k] = mb[k];
+}
+
 template <class R> AD_HD void collide(const Params& P, R f[27], R T, R u_out[3]) {
     R m[kLow];
     low_moments(f, m);

@@ -102,3 +102,31 @@ extern "C" int ad_dual_directional(int n, int m, int steps, double nu, float bet
     *out_dloss = L.d;
     return 1;
 }
+
+// The hand reverse mode of the low-moment collision (collide_low_adjoint) against
+// forward-mode dual numbers on the same map, in FP64: for inputs m[10], T and output
+// adjoints fbar[27], ubar[3], writes the reverse result (mbar_rev[10], tbar_rev) and
+// the one assembled from 11 dual evaluations (mbar_dual[10], tbar_dual).
+extern "C" void ad_collide_vjp(double nu, float beta, int g_axis, int g_sign, float kmax, const double* m,
+                               double T, const double* fbar, const double* ubar, double* mbar_rev,
+                               double* tbar_rev, double* mbar_dual, double* tbar_dual) {
+    Params P{};
+    P.nx = P.ny = P.nz = 4;
+    P.tau_shear = P.tau_normal = 0.5 + 3.0 * nu;
+    P.beta = beta;
+    P.g_axis = g_axis;
+    P.g_sign = g_sign;
+    P.kmax = kmax;
+    P.rho_in = 1.0f;
+    collide_low_adjoint<double>(P, m, T, fbar, ubar, mbar_rev, tbar_rev);
+    for (int i = 0; i <= kLow; ++i) {
+        D md[kLow], fd[27], ud[3];
+        for (int k = 0; k < kLow; ++k) md[k] = D(m[k], k == i ? 1.0 : 0.0);
+        collide_low(P, md, D(T, i == kLow ? 1.0 : 0.0), fd, ud);
+        double acc = 0.0;
+        for (int q = 0; q < 27; ++q) acc += fd[q].d * fbar[q];
+        for (int k = 0; k < 3; ++k) acc += ud[k].d * ubar[k];
+        if (i < kLow) mbar_dual[i] = acc;
+        else *tbar_dual = acc;
+    }
+}

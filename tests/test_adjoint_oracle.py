@@ -7,6 +7,10 @@
   velocities matches to 1e-9 relative.
 * The FP64 dual-number directional derivative equals central finite differences
   of the same FP64 loss (1e-6 relative): the dual propagation is right.
+* The hand-written reverse mode of the collision (collide_low_adjoint, the
+  product's collide adjoint) equals the transposed Jacobian assembled from
+  forward-mode dual numbers on the same map, in FP64, across the density floor,
+  the buoyancy clamp and the speed cap.
 """
 import ctypes
 
@@ -92,3 +96,35 @@ def test_dual_derivative_equals_central_differences(oracle_mod, heat_scale):
     lm, _ = oracle_mod.dual_directional(*args, inflow - eps * di, heat - eps * dh, 0 * di, 0 * dh)
     fd = (lp - lm) / (2 * eps)
     assert abs(dl - fd) <= 1e-6 * abs(fd), (dl, fd)
+
+
+def test_collide_reverse_mode_equals_dual_numbers():
+    import os
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                                   "libadjoint_dual.so"))
+    f = lib.ad_collide_vjp
+    D = ctypes.POINTER(ctypes.c_double)
+    f.argtypes = [ctypes.c_double, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float, D, ctypes.c_double,
+                  D, D, D, D, D, D]
+    rng = np.random.default_rng(0)
+    kmax = 0.57735026919 * 0.6
+    branches = set()
+    for trial in range(400):
+        rho = 1.0 + 0.05 * rng.standard_normal()
+        speed = [0.02, 0.1, 0.4][trial % 3]  # 0.4 is above the speed cap
+        u = rng.standard_normal(3)
+        u = u / np.linalg.norm(u) * speed
+        m = np.array([rho, *(rho * u), rho * (1 / 3 + u[0] ** 2), rho * (1 / 3 + u[1] ** 2),
+                      rho * (1 / 3 + u[2] ** 2), rho * u[0] * u[1], rho * u[0] * u[2], rho * u[1] * u[2]])
+        m = m + 1e-3 * rng.standard_normal(10)
+        T = [0.0, 0.05, 0.5, -0.5][trial % 4]  # beta * 0.5 lies outside the clamp
+        beta = [0.0, 0.1][trial % 2]
+        fb, ub = rng.standard_normal(27), rng.standard_normal(3)
+        out = [np.zeros(10), np.zeros(1), np.zeros(10), np.zeros(1)]
+        p = lambda a: a.ctypes.data_as(D)  # noqa: E731
+        f(1e-3 * (1 + trial % 5), beta, trial % 3, 1 if trial % 2 else -1, kmax, p(m), T, p(fb), p(ub),
+          p(out[0]), p(out[1]), p(out[2]), p(out[3]))
+        rev, dual = np.concatenate(out[:2]), np.concatenate(out[2:])
+        assert np.max(np.abs(rev - dual)) <= 1e-11 * max(1.0, np.max(np.abs(dual))), trial
+        branches.add((speed > kmax, abs(beta * T) > 0.02))
+    assert branches == {(False, False), (True, False), (False, True), (True, True)}
