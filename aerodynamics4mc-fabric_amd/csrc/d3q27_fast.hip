@@ -393,20 +393,32 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
         for (int q = 0; q < 27; ++q)
             St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, Mode == 0 ? q : lat_opp(q), R2.row), g[q], q);
     } else {
-        // push to x + c_q; a target outside the domain is dropped (row: skipped, z:
-        // out-of-range offset), a solid target receives a value no fluid cell reads
+        // one store per direction: pushed to x + c_q, or, when that target is solid
+        // or outside the domain (the mask's bit opp(q)), bounced back into the cell's
+        // own slot opp(q). As for the pull, both candidates lie in one buffer: the
+        // scalar offset is the lower of the two and each lane picks its vector offset.
+        // A neighbour row outside the domain makes every lane of the wave bounce.
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const int dx = lat_cx(q), dy = lat_cy(q), dz = lat_cz(q);
-            if (R2.ok(dx, dy)) St::template bst<StPol>(rout, zb[dz + 1], plane_row<T>(Ni, q, R2.off(dx, dy)), g[q], q);
-        }
-        // target solid or outside: bounce back into the cell's own slot opp(q)
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-            const bool sp = (links >> lat_opp(q)) & 1u;
-            if (__builtin_amdgcn_ballot_w64(sp)) {
-                if (sp) St::template bst<StPol>(rout, zb[1], plane_row<T>(Ni, lat_opp(q), R2.row), g[q], q);
+            const int sb = plane_row<T>(Ni, lat_opp(q), R2.row);
+            if (!R2.ok(dx, dy)) {
+                St::template bst<StPol>(rout, zb[1], sb, g[q], q);
+                continue;
             }
+            const int sn = plane_row<T>(Ni, q, R2.off(dx, dy));
+            const int bounce = (int)(links << (31 - lat_opp(q))) >> 31;
+            int so, vn, vb;
+            if (q <= 13) {  // plane q lies below plane opp(q); a row offset is below the plane stride
+                so = sn;
+                vn = zb[dz + 1];
+                vb = zb[1] + (sb - sn);
+            } else {
+                so = sb;
+                vn = zb[dz + 1] + (sn - sb);
+                vb = zb[1];
+            }
+            St::template bst<StPol>(rout, (bounce & vb) | (~bounce & vn), so, g[q], q);
         }
     }
 }
