@@ -123,8 +123,11 @@ def test_fast_population_array_size_limit(lib):
     """The fast kernels address a population array through one buffer resource with
     32-bit byte offsets: FP32 in place takes 27 planes x 4 B per cell, so a brick of
     20 x 1024 x 1001 cells (2.2 GB) is refused with a message and 19 x 1024 x 1000
-    (2.10 GB) is accepted."""
-    cfg = W.channel_config(0.04, 40.0, sgs=False, buoyancy=False)
+    (2.10 GB) is accepted and steps: uniform flow through the periodic channel is a
+    fixed point, so after two even and two odd in-place steps every cell, the last
+    planes' (q = 26, byte offsets near 2^31) included, still reads the inflow."""
+    u0 = 0.04
+    cfg = W.channel_config(u0, 40.0, sgs=False, buoyancy=False)
     ov = W.reference_overrides(A.KERNEL_FAST)
     for dims, ok in (((20, 1024, 1001), False), ((19, 1024, 1000), True)):
         log(f"size limit {dims} accepted={ok}")
@@ -133,6 +136,16 @@ def test_fast_population_array_size_limit(lib):
         if ok:
             ctx.set_temperature(t)
             assert np.array_equal(ctx.temperature()[:1000], t[:1000])
+            packet = W.make_packet(np.zeros(dims, np.uint8), u0=(u0, 0.0, 0.0), perturb=0.0)
+            ctx.step(packet)
+            del packet
+            for _ in range(3):
+                ctx.step(None)
+            flow = ctx.flow().reshape(-1, 4)
+            assert np.all(np.isfinite(flow))
+            err = float(np.max(np.abs(flow[:, :3] - np.array([u0, 0.0, 0.0], np.float32))))
+            log(f"  uniform flow after 4 steps: max |u - u0| = {err:.2e}")
+            assert err < 1e-5
         else:
             with pytest.raises(A.AeroLbmError, match="too large for the fast kernels"):
                 ctx.set_temperature(t)
