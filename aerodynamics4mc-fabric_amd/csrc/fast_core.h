@@ -249,9 +249,9 @@ struct PrevPost {
 
 // Incoming value for an out-of-domain link q at fluid cell (x,y,z); `bounce` is the
 // cell's own post-collision value of opp(q) (the reference's halfway bounce-back).
-// The candidate values of every rule are loaded unconditionally (clamped in-domain
-// positions) and the rule then selects, so the links of a cell do not form a chain
-// of dependent loads.
+// The face's rule is picked first and only its inputs are loaded: the mirrored
+// source for symmetry, the two layers along the normal for convective outflow,
+// nothing for bounce-back and the equilibrium faces.
 template <class St, bool AA>
 __device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* flags, float ref_p_cell,
                                            const PrevPost<St, AA>& prev, int cell, int x, int y, int z, int q,
@@ -262,21 +262,26 @@ __device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* f
     const int sz = wrapc(z - cz, P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]);
     const int face = oob_face(P, sx, sy, sz);
     const FaceParams& fc = P.faces_dev[face];
-    // symmetry source (ref: jni.cpp:1582-1587) and convective layers (jni.cpp:1494-1516)
-    const int ix = icl(sx, 0, P.nx - 1), iy = icl(sy, 0, P.ny - 1), iz = icl(sz, 0, P.nz - 1);
+    int kind = fc.hydro_kind;
+    if (kind == 0 || kind == 1) kind = P.convective_disabled && kind == 0 ? 2 : 6;
+    if (kind == 2) return bounce;
+    if (kind == 7) {  // symmetry: the mirrored in-domain source (ref: jni.cpp:1582-1587)
+        const int ix = icl(sx, 0, P.nx - 1), iy = icl(sy, 0, P.ny - 1), iz = icl(sz, 0, P.nz - 1);
+        const bool s0 = flags[(ix * P.ny + iy) * P.nz + iz] & kCellSolid;
+        const float p0 = prev.get(P, face, q, ix, iy, iz);
+        return s0 ? bounce : p0;
+    }
+    // convective layers along the face normal (jni.cpp:1494-1516)
     const int dx = (sx < 0 || sx >= P.nx) ? cx : 0;
     const int dy = (sy < 0 || sy >= P.ny) ? cy : 0;
     const int dz = (sz < 0 || sz >= P.nz) ? cz : 0;
     const int x1 = icl(x + dx, 0, P.nx - 1), y1 = icl(y + dy, 0, P.ny - 1), z1 = icl(z + dz, 0, P.nz - 1);
     const int x2 = icl(x + 2 * dx, 0, P.nx - 1), y2 = icl(y + 2 * dy, 0, P.ny - 1), z2 = icl(z + 2 * dz, 0, P.nz - 1);
-    const bool s0 = flags[(ix * P.ny + iy) * P.nz + iz] & kCellSolid;
-    const bool s1 = flags[(x1 * P.ny + y1) * P.nz + z1] & kCellSolid;
-    const bool s2 = flags[(x2 * P.ny + y2) * P.nz + z2] & kCellSolid;
-    const float p0 = prev.get(P, face, q, ix, iy, iz);
-    const float p1 = prev.get(P, face, q, x1, y1, z1);
-    const float p2 = prev.get(P, face, q, x2, y2, z2);
-    int kind = fc.hydro_kind;
-    if (kind == 0 || kind == 1) kind = P.convective_disabled && kind == 0 ? 2 : 6;
+    const bool conv_rule = !(kind >= 3 && kind <= 5);  // equilibrium faces load nothing
+    const bool s1 = conv_rule && (flags[(x1 * P.ny + y1) * P.nz + z1] & kCellSolid);
+    const bool s2 = conv_rule && (flags[(x2 * P.ny + y2) * P.nz + z2] & kCellSolid);
+    const float p1 = conv_rule ? prev.get(P, face, q, x1, y1, z1) : 0.0f;
+    const float p2 = conv_rule ? prev.get(P, face, q, x2, y2, z2) : 0.0f;
     // velocity / pressure / inlet equilibrium (ref: jni.cpp:1518-1561)
     float rho = 1.0f + clampf(ref_p_cell, -0.03f, 0.03f);
     float ux = 0.0f, uy = 0.0f, uz = 0.0f;
@@ -296,9 +301,8 @@ __device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* f
         uz *= sc;
     }
     const float eq = feq_s<St::kShifted>(q, rho, ux, uy, uz);
-    const float sym = s0 ? bounce : p0;
     const float conv = s1 ? bounce : (s2 ? p1 : p1 + 0.15f * (p1 - p2));  // w shift cancels: weights sum to one
-    return kind == 2 ? bounce : (kind >= 3 && kind <= 5) ? eq : (kind == 7 ? sym : conv);
+    return conv_rule ? conv : eq;
 }
 
 // The 9 links of an x-face cell pulled from outside the domain (Side 0: x_min,
