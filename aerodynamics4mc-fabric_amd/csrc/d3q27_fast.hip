@@ -662,9 +662,9 @@ __global__ void __launch_bounds__(256) readout_kernel(StepParams P, CellFields C
 // ---------------------------------------------------------------- batched bricks
 // Every workgroup reads its brick's entry of the table (a wave-uniform index, so
 // the parameters come through scalar loads) and runs the single-brick code on it.
-// At most 96 VGPRs (5 waves per SIMD): left alone the compiler gives the odd
-// (pull + push) instantiation ~100 and 4 waves; 5 waves cost a 24-byte spill and
-// measured +8% on the 56-brick batch, 6 waves (80 VGPRs) spill more and lose 10%.
+// At least 5 waves per SIMD (at most 96 VGPRs): the odd (pull + push)
+// instantiation once needed ~100 VGPRs; with one store per direction it fits in 80
+// (6 waves) and the even one in 72 (7 waves), neither spilling.
 template <class St, class Uv, int Mode, int StPol, bool Shell>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) step_kernel_batch(const BatchBrick* __restrict__ table,
                                                          const uint16_t* __restrict__ owner) {
