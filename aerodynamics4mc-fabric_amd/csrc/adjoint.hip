@@ -85,11 +85,34 @@ __global__ void __launch_bounds__(256) source_kernel(Params P, int m, const R* _
     src[c] = rclamp(v, -0.006f, 0.006f);
 }
 
+// per-cell mask of the solid flags of the 27 cells at offsets -1..1 (bit
+// (dx + 1) * 9 + (dy + 1) * 3 + dz + 1; y and z periodic, outside in x -> 0): the
+// update of a cell then tests its stencil's solid links from one register
+__global__ void __launch_bounds__(256) nbr_kernel(Params P, const uint8_t* __restrict__ solid,
+                                                  uint32_t* __restrict__ nb) {
+    const int N = P.nx * P.ny * P.nz;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    int x, y, z;
+    cell_xyz<float>(P, c, x, y, z);
+    uint32_t m = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int xs = x + dx;
+        if (xs < 0 || xs >= P.nx) continue;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz)
+                if (solid[cidx(P, xs, wrap(y + dy, P.ny), wrap(z + dz, P.nz))])
+                    m |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + dz + 1);
+    }
+    nb[c] = m;
+}
+
 // first: the initial state, whose populations are collided where they are (no
 // stream before the first collision, as the reference's first step)
 template <class R>
 __global__ void __launch_bounds__(256) forward_kernel(Params P, const uint8_t* __restrict__ solid,
-                                                      const R* __restrict__ uin, const R* __restrict__ src,
+                                                      const uint32_t* __restrict__ nbm, const R* __restrict__ uin,
+                                                      const R* __restrict__ src,
                                                       const R* __restrict__ si, R* __restrict__ so, int first) {
     const int N = P.nx * P.ny * P.nz;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,12 +125,13 @@ __global__ void __launch_bounds__(256) forward_kernel(Params P, const uint8_t* _
     int x, y, z;
     cell_xyz<R>(P, c, x, y, z);
     const R* T = si + (size_t)27 * N;
+    const SolidNbrs sq{nbm[c], solid};
     int idx[14];
     R w[8], pre;
-    const R Tn = temperature(P, T, solid, src[c], x, y, z, si[(size_t)28 * N + c], si[(size_t)29 * N + c],
+    const R Tn = temperature(P, T, sq, src[c], x, y, z, si[(size_t)28 * N + c], si[(size_t)29 * N + c],
                              si[(size_t)30 * N + c], idx, w, &pre);
     R g[27], u[3];
-    for (int q = 0; q < 27; ++q) g[q] = first ? si[(size_t)q * N + c] : incoming(P, si, solid, uin, x, y, z, q);
+    for (int q = 0; q < 27; ++q) g[q] = first ? si[(size_t)q * N + c] : incoming(P, si, sq, uin, x, y, z, q);
     collide(P, g, Tn, u);
     for (int q = 0; q < 27; ++q) so[(size_t)q * N + c] = g[q];
     so[(size_t)27 * N + c] = Tn;
@@ -118,6 +142,7 @@ __global__ void __launch_bounds__(256) forward_kernel(Params P, const uint8_t* _
 // adjoints of the step's outputs (a1: F, T, U of state n+1) -> per-cell adjoint of
 // the 27 incoming values (gbar) and of the new temperature (tbar); inflow adjoint
 __global__ void __launch_bounds__(128) collide_adjoint_kernel(Params P, const uint8_t* __restrict__ solid,
+                                                              const uint32_t* __restrict__ nbm,
                                                               const float* __restrict__ uin,
                                                               const float* __restrict__ src,
                                                               const float* __restrict__ s,
@@ -133,12 +158,13 @@ __global__ void __launch_bounds__(128) collide_adjoint_kernel(Params P, const ui
     }
     int x, y, z;
     cell_xyz<float>(P, c, x, y, z);
+    const SolidNbrs sq{nbm[c], solid};
     int idx[14];
     float w[8], pre;
-    const float Tn = temperature(P, s + (size_t)27 * N, solid, src[c], x, y, z, s[(size_t)28 * N + c],
+    const float Tn = temperature(P, s + (size_t)27 * N, sq, src[c], x, y, z, s[(size_t)28 * N + c],
                                  s[(size_t)29 * N + c], s[(size_t)30 * N + c], idx, w, &pre);
     float g[27];
-    for (int q = 0; q < 27; ++q) g[q] = first ? s[(size_t)q * N + c] : incoming(P, s, solid, uin, x, y, z, q);
+    for (int q = 0; q < 27; ++q) g[q] = first ? s[(size_t)q * N + c] : incoming(P, s, sq, uin, x, y, z, q);
     float fbar[27], ubar[3];
     for (int q = 0; q < 27; ++q) fbar[q] = a1[(size_t)q * N + c];
     for (int k = 0; k < 3; ++k) ubar[k] = a1[(size_t)(28 + k) * N + c];
@@ -216,6 +242,7 @@ __global__ void __launch_bounds__(256) stream_adjoint_kernel(Params P, const uin
 
 // temperature and velocity adjoints of state n from tbar (the adjoint of T^{n+1})
 __global__ void __launch_bounds__(256) thermal_adjoint_kernel(Params P, const uint8_t* __restrict__ solid,
+                                                              const uint32_t* __restrict__ nbm,
                                                               const float* __restrict__ src,
                                                               const uint8_t* __restrict__ src_free,
                                                               const float* __restrict__ s,
@@ -236,7 +263,7 @@ __global__ void __launch_bounds__(256) thermal_adjoint_kernel(Params P, const ui
     const float ux = s[(size_t)28 * N + c], uy = s[(size_t)29 * N + c], uz = s[(size_t)30 * N + c];
     int idx[14];
     float w[8], pre;
-    (void)temperature(P, T, solid, src[c], x, y, z, ux, uy, uz, idx, w, &pre);
+    (void)temperature(P, T, SolidNbrs{nbm[c], solid}, src[c], x, y, z, ux, uy, uz, idx, w, &pre);
     const float tb = (pre > 1.0f || pre < -1.0f) ? 0.0f : tbar[c];
     float self = -6.0f * P.alpha * tb;
     for (int k = 0; k < 8; ++k) {
@@ -281,6 +308,7 @@ struct Model {
     std::vector<int> probes;
     std::vector<float> target;
     uint8_t* solid = nullptr;
+    uint32_t* nbmask = nullptr;  // neighbour solid mask of each cell (nbr_kernel)
     uint8_t* src_free = nullptr;
     float *uin = nullptr, *heat = nullptr, *src = nullptr;
     float *ckpt = nullptr, *seg = nullptr;
@@ -316,8 +344,8 @@ bool run_forward(Model& M) {
     for (int n = 0; n < M.steps; ++n) {
         float* nxt = ((n + 1) % M.every == 0) ? state(M, M.ckpt, (n + 1) / M.every) : state(M, M.seg, (n & 1));
         if (nxt == cur) nxt = state(M, M.seg, 1 - (n & 1));
-        hipLaunchKernelGGL(forward_kernel<float>, blocks(N), dim3(256), 0, st, M.P, M.solid, M.uin, M.src, cur, nxt,
-                           n == 0 ? 1 : 0);
+        hipLaunchKernelGGL(forward_kernel<float>, blocks(N), dim3(256), 0, st, M.P, M.solid, M.nbmask, M.uin, M.src,
+                           cur, nxt, n == 0 ? 1 : 0);
         cur = nxt;
     }
     // probe velocities of the final state
@@ -365,12 +393,12 @@ bool run_backward(Model& M) {
                     "checkpoint"))
             return false;
         for (int j = 0; j < len - 1; ++j)
-            hipLaunchKernelGGL(forward_kernel<float>, blocks(N), dim3(256), 0, st, M.P, M.solid, M.uin, M.src,
+            hipLaunchKernelGGL(forward_kernel<float>, blocks(N), dim3(256), 0, st, M.P, M.solid, M.nbmask, M.uin, M.src,
                                state(M, M.seg, j), state(M, M.seg, j + 1), start + j == 0 ? 1 : 0);
         for (int j = len - 1; j >= 0; --j) {
             const float* s = state(M, M.seg, j);
             const int first = start + j == 0 ? 1 : 0;
-            hipLaunchKernelGGL(collide_adjoint_kernel, blocks(N, 128), dim3(128), 0, st, M.P, M.solid, M.uin, M.src, s,
+            hipLaunchKernelGGL(collide_adjoint_kernel, blocks(N, 128), dim3(128), 0, st, M.P, M.solid, M.nbmask, M.uin, M.src, s,
                                M.a1, M.gbar, M.tbar, M.uin_bar, first);
             // the initial populations depend on no parameter: their adjoint is not needed
             if (!first)
@@ -378,7 +406,7 @@ bool run_backward(Model& M) {
                                    M.gbar, M.a0);
             hipLaunchKernelGGL(zero_kernel, blocks(4 * (size_t)N), dim3(256), 0, st, M.a0 + (size_t)27 * N,
                                4 * (size_t)N);
-            hipLaunchKernelGGL(thermal_adjoint_kernel, blocks(N), dim3(256), 0, st, M.P, M.solid, M.src, M.src_free, s,
+            hipLaunchKernelGGL(thermal_adjoint_kernel, blocks(N), dim3(256), 0, st, M.P, M.solid, M.nbmask, M.src, M.src_free, s,
                                M.tbar, M.a0, M.src_bar);
             std::swap(M.a0, M.a1);
         }
@@ -430,7 +458,8 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_adjoint_create(const AeroLbmAdjoint
     M->target.assign(target_uvw, target_uvw + 3 * (size_t)cfg->probe_count);
     const size_t N = M->N, sN = (size_t)kState * N;
     const int nck = M->steps / M->every + 1;
-    bool ok = M->alloc((void**)&M->solid, N) && M->alloc((void**)&M->src_free, N) &&
+    bool ok = M->alloc((void**)&M->solid, N) && M->alloc((void**)&M->nbmask, N * 4) &&
+              M->alloc((void**)&M->src_free, N) &&
               M->alloc((void**)&M->uin, 3 * (size_t)n * n * 4) && M->alloc((void**)&M->heat, (size_t)M->m * M->m * M->m * 4) &&
               M->alloc((void**)&M->src, N * 4) && M->alloc((void**)&M->ckpt, (size_t)nck * sN * 4) &&
               M->alloc((void**)&M->seg, (size_t)std::max(2, M->every) * sN * 4) && M->alloc((void**)&M->a0, sN * 4) &&
@@ -439,6 +468,8 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_adjoint_create(const AeroLbmAdjoint
               M->alloc((void**)&M->uin_bar, 3 * (size_t)n * n * 4);
     if (!ok) return 0;
     if (!hip_ok(hipMemcpy(M->solid, solid, N, hipMemcpyHostToDevice), "upload solid")) return 0;
+    hipLaunchKernelGGL(nbr_kernel, blocks(N), dim3(256), 0, rt().stream, M->P, M->solid, M->nbmask);
+    if (!hip_ok(hipGetLastError(), "nbr_kernel") || !hip_ok(hipStreamSynchronize(rt().stream), "sync")) return 0;
     const long long h = g_next_model++;
     g_models[h] = std::move(M);
     *out_handle = h;
@@ -540,7 +571,7 @@ extern "C" AERO_LBM_CAPI_EXPORT int aero_lbm_adjoint_forward_fp64(long long h, c
                        (uint8_t*)nullptr);
     hipLaunchKernelGGL(init_kernel<double>, blocks(N), dim3(256), 0, st, M->P, M->solid, M->s64[0]);
     for (int s = 0; s < M->steps; ++s)
-        hipLaunchKernelGGL(forward_kernel<double>, blocks(N), dim3(256), 0, st, M->P, M->solid, M->uin64, M->src64,
+        hipLaunchKernelGGL(forward_kernel<double>, blocks(N), dim3(256), 0, st, M->P, M->solid, M->nbmask, M->uin64, M->src64,
                            M->s64[s & 1], M->s64[1 - (s & 1)], s == 0 ? 1 : 0);
     std::vector<double> u(3 * N);
     const double* fin = M->s64[M->steps & 1];

@@ -471,14 +471,36 @@ template <class R> AD_HD void collide(const Params& P, R f[27], R T, R u_out[3])
 }
 
 AD_HD int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+// Solid-flag queries: the flag of in-domain cell c (after periodic wrap), which lies
+// at offset (dx, dy, dz) from the cell being updated. SolidArray reads the flag
+// array; SolidNbrs answers offsets of at most one cell from the cell's 27-bit
+// neighbour mask (bit (dx + 1) * 9 + (dy + 1) * 3 + dz + 1, built with the same
+// wrap) and reads the array for farther cells.
+struct SolidArray {
+    const uint8_t* s;
+    AD_HD bool operator()(int c, int, int, int) const { return s[c] != 0; }
+};
+struct SolidNbrs {
+    uint32_t nb;
+    const uint8_t* s;
+    AD_HD bool operator()(int c, int dx, int dy, int dz) const {
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return s[c] != 0;
+        return (nb >> ((dx + 1) * 9 + (dy + 1) * 3 + dz + 1)) & 1u;
+    }
+};
+AD_HD SolidArray solid_query(const uint8_t* s) { return SolidArray{s}; }
+AD_HD SolidNbrs solid_query(SolidNbrs q) { return q; }
+
+
 AD_HD int cidx(const Params& P, int x, int y, int z) { return (x * P.ny + y) * P.nz + z; }
 
 // incoming population q of fluid cell (x, y, z) from the previous post-collision
 // field F (27 planes of `cells`): pull, halfway bounce-back at solid sources,
 // velocity inlet (equilibrium at the cell's inflow parameter) on x-, convective
 // outflow f1 + 0.15 (f1 - f2) on x+ (ref: jni.cpp:1494-1561)
-template <class R>
-AD_HD R incoming(const Params& P, const R* F, const uint8_t* solid, const R* uin, int x, int y, int z, int q) {
+template <class R, class Solid>
+AD_HD R incoming(const Params& P, const R* F, Solid solid_in, const R* uin, int x, int y, int z, int q) {
+    const auto solid = solid_query(solid_in);
     const int N = P.nx * P.ny * P.nz;
     const int cx = cxq(q);
     const int sx = x - cx, sy = wrap(y - cyq(q), P.ny), sz = wrap(z - czq(q), P.nz);
@@ -489,21 +511,22 @@ AD_HD R incoming(const Params& P, const R* F, const uint8_t* solid, const R* uin
     }
     if (sx >= P.nx) {
         const int x1 = x + cx, x2 = x + 2 * cx;
-        if (solid[cidx(P, x1, y, z)]) return F[(size_t)oppq(q) * N + c];
+        if (solid(cidx(P, x1, y, z), cx, 0, 0)) return F[(size_t)oppq(q) * N + c];
         const R f1 = F[(size_t)q * N + cidx(P, x1, y, z)];
-        if (solid[cidx(P, x2, y, z)]) return f1;
+        if (solid(cidx(P, x2, y, z), 2 * cx, 0, 0)) return f1;
         const R f2 = F[(size_t)q * N + cidx(P, x2, y, z)];
         return f1 + R(0.15) * (f1 - f2);
     }
     const int s = cidx(P, sx, sy, sz);
-    if (solid[s]) return F[(size_t)oppq(q) * N + c];
+    if (solid(s, -cx, -cyq(q), -czq(q))) return F[(size_t)oppq(q) * N + c];
     return F[(size_t)q * N + s];
 }
 
 // temperature at (x, y, z) for the stencil of a cell whose own value is self:
 // periodic in y and z; outside in x or solid -> self
-template <class R>
-AD_HD R t_at(const Params& P, const R* T, const uint8_t* solid, int x, int y, int z, R self, int* idx) {
+template <class R, class Solid>
+AD_HD R t_at(const Params& P, const R* T, const Solid& solid, int x, int y, int z, int dx, int dy, int dz, R self,
+             int* idx) {
     y = wrap(y, P.ny);
     z = wrap(z, P.nz);
     if (x < 0 || x >= P.nx) {
@@ -511,7 +534,7 @@ AD_HD R t_at(const Params& P, const R* T, const uint8_t* solid, int x, int y, in
         return self;
     }
     const int c = cidx(P, x, y, z);
-    if (solid[c]) {
+    if (solid(c, dx, dy, dz)) {
         *idx = -1;
         return self;
     }
@@ -522,9 +545,10 @@ AD_HD R t_at(const Params& P, const R* T, const uint8_t* solid, int x, int y, in
 // first-order semi-Lagrangian advection + FTCS diffusion + source, clamped to
 // +-1 (ref: jni.cpp:2145-2188). The 15 stencil indices (8 corners, 6 neighbours,
 // the cell) and weights go to idx/w (-1: the read fell back to the cell itself).
-template <class R>
-AD_HD R temperature(const Params& P, const R* T, const uint8_t* solid, R src, int x, int y, int z, R ux, R uy, R uz,
+template <class R, class SolidIn>
+AD_HD R temperature(const Params& P, const R* T, SolidIn solid_in, R src, int x, int y, int z, R ux, R uy, R uz,
                     int idx[14], R w_corner[8], R* pre_clamp) {
+    const auto solid = solid_query(solid_in);
     const int c = cidx(P, x, y, z);
     const R tc = T[c];
     const R px = R(float(x)) - ux, py = R(float(y)) - uy, pz = R(float(z)) - uz;
@@ -532,7 +556,8 @@ AD_HD R temperature(const Params& P, const R* T, const uint8_t* solid, R src, in
     const R fx = px - R(float(x0)), fy = py - R(float(y0)), fz = pz - R(float(z0));
     R cv[8];
     for (int k = 0; k < 8; ++k)
-        cv[k] = t_at(P, T, solid, x0 + (k >> 2), y0 + ((k >> 1) & 1), z0 + (k & 1), tc, &idx[k]);
+        cv[k] = t_at(P, T, solid, x0 + (k >> 2), y0 + ((k >> 1) & 1), z0 + (k & 1), x0 + (k >> 2) - x,
+                     y0 + ((k >> 1) & 1) - y, z0 + (k & 1) - z, tc, &idx[k]);
     // weights of corner k = (dx, dy, dz)
     for (int k = 0; k < 8; ++k) {
         const R wx = (k >> 2) ? fx : R(1) - fx;
@@ -544,7 +569,9 @@ AD_HD R temperature(const Params& P, const R* T, const uint8_t* solid, R src, in
     for (int k = 0; k < 8; ++k) adv = adv + w_corner[k] * cv[k];
     const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
     R sum = R(0);
-    for (int k = 0; k < 6; ++k) sum = sum + t_at(P, T, solid, x + nb[k][0], y + nb[k][1], z + nb[k][2], tc, &idx[8 + k]);
+    for (int k = 0; k < 6; ++k)
+        sum = sum + t_at(P, T, solid, x + nb[k][0], y + nb[k][1], z + nb[k][2], nb[k][0], nb[k][1], nb[k][2], tc,
+                         &idx[8 + k]);
     const R pre = adv + (R(P.alpha) * (sum - R(6) * tc) + src);
     *pre_clamp = pre;
     return rclamp(pre, -1.0f, 1.0f);
