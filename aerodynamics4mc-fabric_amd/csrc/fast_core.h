@@ -221,6 +221,27 @@ __device__ __forceinline__ int oob_face(const StepParams& P, int sx, int sy, int
     return 5;
 }
 
+// A face's rule kind and face-store offset for a per-lane face index, picked from
+// the parameter block's own copy of the six faces (wave-uniform values) instead of
+// an indexed load from the device face table: a rare cell's face rule then waits
+// on no memory load before it can load its inputs.
+__device__ __forceinline__ int face_kind(const StepParams& P, int face) {
+    int k = P.face[5].hydro_kind;
+    k = face == 4 ? P.face[4].hydro_kind : k;
+    k = face == 3 ? P.face[3].hydro_kind : k;
+    k = face == 2 ? P.face[2].hydro_kind : k;
+    k = face == 1 ? P.face[1].hydro_kind : k;
+    return face == 0 ? P.face[0].hydro_kind : k;
+}
+__device__ __forceinline__ long long face_store_off(const StepParams& P, int face) {
+    long long o = P.face[5].store_off;
+    o = face == 4 ? P.face[4].store_off : o;
+    o = face == 3 ? P.face[3].store_off : o;
+    o = face == 2 ? P.face[2].store_off : o;
+    o = face == 1 ? P.face[1].store_off : o;
+    return face == 0 ? P.face[0].store_off : o;
+}
+
 // Post-collision value (stored form) of direction q at in-domain cell (sx,sy,sz)
 // as the previous step left it: direct A-B read, or the A-A face store of `face`.
 template <class St, bool AA>
@@ -243,7 +264,7 @@ struct PrevPost {
         // faces without a store (and layers beyond the third) load nothing: the caller
         // computes every rule's candidates and selects afterwards
         const bool ok = ((P.face_store_faces >> face) & 1) && layer >= 0 && layer <= 2;
-        return ok ? store[P.faces_dev[face].store_off + ((size_t)(layer * 9 + j) * A + a) * B + b] : 0.0f;
+        return ok ? store[face_store_off(P, face) + ((size_t)(layer * 9 + j) * A + a) * B + b] : 0.0f;
     }
 };
 
@@ -261,8 +282,8 @@ __device__ __forceinline__ float face_link(const StepParams& P, const uint8_t* f
     const int sy = wrapc(y - cy, P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);
     const int sz = wrapc(z - cz, P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]);
     const int face = oob_face(P, sx, sy, sz);
-    const FaceParams& fc = P.faces_dev[face];
-    int kind = fc.hydro_kind;
+    const FaceParams& fc = P.faces_dev[face];  // velocity and pressure of the equilibrium faces
+    int kind = face_kind(P, face);
     if (kind == 0 || kind == 1) kind = P.convective_disabled && kind == 0 ? 2 : 6;
     if (kind == 2) return bounce;
     if (kind == 7) {  // symmetry: the mirrored in-domain source (ref: jni.cpp:1582-1587)
