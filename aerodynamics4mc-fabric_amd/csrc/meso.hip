@@ -71,6 +71,10 @@ struct Params {
     float amb_relax, deep_relax, surf_relax, wind_relax;
     float oh_e, oh_o, ot_e, ot_o;
     float vscale, max_speed;
+    // coefficient planes holding one value in every cell (bit k for plane k): the
+    // kernels take that value from here instead of reading the plane
+    uint32_t uniform;
+    float uval[kStats];
 };
 
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
@@ -88,6 +92,12 @@ __device__ __forceinline__ float tfeq(int q, float t, float ux, float uy, float 
 }
 
 __device__ __forceinline__ int idx(const Params& P, int x, int y, int z) { return (x * P.ny + y) * P.nz + z; }
+
+// coefficient plane k at cell i (k is a compile-time constant at every call, so the
+// uniform test is a wave-uniform branch on a kernel parameter)
+__device__ __forceinline__ float sv(const Params& P, const float* __restrict__ S, int k, size_t N, size_t i) {
+    return ((P.uniform >> k) & 1u) ? P.uval[k] : S[k * N + i];
+}
 
 // ---------------------------------------------------------------- forcing reduction
 __global__ void __launch_bounds__(256) prep_kernel(Params P, const float* __restrict__ F, float* __restrict__ S,
@@ -190,7 +200,7 @@ __device__ __forceinline__ Pulled pull(const Params& P, const float* __restrict_
         const int src = idx(P, sx, sy, sz);
         if ((links >> q) & 1u) {
             v.f = pf[opp(q) * N + i];
-            v.g = tfeq(q, S[kSurfT * N + src], 0.0f, 0.0f, 0.0f);
+            v.g = tfeq(q, sv(P, S, kSurfT, N, src), 0.0f, 0.0f, 0.0f);
             bounced = true;
         } else {
             v.f = pf[q * N + src];
@@ -198,7 +208,7 @@ __device__ __forceinline__ Pulled pull(const Params& P, const float* __restrict_
         }
     } else if (sy < 0) {
         v.f = pf[opp(q) * N + i];
-        v.g = tfeq(q, S[kSurfT * N + i], 0.0f, 0.0f, 0.0f);
+        v.g = tfeq(q, sv(P, S, kSurfT, N, i), 0.0f, 0.0f, 0.0f);
         bounced = true;
     } else {
         v.f = hfeq(q, 1.0f, bgx, bgy, bgz);
@@ -216,14 +226,14 @@ __device__ __forceinline__ Pulled pull(const Params& P, const float* __restrict_
 // the same rules for a cell at least two cells from the side and top faces and
 // above the floor: every source lies in the domain and no blending applies, so a
 // pull is the neighbour's value or, from a solid neighbour, the bounce
-__device__ __forceinline__ Pulled pull_interior(const float* __restrict__ pf, const float* __restrict__ pg,
+__device__ __forceinline__ Pulled pull_interior(const Params& P, const float* __restrict__ pf, const float* __restrict__ pg,
                                                 const float* __restrict__ S, uint32_t links, size_t N, int i, int q,
                                                 int sxy, int sy1) {
     const int src = i - (cx(q) * sxy + cy(q) * sy1 + cz(q));
     Pulled v;
     if ((links >> q) & 1u) {
         v.f = pf[opp(q) * N + i];
-        v.g = tfeq(q, S[kSurfT * N + src], 0.0f, 0.0f, 0.0f);
+        v.g = tfeq(q, sv(P, S, kSurfT, N, src), 0.0f, 0.0f, 0.0f);
     } else {
         v.f = pf[q * N + src];
         v.g = pg[q * N + src];
@@ -275,11 +285,11 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
     const uint32_t lk = links[i];
     if (lk & kSelfSolid) {
         // solid targets (mesoscale.cpp:448-457); their populations are never read
-        surf_out[i] = S[kSurfT * N + i];
-        deep_out[i] = S[kDeepT * N + i];
+        surf_out[i] = sv(P, S, kSurfT, N, i);
+        deep_out[i] = sv(P, S, kDeepT, N, i);
         return;
     }
-    const float bgx = S[kBx * N + i], bgy = S[kBy * N + i], bgz = S[kBz * N + i], amb_t = S[kAmbT * N + i];
+    const float bgx = sv(P, S, kBx, N, i), bgy = sv(P, S, kBy, N, i), bgz = sv(P, S, kBz, N, i), amb_t = sv(P, S, kAmbT, N, i);
     float f[kQ], g[kQ];
     if (local_first) {
 #pragma unroll
@@ -291,7 +301,7 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
         const int sy1 = P.nz, sxy = P.ny * P.nz;
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const Pulled v = pull_interior(pf_in, pg_in, S, lk, N, i, q, sxy, sy1);
+            const Pulled v = pull_interior(P, pf_in, pg_in, S, lk, N, i, q, sxy, sy1);
             f[q] = v.f;
             g[q] = v.g;
         }
@@ -318,8 +328,8 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
     ux /= rho;
     uy /= rho;
     uz /= rho;
-    const float deep_t = S[kDeepT * N + i], surf_t = S[kSurfT * N + i];
-    const float sx = S[kSlopeX * N + i], sz = S[kSlopeZ * N + i];
+    const float deep_t = sv(P, S, kDeepT, N, i), surf_t = sv(P, S, kSurfT, N, i);
+    const float sx = sv(P, S, kSlopeX, N, i), sz = sv(P, S, kSlopeZ, N, i);
     const float s_hl = hlap(P, surf_in, x, y, z);
     const float s_vl = vlap(P, surf_in, x, y, z);
     const float d_vl = vlap(P, deep_in, x, y, z);
@@ -327,23 +337,23 @@ __global__ void __launch_bounds__(256) collide_kernel(Params P, const float* __r
     const float a_vl = vlap(P, ambient, x, y, z);
     const float deep_c = deep_in[i], surf_c = surf_in[i];
     const float ndeep = deep_c + P.deep_relax * (deep_t - deep_c) + 0.01f * d_vl;
-    const float nsurf = surf_c + P.surf_relax * S[kDamp * N + i] * (surf_t - surf_c) + 0.10f * (ndeep - surf_c) +
-                        S[kN4 * N + i] + S[kN5 * N + i] + 0.05f * s_hl + 0.025f * s_vl;
+    const float nsurf = surf_c + P.surf_relax * sv(P, S, kDamp, N, i) * (surf_t - surf_c) + 0.10f * (ndeep - surf_c) +
+                        sv(P, S, kN4, N, i) + sv(P, S, kN5, N, i) + 0.05f * s_hl + 0.025f * s_vl;
     const float push = 0.0008f * (nsurf - amb);
     const float nsw = 1.0f - (float)y / smax(1.0f, (float)(P.ny - 1));
     const float lift = 0.0015f * (bgx * sx + bgz * sz) * nsw;
     const float blift = 0.0009f * (nsurf - amb) * nsw;
     const float steer_x = -0.005f * sx * bgz;
     const float steer_z = 0.005f * sz * bgx;
-    const float drag = S[kDrag * N + i];
+    const float drag = sv(P, S, kDrag, N, i);
     const float tux = clamp_speed(bgx + steer_x + push * sx - drag * ux);
     const float tuz = clamp_speed(bgz + steer_z + push * sz - drag * uz);
     const float tuy = clamp_speed(bgy + lift + blift - drag * uy);
     const float rux = clamp_speed((1.0f - P.wind_relax) * ux + P.wind_relax * tux);
     const float ruy = clamp_speed((1.0f - P.wind_relax) * uy + P.wind_relax * tuy);
     const float ruz = clamp_speed((1.0f - P.wind_relax) * uz + P.wind_relax * tuz);
-    const float src = P.amb_relax * (amb_t - amb) + S[kKh * N + i] * (nsurf - amb) + 0.01f * (ndeep - amb) +
-                      S[kT4 * N + i] + S[kT5 * N + i] + S[kT6 * N + i] + S[kT7 * N + i] + 0.01f * a_hl +
+    const float src = P.amb_relax * (amb_t - amb) + sv(P, S, kKh, N, i) * (nsurf - amb) + 0.01f * (ndeep - amb) +
+                      sv(P, S, kT4, N, i) + sv(P, S, kT5, N, i) + sv(P, S, kT6, N, i) + sv(P, S, kT7, N, i) + 0.01f * a_hl +
                       0.02f * a_vl;
     const float teq = amb + src;
     surf_out[i] = nsurf;
@@ -390,7 +400,7 @@ __global__ void __launch_bounds__(256) macro_kernel(Params P, const float* __res
     float amb, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     const uint32_t lk = links[i];
     if (lk & kSelfSolid) {
-        amb = S[kSurfT * N + i];
+        amb = sv(P, S, kSurfT, N, i);
     } else {
         const int z = i % P.nz, y = (i / P.nz) % P.ny, x = i / (P.nz * P.ny);
         const float ea = edge_alpha(P, x, y, z);
@@ -399,17 +409,17 @@ __global__ void __launch_bounds__(256) macro_kernel(Params P, const float* __res
         const bool band = min(min(min(x, z), min(P.nx - 1 - x, P.nz - 1 - z)), P.ny - 1 - y) <= 1;
         float bgx = 0.0f, bgy = 0.0f, bgz = 0.0f, amb_t = 0.0f;
         if (band) {
-            bgx = S[kBx * N + i];
-            bgy = S[kBy * N + i];
-            bgz = S[kBz * N + i];
-            amb_t = S[kAmbT * N + i];
+            bgx = sv(P, S, kBx, N, i);
+            bgy = sv(P, S, kBy, N, i);
+            bgz = sv(P, S, kBz, N, i);
+            amb_t = sv(P, S, kAmbT, N, i);
         }
         float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
         amb = 0.0f;
         if (!out && __all(interior(P, x, y, z))) {
             const int sy1 = P.nz, sxy = P.ny * P.nz;
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) amb += pull_interior(pf, pg, S, lk, N, i, q, sxy, sy1).g;
+            for (int q = 0; q < kQ; ++q) amb += pull_interior(P, pf, pg, S, lk, N, i, q, sxy, sy1).g;
         } else if (out) {
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {
@@ -456,7 +466,7 @@ __global__ void __launch_bounds__(256) stream_kernel(Params P, const float* __re
     const size_t N = (size_t)P.cells;
     const uint32_t lk = links[i];
     if (lk & kSelfSolid) {
-        const float wt = S[kSurfT * N + i];
+        const float wt = sv(P, S, kSurfT, N, i);
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
             f_out[q * N + i] = hfeq(q, 1.0f, 0.0f, 0.0f, 0.0f);
@@ -466,13 +476,37 @@ __global__ void __launch_bounds__(256) stream_kernel(Params P, const float* __re
     }
     const int z = i % P.nz, y = (i / P.nz) % P.ny, x = i / (P.nz * P.ny);
     const float ea = edge_alpha(P, x, y, z);
-    const float bgx = S[kBx * N + i], bgy = S[kBy * N + i], bgz = S[kBz * N + i], amb_t = S[kAmbT * N + i];
+    const float bgx = sv(P, S, kBx, N, i), bgy = sv(P, S, kBy, N, i), bgz = sv(P, S, kBz, N, i), amb_t = sv(P, S, kAmbT, N, i);
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         const Pulled v = pull(P, pf, pg, S, lk, i, x, y, z, q, ea, bgx, bgy, bgz, amb_t);
         f_out[q * N + i] = v.f;
         g_out[q * N + i] = v.g;
     }
+}
+
+// ---------------------------------------------------------------- uniform planes
+// u[k] = plane k's first value, u[kStats] = bit k set when plane k holds another value
+__global__ void uniform_init_kernel(Params P, const float* __restrict__ S, float* __restrict__ u) {
+    const int k = threadIdx.x;
+    if (k < kStats) u[k] = S[(size_t)k * P.cells];
+    if (k == kStats) reinterpret_cast<uint32_t*>(u)[kStats] = 0u;
+}
+__global__ void __launch_bounds__(256) uniform_kernel(Params P, const float* __restrict__ S, float* __restrict__ u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)P.cells;
+    uint32_t differs = 0;
+    if (i < P.cells) {
+#pragma unroll
+        for (int k = 0; k < kStats; ++k)
+            if (__float_as_uint(S[k * N + i]) != __float_as_uint(S[k * N])) differs |= 1u << k;
+    }
+    // one atomic per wavefront: OR of the lanes' masks, plane by plane
+    uint32_t wave = 0;
+#pragma unroll
+    for (int k = 0; k < kStats; ++k)
+        if (__ballot((differs >> k) & 1u)) wave |= 1u << k;
+    if (wave && (threadIdx.x & 63) == 0) atomicOr(reinterpret_cast<uint32_t*>(u) + kStats, wave);
 }
 
 // ---------------------------------------------------------------- host runtime
@@ -491,6 +525,9 @@ struct Context {
     float* surface[2] = {nullptr, nullptr};
     float* deep[2] = {nullptr, nullptr};
     float* stat = nullptr;
+    float* uni_dev = nullptr;  // uniform-plane values and flags (uniform_kernel)
+    uint32_t uniform = 0;      // planes holding one value, and that value
+    float uval[kStats] = {};
     uint8_t* solid = nullptr;
     uint32_t* links = nullptr;
     float* forcing_dev = nullptr;
@@ -516,10 +553,12 @@ struct Context {
                  alloc((void**)&surface[k], N * 4) && alloc((void**)&deep[k], N * 4);
         }
         ok = ok && alloc((void**)&ambient, N * 4) && alloc((void**)&stat, kStats * N * 4) &&
+             alloc((void**)&uni_dev, (kStats + 1) * 4) &&
              alloc((void**)&solid, N) && alloc((void**)&links, N * 4) && alloc((void**)&forcing_dev, kForcing * N * 4) &&
              alloc((void**)&out_dev, kState * N * 4);
         forcing_host.clear();
         prep_vscale = NAN;
+        uniform = 0;
         initialized = false;
         return ok;
     }
@@ -588,13 +627,15 @@ static Params make_params(const Context& c) {
     P.ot_e = 1.0f / tau_t;
     P.ot_o = 1.0f / odd_tau(tau_t);
     P.max_speed = 0.35f * P.vscale;
+    P.uniform = c.uniform;
+    std::memcpy(P.uval, c.uval, sizeof(P.uval));
     return P;
 }
 
 static dim3 grid1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 // upload the forcing (when it changed) and derive the coefficient planes; seed on first use
-static bool prepare(Context& c, const Params& P, const float* forcing) {
+static bool prepare(Context& c, Params& P, const float* forcing) {
     hipStream_t st = rt().stream;
     const size_t n = c.cells * kForcing;
     const bool changed = c.forcing_host.size() != n || std::memcmp(c.forcing_host.data(), forcing, n * 4) != 0;
@@ -618,6 +659,18 @@ static bool prepare(Context& c, const Params& P, const float* forcing) {
         hipLaunchKernelGGL(prep_kernel, grid1(c.cells), dim3(256), 0, st, P, c.forcing_dev, c.stat, c.solid);
         hipLaunchKernelGGL(mask_kernel, grid1(c.cells), dim3(256), 0, st, P, c.solid, c.links);
         c.prep_vscale = P.vscale;
+        hipLaunchKernelGGL(uniform_init_kernel, dim3(1), dim3(64), 0, st, P, c.stat, c.uni_dev);
+        hipLaunchKernelGGL(uniform_kernel, grid1(c.cells), dim3(256), 0, st, P, c.stat, c.uni_dev);
+        float u[kStats + 1];
+        if (!hip_ok(hipMemcpyAsync(u, c.uni_dev, sizeof(u), hipMemcpyDeviceToHost, st), "uniform planes") ||
+            !hip_ok(hipStreamSynchronize(st), "sync"))
+            return false;
+        uint32_t differs;
+        std::memcpy(&differs, &u[kStats], 4);
+        c.uniform = ~differs & ((1u << kStats) - 1u);
+        std::memcpy(c.uval, u, sizeof(c.uval));
+        P.uniform = c.uniform;
+        std::memcpy(P.uval, c.uval, sizeof(P.uval));
     }
     if (!c.initialized) {
         hipLaunchKernelGGL(seed_kernel, grid1(c.cells), dim3(256), 0, st, P, c.forcing_dev, c.pf[c.cur], c.pg[c.cur],
@@ -719,7 +772,7 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_step_context(long long key, const Ae
     }
     meso::Context* c = meso::bind(key, config);
     if (!c) return 0;
-    const meso::Params P = meso::make_params(*c);
+    meso::Params P = meso::make_params(*c);
     if (!meso::prepare(*c, P, forcing)) return 0;
     meso::launch_step(*c, P, true);
     hipStream_t st = rt().stream;
@@ -769,7 +822,7 @@ AERO_LBM_CAPI_EXPORT int aero_lbm_mesoscale_run_steps_timed(long long key, const
     }
     meso::Context* c = meso::bind(key, config);
     if (!c) return 0;
-    const meso::Params P = meso::make_params(*c);
+    meso::Params P = meso::make_params(*c);
     if (!meso::prepare(*c, P, forcing)) return 0;
     hipStream_t st = rt().stream;
     hipEvent_t e0, e1;

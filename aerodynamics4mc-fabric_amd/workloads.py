@@ -327,6 +327,19 @@ def meso_forcing(nx: int, ny: int, nz: int, wind_lattice: float = 0.04, angle_de
     return F.reshape(-1)
 
 
+def meso_forcing_varying(nx: int, ny: int, nz: int, seed: int = 11, base: np.ndarray | None = None) -> np.ndarray:
+    """meso_forcing (or `base`) with every channel the coefficient planes derive from
+    varied from cell to cell by a seeded uniform amount: temperatures and nested deltas
+    (up to +1 K / +0.5 K), roughness, humidity, heating, updrafts. No coefficient plane
+    is then uniform over the domain."""
+    F = (meso_forcing(nx, ny, nz) if base is None else base).reshape(nx, ny, nz, MESO_CHANNELS).astype(np.float64)
+    r = np.random.default_rng(seed).random(F.shape)
+    for ch, amp in ((2, 1.0), (3, 1.0), (4, 1.0), (5, 0.1), (9, 0.2), (10, 0.3), (11, 0.1), (16, 0.2),
+                    (17, 0.1), (18, 0.05), (19, 0.05), (22, 0.5), (23, 0.5)):
+        F[..., ch] += amp * r[..., ch]
+    return F.astype(np.float32).reshape(-1)
+
+
 def meso_config(nx: int, ny: int, nz: int, nu_lattice: float = 5e-4, dx: float = 64.0, dt: float = 5.0):
     """Mesoscale config whose molecular viscosity gives lattice viscosity `nu_lattice`."""
     cfg = A.MesoscaleConfig()

@@ -75,6 +75,25 @@ def test_meso_forcing_change_timed_run_and_state(oracle_mod, lib):
     ctx.release()
 
 
+def test_meso_uniform_and_varying_planes_bit_exact(oracle_mod, lib):
+    """Coefficient planes holding one value everywhere are passed to the kernels as
+    parameters; planes that vary are read per cell. Uniform forcing, fully varying
+    forcing and back: every step bit-identical to the FP32 oracle."""
+    dims = (18, 10, 14)
+    cfg = W.meso_config(*dims)
+    F1 = terrain_case(dims)
+    F2 = W.meso_forcing_varying(*dims, base=terrain_case(dims, humidity=0.2))
+    ctx = A.Mesoscale(cfg)
+    o = oracle_for(oracle_mod, cfg)
+    for k, F in enumerate((F1, F2, F1)):
+        log(f"meso plane case {k}")
+        for _ in range(3):
+            got = ctx.step(F)
+            ref = o.step(F)
+            assert np.array_equal(got, ref), (k, float(np.max(np.abs(got - ref))))
+    ctx.release()
+
+
 def test_meso_tau_floor_override(oracle_mod, lib):
     """Lattice viscosity 5e-4 (tau = 0.5015) with the tau floor lowered to 0.5015."""
     dims = (16, 10, 16)
