@@ -100,38 +100,70 @@ __device__ __forceinline__ float sv(const Params& P, const float* __restrict__ S
 }
 
 // ---------------------------------------------------------------- forcing reduction
-__global__ void __launch_bounds__(256) prep_kernel(Params P, const float* __restrict__ F, float* __restrict__ S,
-                                                   uint8_t* __restrict__ solid) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.cells) return;
-    const size_t N = (size_t)P.cells;
+// the 17 coefficient plane values of cell i (ref: mesoscale.cpp:372-391 for the slope)
+__device__ __forceinline__ void plane_values(const Params& P, const float* __restrict__ F, int i, float v[kStats]) {
     const int z = i % P.nz, y = (i / P.nz) % P.ny, x = i / (P.nz * P.ny);
     const float* p = F + (size_t)i * kForcing;
-    solid[i] = p[24] > 0.5f ? 1 : 0;
-    S[kAmbT * N + i] = p[2] + p[22];
-    S[kDeepT * N + i] = p[3];
-    S[kSurfT * N + i] = p[4] + p[23];
+    v[kAmbT] = p[2] + p[22];
+    v[kDeepT] = p[3];
+    v[kSurfT] = p[4] + p[23];
     const float rough = smax(0.0f, p[5]);
-    S[kDrag * N + i] = clampf(rough * 0.02f, 0.0f, 0.25f);
+    v[kDrag] = clampf(rough * 0.02f, 0.0f, 0.25f);
     const float hum = clampf(p[9] + p[11] + p[17], 0.0f, 1.0f);
-    S[kDamp * N + i] = 1.0f - 0.12f * hum;
-    S[kKh * N + i] = 0.018f + 0.004f * hum;
-    S[kT4 * N + i] = 0.08f * p[10] * (0.5f + 0.5f * hum);
-    S[kT5 * N + i] = 0.10f * p[16] * (0.45f + 0.55f * hum);
-    S[kT6 * N + i] = 0.06f * p[18];
-    S[kT7 * N + i] = 0.06f * p[19];
-    S[kN4 * N + i] = 0.04f * p[10];
-    S[kN5 * N + i] = 0.08f * p[16];
-    // terrain slope from the clamped neighbours' heights (mesoscale.cpp:372-391)
+    v[kDamp] = 1.0f - 0.12f * hum;
+    v[kKh] = 0.018f + 0.004f * hum;
+    v[kT4] = 0.08f * p[10] * (0.5f + 0.5f * hum);
+    v[kT5] = 0.10f * p[16] * (0.45f + 0.55f * hum);
+    v[kT6] = 0.06f * p[18];
+    v[kT7] = 0.06f * p[19];
+    v[kN4] = 0.04f * p[10];
+    v[kN5] = 0.08f * p[16];
+    // terrain slope from the clamped neighbours' heights
     const int xp = min(P.nx - 1, x + 1), xm = max(0, x - 1), zp = min(P.nz - 1, z + 1), zm = max(0, z - 1);
-    S[kSlopeX * N + i] = 0.5f * (F[(size_t)idx(P, xp, y, z) * kForcing] - F[(size_t)idx(P, xm, y, z) * kForcing]);
-    S[kSlopeZ * N + i] = 0.5f * (F[(size_t)idx(P, x, y, zp) * kForcing] - F[(size_t)idx(P, x, y, zm) * kForcing]);
+    v[kSlopeX] = 0.5f * (F[(size_t)idx(P, xp, y, z) * kForcing] - F[(size_t)idx(P, xm, y, z) * kForcing]);
+    v[kSlopeZ] = 0.5f * (F[(size_t)idx(P, x, y, zp) * kForcing] - F[(size_t)idx(P, x, y, zm) * kForcing]);
     const float bx = p[6] + p[20] + p[12] + p[14];
     const float bz = p[7] + p[21] + p[13] + p[15];
     const float by = p[19] + p[18];
-    S[kBx * N + i] = clamp_speed(bx / P.vden);
-    S[kBy * N + i] = clamp_speed(by / P.vden);
-    S[kBz * N + i] = clamp_speed(bz / P.vden);
+    v[kBx] = clamp_speed(bx / P.vden);
+    v[kBy] = clamp_speed(by / P.vden);
+    v[kBz] = clamp_speed(bz / P.vden);
+}
+
+// uni[kStats] (as bits) cleared before prep_kernel ORs into it
+__global__ void uniform_clear_kernel(float* __restrict__ uni) { reinterpret_cast<uint32_t*>(uni)[kStats] = 0u; }
+
+// Coefficient planes and solid flags from the forcing. Each thread also derives
+// cell 0's values (the same arithmetic on the same inputs, so bitwise equal where
+// the planes are uniform) and flags the planes where its cell differs: uni[k] =
+// cell 0's value of plane k, bit k of uni[kStats] = plane k is not uniform. One
+// atomic per wavefront, skipped once the bits it would set are visible.
+__global__ void __launch_bounds__(256) prep_kernel(Params P, const float* __restrict__ F, float* __restrict__ S,
+                                                   uint8_t* __restrict__ solid, float* __restrict__ uni) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)P.cells;
+    uint32_t differs = 0;
+    if (i < P.cells) {
+        solid[i] = F[(size_t)i * kForcing + 24] > 0.5f ? 1 : 0;
+        float v[kStats], r[kStats];
+        plane_values(P, F, i, v);
+        plane_values(P, F, 0, r);
+#pragma unroll
+        for (int k = 0; k < kStats; ++k) {
+            S[k * N + i] = v[k];
+            if (__float_as_uint(v[k]) != __float_as_uint(r[k])) differs |= 1u << k;
+            if (i == 0) uni[k] = v[k];
+        }
+    }
+    uint32_t wave = 0;
+#pragma unroll
+    for (int k = 0; k < kStats; ++k)
+        if (__ballot((differs >> k) & 1u)) wave |= 1u << k;
+    if (wave && (threadIdx.x & 63) == 0) {
+        uint32_t* flags = reinterpret_cast<uint32_t*>(uni) + kStats;
+        const uint32_t seen = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (wave & ~seen) atomicOr(flags, wave);
+    }
 }
 
 // per-cell link mask: bit q (1..18) when direction q's source cell lies in the
@@ -485,30 +517,6 @@ __global__ void __launch_bounds__(256) stream_kernel(Params P, const float* __re
     }
 }
 
-// ---------------------------------------------------------------- uniform planes
-// u[k] = plane k's first value, u[kStats] = bit k set when plane k holds another value
-__global__ void uniform_init_kernel(Params P, const float* __restrict__ S, float* __restrict__ u) {
-    const int k = threadIdx.x;
-    if (k < kStats) u[k] = S[(size_t)k * P.cells];
-    if (k == kStats) reinterpret_cast<uint32_t*>(u)[kStats] = 0u;
-}
-__global__ void __launch_bounds__(256) uniform_kernel(Params P, const float* __restrict__ S, float* __restrict__ u) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t N = (size_t)P.cells;
-    uint32_t differs = 0;
-    if (i < P.cells) {
-#pragma unroll
-        for (int k = 0; k < kStats; ++k)
-            if (__float_as_uint(S[k * N + i]) != __float_as_uint(S[k * N])) differs |= 1u << k;
-    }
-    // one atomic per wavefront: OR of the lanes' masks, plane by plane
-    uint32_t wave = 0;
-#pragma unroll
-    for (int k = 0; k < kStats; ++k)
-        if (__ballot((differs >> k) & 1u)) wave |= 1u << k;
-    if (wave && (threadIdx.x & 63) == 0) atomicOr(reinterpret_cast<uint32_t*>(u) + kStats, wave);
-}
-
 // ---------------------------------------------------------------- host runtime
 struct Context {
     AeroLbmMesoscaleConfig cfg{};
@@ -525,7 +533,7 @@ struct Context {
     float* surface[2] = {nullptr, nullptr};
     float* deep[2] = {nullptr, nullptr};
     float* stat = nullptr;
-    float* uni_dev = nullptr;  // uniform-plane values and flags (uniform_kernel)
+    float* uni_dev = nullptr;  // uniform-plane values and flags (prep_kernel)
     uint32_t uniform = 0;      // planes holding one value, and that value
     float uval[kStats] = {};
     uint8_t* solid = nullptr;
@@ -656,11 +664,10 @@ static bool prepare(Context& c, Params& P, const float* forcing) {
             return false;
     }
     if (replan) {
-        hipLaunchKernelGGL(prep_kernel, grid1(c.cells), dim3(256), 0, st, P, c.forcing_dev, c.stat, c.solid);
+        hipLaunchKernelGGL(uniform_clear_kernel, dim3(1), dim3(1), 0, st, c.uni_dev);
+        hipLaunchKernelGGL(prep_kernel, grid1(c.cells), dim3(256), 0, st, P, c.forcing_dev, c.stat, c.solid, c.uni_dev);
         hipLaunchKernelGGL(mask_kernel, grid1(c.cells), dim3(256), 0, st, P, c.solid, c.links);
         c.prep_vscale = P.vscale;
-        hipLaunchKernelGGL(uniform_init_kernel, dim3(1), dim3(64), 0, st, P, c.stat, c.uni_dev);
-        hipLaunchKernelGGL(uniform_kernel, grid1(c.cells), dim3(256), 0, st, P, c.stat, c.uni_dev);
         float u[kStats + 1];
         if (!hip_ok(hipMemcpyAsync(u, c.uni_dev, sizeof(u), hipMemcpyDeviceToHost, st), "uniform planes") ||
             !hip_ok(hipStreamSynchronize(st), "sync"))
