@@ -347,7 +347,6 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
                 continue;
             }
             const int sa = plane_row<T>(Ni, qa, R.ok(dx, dy) ? R.off(dx, dy) : R.row);
-            const int bounce = (int)(links << (31 - q)) >> 31;
             int so, va, vb;  // scalar offset; vector offsets of the streamed and the bounced value
             if (qa < qb) {
                 so = sa;
@@ -358,7 +357,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
                 va = zb[dz + 1] + (sa - sb);
                 vb = zb[1];
             }
-            g[q] = St::template bld<kPolDefault>(rin, (bounce & vb) | (~bounce & va), so, q);
+            g[q] = St::template bld<kPolDefault>(rin, pick_by_bit(links, q, va, vb), so, q);
         }
     }
     // x face planes flagged xface_bulk (inlet, outflow, symmetry, Dirichlet / Neumann
@@ -407,7 +406,6 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
                 continue;
             }
             const int sn = plane_row<T>(Ni, q, R2.off(dx, dy));
-            const int bounce = (int)(links << (31 - lat_opp(q))) >> 31;
             int so, vn, vb;
             if (q <= 13) {  // plane q lies below plane opp(q); a row offset is below the plane stride
                 so = sn;
@@ -418,7 +416,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
                 vn = zb[dz + 1] + (sn - sb);
                 vb = zb[1];
             }
-            St::template bst<StPol>(rout, (bounce & vb) | (~bounce & vn), so, g[q], q);
+            St::template bst<StPol>(rout, pick_by_bit(links, lat_opp(q), vn, vb), so, g[q], q);
         }
     }
 }

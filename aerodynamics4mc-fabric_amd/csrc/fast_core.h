@@ -159,6 +159,16 @@ __device__ __forceinline__ void bst_f32(Rsrc r, int off, float v) {
 // all ones where bit b of v is set, else zero
 __device__ __forceinline__ int bit_mask(uint32_t v, int b) { return (int)(v << (31 - b)) >> 31; }
 
+// Per-lane pick of b where bit `bit` of v is set, else a: one bitfield extract to a
+// 0 / all-ones mask and one three-input bitwise operation. The extract is opaque to
+// the compiler, which would otherwise rewrite the pair as a compare, a select and a
+// wait on VCC. `bit` must be a constant once the caller's loop is unrolled.
+__device__ __forceinline__ int pick_by_bit(uint32_t v, int bit, int a, int b) {
+    int m;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"(v), "i"(bit));
+    return (m & b) | (~m & a);
+}
+
 __device__ constexpr float wq(int q) {
     return (lat_cx(q) * lat_cx(q) + lat_cy(q) * lat_cy(q) + lat_cz(q) * lat_cz(q)) == 0   ? 8.0f / 27.0f
            : (lat_cx(q) * lat_cx(q) + lat_cy(q) * lat_cy(q) + lat_cz(q) * lat_cz(q)) == 1 ? 2.0f / 27.0f
