@@ -188,6 +188,9 @@ bool build_tables(Batch& b) {
     g.gather_blocks = (int)gather.size();
     g.update_blocks = (int)update.size();
     g.publish_blocks = (int)publish.size();
+    g.yface = 0;
+    for (const BatchBrick& e : host)
+        if (e.P.yface_bulk[0] || e.P.yface_bulk[1]) g.yface = 1;
     b.grids.assign(K, g);
     return true;
 }

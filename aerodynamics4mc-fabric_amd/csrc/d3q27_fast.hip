@@ -93,11 +93,21 @@ __device__ __forceinline__ TileCoord tile_cell(const StepParams& P, int L) {
     return t;
 }
 
-// The tile L lies in an x face plane flagged xface_bulk: its workgroup takes the
-// instantiation of the step body with the face rules, every other workgroup the
-// one without them (separate register allocation for the two paths).
+// The tile L lies in an x face plane flagged xface_bulk (xface_tile), or holds the
+// first or last row of y of a y face flagged yface_bulk (yface_tile): its workgroup
+// takes the instantiation of the step body with the face rules, every other
+// workgroup the one without them. Kernels launched for contexts with such y faces
+// are separate instantiations (template YF), so the others keep their code.
 __device__ __forceinline__ bool xface_tile(const StepParams& P, int L) {
     return (L >> 3) < tile_layout(P).fper;
+}
+__device__ __forceinline__ bool yface_tile(const StepParams& P, int L) {
+    if (!(P.yface_bulk[0] | P.yface_bulk[1])) return false;
+    const TileLayout T = tile_layout(P);
+    const int nbz = (P.nz + kTileZ - 1) / kTileZ, nby = (P.ny + kTileY - 1) / kTileY;
+    const int b = (L & 7) * T.bper + ((L >> 3) - T.fper);
+    const int by = (b % T.nbp) / nbz;
+    return (P.yface_bulk[0] && by == 0) || (P.yface_bulk[1] && by == nby - 1);
 }
 
 // Wave-uniform neighbour rows of the wave's (x, y) row: element offset of row
@@ -274,7 +284,7 @@ __device__ __forceinline__ void face_store(const StepParams& P, int x, int y, in
 // L: the workgroup's tile index within the brick. Shell: the context has a
 // boundary-reference shell (a separate instantiation, so contexts without one keep
 // the shorter kernel and its register budget).
-template <class St, class Uv, int Mode, int StPol, bool Shell, bool XF, bool GravZ = false>
+template <class St, class Uv, int Mode, int StPol, bool Shell, bool XF, bool GravZ = false, bool YF = false>
 __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields& C, const uint32_t* __restrict__ mask,
                                           const typename St::T* __restrict__ fin, typename St::T* fout,
                                           const float* __restrict__ t_in, float* __restrict__ t_out,
@@ -305,6 +315,7 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     const Rsrc rin = make_rsrc(src, 27 * Ni * (int)sizeof(T));
     const Rsrc rout = make_rsrc(fout, 27 * Ni * (int)sizeof(T));
     const bool xlo = XF && x == 0 && P.xface_bulk[0], xhi = XF && x == P.nx - 1 && P.xface_bulk[1];
+    const bool ylo = YF && XF && y == 0 && P.yface_bulk[0], yhi = YF && XF && y == P.ny - 1 && P.yface_bulk[1];
     // temperature first: its stencil loads and arithmetic finish before the 27
     // population loads, which keeps the two sets of values from being live together
     float temp = 0.0f;
@@ -372,6 +383,12 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
             if (xlo) xface_links<0, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
             if (xhi) xface_links<1, St, Mode != 0>(Q, C.flags, rp, prev, x, y, z, g);
         }
+        if (YF && XF && !local_first && (ylo || yhi)) {
+            const StepParams& Q = *P.self_dev;
+            const PrevPost<St, Mode != 0> prev{fin, store_in, (size_t)Q.plane};
+            if (ylo) yface_links<0, St, Mode != 0>(Q, C.flags, prev, x, y, z, g);
+            if (yhi) yface_links<1, St, Mode != 0>(Q, C.flags, prev, x, y, z, g);
+        }
         CollideResult r = collide<St::kShifted, GravZ>(P, g, temp);
         if (!r.ok) reseed<St::kShifted>(*P.self_dev, C, cell, g, r);
         if (Shell && P.shell_on) {
@@ -421,16 +438,16 @@ __device__ __forceinline__ void bulk_body(const StepParams& P, const CellFields&
     }
 }
 
-template <class St, class Uv, int Mode, int StPol, bool Shell, bool GravZ>
+template <class St, class Uv, int Mode, int StPol, bool Shell, bool GravZ, bool YF>
 __global__ void __launch_bounds__(256) step_kernel(StepParams P, CellFields C, const uint32_t* __restrict__ mask,
                                                    const typename St::T* __restrict__ fin, typename St::T* fout,
                                                    const float* __restrict__ t_in, float* __restrict__ t_out,
                                                    typename Uv::T* __restrict__ u_state,
                                                    const float* __restrict__ store_in, float* __restrict__ store_out,
                                                    int local_first) {
-    if (xface_tile(P, blockIdx.x))
-        bulk_body<St, Uv, Mode, StPol, Shell, true, GravZ>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
-                                                           store_out, local_first, blockIdx.x);
+    if (xface_tile(P, blockIdx.x) || (YF && yface_tile(P, blockIdx.x)))
+        bulk_body<St, Uv, Mode, StPol, Shell, true, GravZ, YF>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
+                                                               store_out, local_first, blockIdx.x);
     else
         bulk_body<St, Uv, Mode, StPol, Shell, false, GravZ>(P, C, mask, fin, fout, t_in, t_out, u_state, store_in,
                                                             store_out, local_first, blockIdx.x);
@@ -663,16 +680,17 @@ __global__ void __launch_bounds__(256) readout_kernel(StepParams P, CellFields C
 // At least 5 waves per SIMD (at most 96 VGPRs): the odd (pull + push)
 // instantiation once needed ~100 VGPRs; with one store per direction it fits in 80
 // (6 waves) and the even one in 72 (7 waves), neither spilling.
-template <class St, class Uv, int Mode, int StPol, bool Shell>
+template <class St, class Uv, int Mode, int StPol, bool Shell, bool YF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) step_kernel_batch(const BatchBrick* __restrict__ table,
                                                          const uint16_t* __restrict__ owner) {
     using T = typename St::T;
     const int b = __builtin_amdgcn_readfirstlane((int)owner[blockIdx.x]);
     const BatchBrick& B = table[b];
     const int L = (int)blockIdx.x - B.bulk_begin;
-    if (xface_tile(B.P, L))
-        bulk_body<St, Uv, Mode, StPol, Shell, true>(B.P, B.C, B.mask, (const T*)B.f, (T*)B.f, B.t_in, B.t_out,
-                                                    (typename Uv::T*)B.u_state, B.store_in, B.store_out, 0, L);
+    if (xface_tile(B.P, L) || (YF && yface_tile(B.P, L)))
+        bulk_body<St, Uv, Mode, StPol, Shell, true, false, YF>(B.P, B.C, B.mask, (const T*)B.f, (T*)B.f, B.t_in,
+                                                               B.t_out, (typename Uv::T*)B.u_state, B.store_in,
+                                                               B.store_out, 0, L);
     else
         bulk_body<St, Uv, Mode, StPol, Shell, false>(B.P, B.C, B.mask, (const T*)B.f, (T*)B.f, B.t_in, B.t_out,
                                                      (typename Uv::T*)B.u_state, B.store_in, B.store_out, 0, L);
@@ -865,7 +883,9 @@ __global__ void __launch_bounds__(256) mask_kernel(StepParams P, const uint8_t* 
             const bool therm = fc.thermal_kind == 4 || fc.thermal_kind == 5;
             if (therm) m |= kMaskTherm;
             // links leaving through an x face flagged xface_bulk are the bulk kernel's
-            if ((!plain_bounce || therm) && !(face < 2 && P.xface_bulk[face])) special_elsewhere = true;
+            // and so are the symmetry links leaving through a y face flagged yface_bulk
+            const bool bulk_face = (face < 2 && P.xface_bulk[face]) || (face >= 2 && face < 4 && P.yface_bulk[face - 2]);
+            if ((!plain_bounce || therm) && !bulk_face) special_elsewhere = true;
         } else if (flags[(sx * P.ny + sy) * P.nz + sz] & kCellSolid) {
             m |= 1u << q;
         }
@@ -1001,15 +1021,21 @@ static void launch_bulk(const StepParams& P, const CellFields& C, const uint32_t
     const dim3 grid((unsigned)fast::tile_grid(P)), block(fast::kTileZ, fast::kTileY);
     // gravity along z (or no buoyancy): the instantiation without the x/y force branch
     const bool grav_z = !P.buoyancy_on || P.g_axis == 2;
-    if (P.shell_on)
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, true, false>), grid, block, 0, s, P, C, mask,
-                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
-    else if (grav_z)
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, false, true>), grid, block, 0, s, P, C, mask,
-                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
-    else
-        hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, false, false>), grid, block, 0, s, P, C, mask,
-                           (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf);
+    const bool yf = P.yface_bulk[0] || P.yface_bulk[1];
+#define AERO_STEP(SH, GZ, YF)                                                                                      \
+    hipLaunchKernelGGL((fast::step_kernel<St, Uv, Mode, StPol, SH, GZ, YF>), grid, block, 0, s, P, C, mask,      \
+                       (const T*)fin, (T*)fout, t_in, t_out, (U*)u_state, store_in, store_out, lf)
+    if (P.shell_on) {
+        if (yf) AERO_STEP(true, false, true);
+        else AERO_STEP(true, false, false);
+    } else if (grav_z) {
+        if (yf) AERO_STEP(false, true, true);
+        else AERO_STEP(false, true, false);
+    } else {
+        if (yf) AERO_STEP(false, false, true);
+        else AERO_STEP(false, false, false);
+    }
+#undef AERO_STEP
 }
 
 template <class St, class Uv, int Mode, int StPol>
@@ -1105,13 +1131,15 @@ static void batch_step_t(bool shell, const BatchBrick* table, const BatchGrids& 
         hipLaunchKernelGGL((fast::rare_update_batch<St, Uv, Mode>), dim3((unsigned)g.update_blocks), dim3(128), 0, s,
                            table, g.update_owner);
     }
-    const dim3 block(fast::kTileZ, fast::kTileY);
-    if (shell)
-        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, true>), dim3((unsigned)g.bulk_blocks), block,
-                           0, s, table, g.bulk_owner);
+    const dim3 block(fast::kTileZ, fast::kTileY), grid((unsigned)g.bulk_blocks);
+    if (shell && g.yface)
+        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, true, true>), grid, block, 0, s, table, g.bulk_owner);
+    else if (shell)
+        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, true, false>), grid, block, 0, s, table, g.bulk_owner);
+    else if (g.yface)
+        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, false, true>), grid, block, 0, s, table, g.bulk_owner);
     else
-        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, false>), dim3((unsigned)g.bulk_blocks), block,
-                           0, s, table, g.bulk_owner);
+        hipLaunchKernelGGL((fast::step_kernel_batch<St, Uv, Mode, StPol, false, false>), grid, block, 0, s, table, g.bulk_owner);
 }
 
 template <class St, class Uv>

@@ -399,6 +399,30 @@ __device__ __forceinline__ void xface_links(const StepParams& P, const uint8_t* 
     }
 }
 
+// The 9 links of a cell in the y_min (Side 0, c_y = +1) or y_max (Side 1, c_y = -1)
+// row of a symmetry face flagged yface_bulk, for the bulk kernel: the rule of
+// face_link for kind 7 (the mirrored in-domain source of the previous step, or
+// bounce-back from a solid one). Links whose source lies outside in x take the x
+// face's rule (the reference's face order x, y, z) and are left alone; x is
+// wave-uniform, so that test is a wave-uniform branch. g[q] holds the bounce-back
+// candidate on entry.
+template <int Side, class St, bool AA>
+__device__ __forceinline__ void yface_links(const StepParams& P, const uint8_t* __restrict__ flags,
+                                            const PrevPost<St, AA>& prev, int x, int y, int z, float g[27]) {
+    constexpr int iy = Side == 0 ? 2 : 0;
+    const int ys = Side == 0 ? 0 : P.ny - 1;  // the mirrored source row
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const int q = (j / 3) * 9 + iy * 3 + j % 3;
+        const int sx = wrapc(x - lat_cx(q), P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
+        if (sx < 0 || sx >= P.nx) continue;
+        const int iz = icl(wrapc(z - lat_cz(q), P.nz, P.hydro_wrap_lo[2], P.hydro_wrap_hi[2]), 0, P.nz - 1);
+        const bool s0 = flags[(sx * P.ny + ys) * P.nz + iz] & kCellSolid;
+        const float p0 = prev.get(P, 2 + Side, q, sx, ys, iz);
+        g[q] = s0 ? g[q] : p0;
+    }
+}
+
 __device__ __forceinline__ bool link_oob(const StepParams& P, int x, int y, int z, int q) {
     const int sx = wrapc(x - lat_cx(q), P.nx, P.hydro_wrap_lo[0], P.hydro_wrap_hi[0]);
     const int sy = wrapc(y - lat_cy(q), P.ny, P.hydro_wrap_lo[1], P.hydro_wrap_hi[1]);

@@ -17,7 +17,8 @@ constexpr uint32_t kMaskFace = 1u << 28;   // a link leaves through a face whose
 constexpr uint32_t kMaskTherm = 1u << 29;  // a stencil neighbour is outside the domain at a Dirichlet/Neumann thermal face
 // the cell is updated by the rare-cell kernel: a face rule other than plain
 // bounce-back or a Dirichlet / Neumann thermal face applies (except through the
-// x faces flagged StepParams::xface_bulk, whose planes the bulk kernel updates),
+// x faces flagged StepParams::xface_bulk and the symmetry y faces flagged
+// StepParams::yface_bulk, whose planes and rows the bulk kernel updates),
 // the sponge touches it, or the thermal periodicity differs from the hydrodynamic one
 constexpr uint32_t kMaskRare = 1u << 30;
 
@@ -98,6 +99,7 @@ struct BatchGrids {
     const uint16_t* update_owner;
     const uint16_t* publish_owner;
     int bulk_blocks, gather_blocks, update_blocks, publish_blocks;
+    int yface = 1;  // some brick has a y face flagged yface_bulk: launch the kernels with the y rules
 };
 void launch_batch_step(FastStorage st, int mode, int store_policy, bool shell, const BatchBrick* table,
                        const BatchGrids& g, hipStream_t s);

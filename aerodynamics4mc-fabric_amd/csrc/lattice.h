@@ -91,6 +91,10 @@ struct StepParams {
     // bounce-back, or its thermal rule is Dirichlet / Neumann). A plane of x is a
     // set of whole wavefront rows, so the rules run as a wave-uniform branch.
     int xface_bulk[2];
+    // y_min / y_max: a symmetry face (not periodic, no Dirichlet / Neumann thermal
+    // rule) whose rows run the symmetry rule in the bulk step kernel; a row of
+    // constant y is a whole wavefront, as for the x planes
+    int yface_bulk[2];
     // temperature by BFECC advection (heated-cavity presets): three passes before
     // the step leave the new temperature in t_out, which the exact step kernel reads
     int thermal_bfecc;

@@ -380,6 +380,13 @@ bool resolve_params(int nx, int ny, int nz, int in_ch, StepParams& P) {
         const bool therm = tk == AERO_LBM_THERMAL_BOUNDARY_TEMPERATURE_DIRICHLET || tk == AERO_LBM_THERMAL_BOUNDARY_HEAT_FLUX_NEUMANN;
         P.xface_bulk[side] = !wraps && (!plain_bounce || therm) ? 1 : 0;
     }
+    for (int side = 0; side < 2; ++side) {
+        const bool wraps = side ? P.hydro_wrap_hi[1] : P.hydro_wrap_lo[1];
+        const int tk = P.face[2 + side].thermal_kind;
+        const bool therm = tk == AERO_LBM_THERMAL_BOUNDARY_TEMPERATURE_DIRICHLET || tk == AERO_LBM_THERMAL_BOUNDARY_HEAT_FLUX_NEUMANN;
+        P.yface_bulk[side] =
+            !wraps && !therm && P.face[2 + side].hydro_kind == AERO_LBM_HYDRO_BOUNDARY_SYMMETRY ? 1 : 0;
+    }
     return true;
 }
 
